@@ -1,0 +1,170 @@
+"""Benchmark: P2-P1 Navier-Stokes Jacobian assembly throughput on one MI355X.
+
+Workload: FEDDLib's FE::assemblyNavierStokes(..., reAssemble=false, "Jacobian") --
+all blocks A(0,0), B^T(0,1), B(1,0) (C(1,1) is empty) -- on the unit cube with
+40^3 x 6 = 384,000 tets (P2 velocity / P1 pressure), nu = 0.01, rho = 1,
+Linearization "Newton", velocity iterate = 8 seeded sine modes of amplitude 1.
+A "step" is one such assembly with the inputs resident in HBM.
+
+Prints ONE JSON line (rank 0).  Usage:
+  python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--M 40] [--cpu-sample 50000]
+"""
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+NU, RHO = 0.01, 1.0
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--M", type=int, default=40)
+    ap.add_argument("--cpu-sample", type=int, default=50000,
+                    help="elements of the CPU-baseline sample (0 disables it)")
+    ap.add_argument("--no-check", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every box here has a single MI355X and this build "
+              "has no multi-GPU path (see DESIGN.md, 'How it would shard').", file=sys.stderr)
+        sys.exit(2)
+
+    from feddlib_amd import native as N
+    from feddlib_amd import workloads as W
+
+    M = args.M
+    t0 = time.time()
+    p2, e2, p1, e1, u, p = W.ns_problem(M, 0)
+    E = e2.shape[0]
+    ctx = N.Context(0)
+    ctx.mesh_upload(N.MESH_PRIMARY, p2, e2)
+    ctx.mesh_upload(N.MESH_PRESSURE, p1, e1)
+    ctx.graph_build(N.GRAPH_NS)
+    ctx.vector_upload(N.VEC_U, u)
+    ctx.vector_upload(N.VEC_P, p)
+    setup_s = time.time() - t0
+    nnzA = ctx.block_info(N.BLOCK_A)[2]
+    nnzBT = ctx.block_info(N.BLOCK_BT)[2]
+    nnzB = ctx.block_info(N.BLOCK_B)[2]
+    print("# setup %.1fs  E=%d  nnz A=%d BT=%d B=%d" % (setup_s, E, nnzA, nnzBT, nnzB), file=sys.stderr,
+          flush=True)
+
+    def step(precision):
+        ctx.assembly_navier_stokes(NU, RHO, True, N.NS_JACOBIAN, reassemble=False, precision=precision)
+
+    def timed(precision, steps, warmup):
+        for _ in range(warmup):
+            step(precision)
+        ctx.synchronize()
+        ctx.kernel_timing(True)
+        ctx.event_record(0)
+        for _ in range(steps):
+            step(precision)
+        ctx.event_record(1)
+        ctx.synchronize()
+        ms = ctx.event_elapsed(0, 1)
+        kv = ctx.kernel_time("ns_vel_rows")
+        kp = ctx.kernel_time("ns_pres_rows")
+        ctx.kernel_timing(False)
+        return ms / steps, kv, kp
+
+    ms64, kv64, kp64 = timed(N.FP64, args.steps, args.warmup)
+    vals64 = ctx.block_values(N.BLOCK_A) if not args.no_check else None
+    ms32, kv32, kp32 = timed(N.FP32_LOCAL, args.steps, args.warmup)
+    fp32_err = None
+    if vals64 is not None:
+        v32 = ctx.block_values(N.BLOCK_A)
+        rp, _ = ctx.block_graph(N.BLOCK_A)
+        rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+        rowmax = np.zeros(len(rp) - 1)
+        np.maximum.at(rowmax, rows, np.abs(vals64))
+        den = np.maximum(np.abs(vals64), 1e-3 * rowmax[rows])
+        fp32_err = float(np.max(np.abs(v32 - vals64) / den))
+        del v32, rows, rowmax, den
+    print("# fp64 %.3f ms/step  fp32-local %.3f ms/step" % (ms64, ms32), file=sys.stderr, flush=True)
+
+    # Algorithmic bytes (unique data, one pass): values written once, connectivity,
+    # iterate and vertex coordinates read once.
+    n2, n1 = p2.shape[0], p1.shape[0]
+    bytes_vel = 8 * (nnzA + nnzBT) + 4 * 10 * E + 8 * 3 * n2 + 24 * n1
+    bytes_step = 8 * (nnzA + nnzBT + nnzB) + 4 * (10 + 4) * E + 8 * 3 * n2 + 24 * n1
+    vel_avg_ms = kv64[0] / max(kv64[1], 1)
+    achieved = bytes_vel / (vel_avg_ms * 1e-3) / 1e9
+
+    cpu = None
+    if args.cpu_sample > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as O
+        n = min(args.cpu_sample, E)
+        secs = O.time_ns_jacobian(p2, e2, e1, n1, u, p, NU, RHO, n)
+        cpu = {"value": n / secs, "unit": "elems/s", "cores": 1, "kind": "port",
+               "sample": "first %d of %d elements of the same workload: FEDDLib's element loop restated "
+                         "in C++ (per-element quadrature tables, 34x34 element matrix, row-wise "
+                         "insertion, Tpetra-style fillComplete); %.1f s on %s" % (n, E, secs, cpu_model())}
+
+    out = {
+        "metric": "P2-P1 NS Jacobian elems/s",
+        "value": E / (ms64 * 1e-3),
+        "unit": "elems/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms64,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "FE::assemblyNavierStokes Jacobian (A, B^T, B; Newton linearisation), P2-P1 "
+                        "Taylor-Hood, unit cube %d^3x6 = %d tets, nu=0.01 rho=1, u = 8 seeded sine modes"
+                        % (M, E),
+            "tets": E, "velocity_dofs": 3 * n2, "pressure_dofs": n1,
+            "nnz": {"A": nnzA, "BT": nnzBT, "B": nnzB, "C": 0},
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": "k_ns_vel_rows<double>", "achieved": achieved,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "algorithmic_bytes_per_launch": bytes_vel, "avg_launch_ms": vel_avg_ms,
+        },
+        "step": {
+            "algorithmic_bytes": bytes_step,
+            "achieved_GBs": bytes_step / (ms64 * 1e-3) / 1e9,
+            "frac": bytes_step / (ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "pres_rows_avg_ms": kp64[0] / max(kp64[1], 1),
+        },
+        "fp32_local": {
+            "value": E / (ms32 * 1e-3), "ms_per_step": ms32,
+            "max_rel_err_A_vs_fp64": fp32_err,
+            "vel_rows_avg_ms": kv32[0] / max(kv32[1], 1),
+        },
+        "cpu_baseline": cpu,
+    }
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,766 @@
+// Host side of libfeddgpu: context, mesh upload, Tpetra-identical graph
+// construction, row-gather schedules, colouring and the C ABI entry points.
+#include <algorithm>
+#include <array>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <thread>
+
+#include "fedd_internal.h"
+#include "fedd_perm.h"
+
+namespace fedd {
+
+int fail(Ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int alloc(Ctx* c, DevBuf& b, size_t bytes) {
+  b.release();
+  if (bytes == 0) return FEDD_OK;
+  FEDD_HIP(c, hipMalloc(&b.p, bytes));
+  b.bytes = bytes;
+  return FEDD_OK;
+}
+
+int upload(Ctx* c, DevBuf& b, const void* host, size_t bytes) {
+  int rc = alloc(c, b, bytes);
+  if (rc) return rc;
+  if (bytes) FEDD_HIP(c, hipMemcpyAsync(b.p, host, bytes, hipMemcpyHostToDevice, c->stream));
+  return FEDD_OK;
+}
+
+static hipEvent_t kevent(Ctx* c) {
+  if (c->kpool.empty()) {
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+  }
+  hipEvent_t e = c->kpool.back();
+  c->kpool.pop_back();
+  return e;
+}
+
+void ktimer_resolve(Ctx* c) {
+  for (auto& k : c->kpending) {
+    float ms = 0.f;
+    (void)hipEventSynchronize(k.b);
+    (void)hipEventElapsedTime(&ms, k.a, k.b);
+    auto& t = c->ktime[k.name];
+    t.first += ms;
+    t.second += 1;
+    c->kpool.push_back(k.a);
+    c->kpool.push_back(k.b);
+  }
+  c->kpending.clear();
+}
+
+void ktimer_begin(Ctx* c) {
+  if (!c->timing) return;
+  if (c->kpending.size() >= 4096) ktimer_resolve(c);
+  c->kopen = kevent(c);
+  (void)hipEventRecord(c->kopen, c->stream);
+}
+
+void ktimer_end(Ctx* c, const char* name) {
+  if (!c->timing || !c->kopen) return;
+  hipEvent_t b = kevent(c);
+  (void)hipEventRecord(b, c->stream);
+  c->kpending.push_back({name, c->kopen, b});
+  c->kopen = nullptr;
+}
+
+namespace {
+
+unsigned nthreads() {
+  unsigned n = std::thread::hardware_concurrency();
+  return std::max(1u, std::min(n, 16u));
+}
+
+template <class F>
+void parallel_for(int64_t n, F f) {
+  unsigned T = nthreads();
+  if (n < 4096 || T == 1) {
+    f(0, n, 0);
+    return;
+  }
+  std::vector<std::thread> th;
+  int64_t step = (n + T - 1) / T;
+  for (unsigned t = 0; t < T; ++t) {
+    int64_t a = t * step, b = std::min(n, a + step);
+    if (a >= b) break;
+    th.emplace_back(f, a, b, (int)t);
+  }
+  for (auto& x : th) x.join();
+}
+
+void build_incidence(const Mesh& m, Incidence& inc) {
+  inc.ptr.assign(m.npts + 1, 0);
+  for (int64_t e = 0; e < m.nelem; ++e)
+    for (int k = 0; k < m.npe; ++k) inc.ptr[m.elems[e * m.npe + k] + 1]++;
+  for (int64_t i = 0; i < m.npts; ++i) inc.ptr[i + 1] += inc.ptr[i];
+  inc.elem.resize(inc.ptr[m.npts]);
+  inc.loc.resize(inc.ptr[m.npts]);
+  std::vector<int64_t> fill(inc.ptr.begin(), inc.ptr.end() - 1);
+  for (int64_t e = 0; e < m.nelem; ++e)
+    for (int k = 0; k < m.npe; ++k) {
+      int32_t n = m.elems[e * m.npe + k];
+      int64_t pos = fill[n]++;
+      inc.elem[pos] = (int32_t)e;
+      inc.loc[pos] = (uint8_t)k;
+    }
+}
+
+// For every row node i of `rows`, the sorted unique nodes of mesh `cols` found in
+// the elements incident to i (the elements of `cols` have the same ids as those
+// of `rows`).  This is exactly the column set Tpetra holds after inserting every
+// element's rows and calling fillComplete.
+void build_adjacency(const Incidence& inc, int64_t nrows, const Mesh& cols, Adjacency& adj) {
+  adj.ptr.assign(nrows + 1, 0);
+  std::vector<int64_t> cnt(nrows, 0);
+  parallel_for(nrows, [&](int64_t a, int64_t b, int) {
+    std::vector<int32_t> buf;
+    for (int64_t i = a; i < b; ++i) {
+      buf.clear();
+      for (int64_t q = inc.ptr[i]; q < inc.ptr[i + 1]; ++q) {
+        const int32_t* n = &cols.elems[(int64_t)inc.elem[q] * cols.npe];
+        buf.insert(buf.end(), n, n + cols.npe);
+      }
+      std::sort(buf.begin(), buf.end());
+      cnt[i] = std::unique(buf.begin(), buf.end()) - buf.begin();
+    }
+  });
+  for (int64_t i = 0; i < nrows; ++i) adj.ptr[i + 1] = adj.ptr[i] + cnt[i];
+  adj.idx.resize(adj.ptr[nrows]);
+  parallel_for(nrows, [&](int64_t a, int64_t b, int) {
+    std::vector<int32_t> buf;
+    for (int64_t i = a; i < b; ++i) {
+      buf.clear();
+      for (int64_t q = inc.ptr[i]; q < inc.ptr[i + 1]; ++q) {
+        const int32_t* n = &cols.elems[(int64_t)inc.elem[q] * cols.npe];
+        buf.insert(buf.end(), n, n + cols.npe);
+      }
+      std::sort(buf.begin(), buf.end());
+      auto end = std::unique(buf.begin(), buf.end());
+      std::copy(buf.begin(), end, adj.idx.begin() + adj.ptr[i]);
+    }
+  });
+}
+
+inline int find_slot(const Adjacency& adj, int64_t i, int32_t j) {
+  const int32_t* b = adj.idx.data() + adj.ptr[i];
+  const int32_t* e = adj.idx.data() + adj.ptr[i + 1];
+  return (int)(std::lower_bound(b, e, j) - b);
+}
+
+// Row-gather schedule.  Nodes are cut into contiguous chunks whose rows fit the
+// LDS budget; inside a chunk the (node, element) pairs are ordered vertex class
+// first, then edge class, interleaving nodes so that neighbouring lanes add into
+// different rows.  Slots are stored in canonical column order (fedd_perm.h).
+struct PlanSpec {
+  int rows_per_node;   // LDS rows per node of the main block (1 or 3)
+  int comps_per_col;   // values per neighbour node per row (1 or 3)
+  const Adjacency* main;
+  const Adjacency* second;  // B^T-like block (may be null)
+  int second_rows;          // rows per node of the second block
+  const Mesh* cols_main;    // mesh whose element nodes index `main`
+  const Mesh* cols_second;  // mesh whose element nodes index `second`
+  int lds_budget_doubles;
+};
+
+int build_plan(Ctx* c, const Incidence& inc, int64_t nnodes, const Mesh& rows_mesh,
+               const PlanSpec& s, GatherPlan& plan) {
+  std::vector<int32_t> chunk_node(1, 0), chunk_pair(1, 0), chunk_split;
+  int64_t npairs = inc.ptr[nnodes];
+  std::vector<int32_t> pair_elem(npairs);
+  std::vector<uint8_t> pair_meta(npairs * 16, 0);
+  int max_lds = 0;
+  auto node_lds = [&](int64_t i) {
+    int64_t v = (int64_t)s.rows_per_node * s.comps_per_col * s.main->deg(i);
+    if (s.second) v += (int64_t)s.second_rows * s.second->deg(i);
+    return v;
+  };
+  for (int64_t i = 0; i < nnodes; ++i) {
+    if (s.main->deg(i) > 255 || (s.second && s.second->deg(i) > 255))
+      return fail(c, FEDD_ERR_LIMIT, "a node has more than 255 neighbour nodes");
+    if (node_lds(i) > s.lds_budget_doubles)
+      return fail(c, FEDD_ERR_LIMIT, "a single row block exceeds the LDS budget");
+  }
+  int64_t n0 = 0;
+  int64_t pair_out = 0;
+  std::vector<std::array<int64_t, 3>> order;  // (key, node, q)
+  while (n0 < nnodes) {
+    int64_t n1 = n0, lds = 0;
+    while (n1 < nnodes && n1 - n0 < 256 && lds + node_lds(n1) <= s.lds_budget_doubles &&
+           (inc.ptr[n1 + 1] - inc.ptr[n0]) <= 1024) {
+      lds += node_lds(n1);
+      ++n1;
+    }
+    if (n1 == n0) n1 = n0 + 1;  // cannot happen given the checks above
+    max_lds = std::max<int64_t>(max_lds, lds);
+    order.clear();
+    for (int64_t i = n0; i < n1; ++i) {
+      int rank[2] = {0, 0};
+      for (int64_t q = inc.ptr[i]; q < inc.ptr[i + 1]; ++q) {
+        int li = inc.loc[q];
+        int cls = (rows_mesh.npe == 10 && li >= 4) ? 1 : 0;
+        int64_t key = ((int64_t)cls << 40) | ((int64_t)rank[cls]++ << 20) | (i - n0);
+        order.push_back({key, i, q});
+      }
+    }
+    std::sort(order.begin(), order.end());
+    int32_t split = -1;
+    for (size_t k = 0; k < order.size(); ++k) {
+      int64_t i = order[k][1], q = order[k][2];
+      int cls = (int)(order[k][0] >> 40);
+      if (cls == 1 && split < 0) split = (int32_t)(pair_out);
+      int32_t e = inc.elem[q];
+      int li = inc.loc[q];
+      pair_elem[pair_out] = e;
+      uint8_t* m = &pair_meta[pair_out * 16];
+      m[0] = (uint8_t)li;
+      m[1] = (uint8_t)(i - n0);
+      // the relabelling acts on the column element: P2 columns use the tet-10
+      // permutation (its vertex part is the tet-4 one), P1 columns the tet-4 one
+      int ncol = s.cols_main->npe;
+      const int8_t* perm = ncol == 10 ? kPermP2[li] : kPermP1[li];
+      for (int cpos = 0; cpos < ncol; ++cpos) {
+        int32_t j = s.cols_main->elems[(int64_t)e * ncol + perm[cpos]];
+        m[2 + cpos] = (uint8_t)find_slot(*s.main, i, j);
+      }
+      if (s.second) {
+        for (int cpos = 0; cpos < 4; ++cpos) {
+          int32_t j = s.cols_second->elems[(int64_t)e * 4 + perm[cpos]];
+          m[12 + cpos] = (uint8_t)find_slot(*s.second, i, j);
+        }
+      }
+      ++pair_out;
+    }
+    if (split < 0) split = (int32_t)pair_out;
+    chunk_split.push_back(split);
+    chunk_node.push_back((int32_t)n1);
+    chunk_pair.push_back((int32_t)pair_out);
+    n0 = n1;
+  }
+  plan.nchunks = (int64_t)chunk_split.size();
+  plan.npairs = npairs;
+  plan.lds_doubles = max_lds;
+  int rc;
+  if ((rc = upload(c, plan.d_chunk_node, chunk_node.data(), chunk_node.size() * 4))) return rc;
+  if ((rc = upload(c, plan.d_chunk_pair, chunk_pair.data(), chunk_pair.size() * 4))) return rc;
+  if ((rc = upload(c, plan.d_chunk_split, chunk_split.data(), chunk_split.size() * 4))) return rc;
+  if ((rc = upload(c, plan.d_pair_elem, pair_elem.data(), pair_elem.size() * 4))) return rc;
+  if ((rc = upload(c, plan.d_pair_meta, pair_meta.data(), pair_meta.size()))) return rc;
+  return FEDD_OK;
+}
+
+// Structured node adjacency buffers kept while the device expands them.
+struct GraphHost {
+  Adjacency a, bt, b;
+};
+
+}  // namespace
+}  // namespace fedd
+
+using namespace fedd;
+
+// ----------------------------------------------------------------------------
+// Structured unit-cube mesh (MeshStructured::buildMesh3D, one rank, N = 1,
+// MeshStructured_def.hpp:622-1009).  Every hex (r,s,t) is split into the same
+// six tets sharing the diagonal (r,s,t)-(r+1,s+1,t+1); P2 edge nodes are the
+// fine-lattice midpoints of the element edges 4:(0,1) 5:(1,2) 6:(0,2) 7:(0,3)
+// 8:(1,3) 9:(2,3).
+// ----------------------------------------------------------------------------
+static const int kHexTets[6][4][3] = {
+    {{1, 0, 0}, {0, 0, 0}, {1, 0, 1}, {1, 1, 1}}, {{0, 0, 1}, {0, 0, 0}, {1, 0, 1}, {1, 1, 1}},
+    {{1, 0, 0}, {0, 0, 0}, {1, 1, 0}, {1, 1, 1}}, {{0, 0, 0}, {0, 1, 0}, {1, 1, 0}, {1, 1, 1}},
+    {{0, 0, 0}, {0, 1, 0}, {0, 1, 1}, {1, 1, 1}}, {{0, 0, 0}, {0, 0, 1}, {0, 1, 1}, {1, 1, 1}}};
+static const int kTetEdges[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};
+
+extern "C" int fedd_mesh_structured(int M, int fe, double* pts, int32_t* bc, int32_t* elems) {
+  if (M < 1 || (fe != 1 && fe != 2) || !pts || !elems) return FEDD_ERR_ARG;
+  const int64_t P = fe == 1 ? M + 1 : 2 * M + 1;  // lattice points per direction
+  if (P * P * P > INT32_MAX) return FEDD_ERR_LIMIT;
+  const double eps = std::numeric_limits<double>::epsilon();
+  const double h = 1.0 / M;
+  for (int64_t t = 0; t < P; ++t)
+    for (int64_t s = 0; s < P; ++s)
+      for (int64_t r = 0; r < P; ++r) {
+        const int64_t id = r + s * P + t * P * P;
+        const int64_t q[3] = {r, s, t};
+        int flag = 0;
+        for (int d = 0; d < 3; ++d) {
+          double x = fe == 1 ? q[d] * h : q[d] * h / 2.0;
+          if (x < eps && x > -eps) x = 0.0;
+          if (x > 1.0 - eps || x < eps) flag = 1;
+          pts[3 * id + d] = x;
+        }
+        if (bc) bc[id] = flag;
+      }
+  const int npe = fe == 1 ? 4 : 10;
+  const int sc = fe == 1 ? 1 : 2;  // vertex stride in the lattice
+  int64_t e = 0;
+  for (int64_t t = 0; t < M; ++t)
+    for (int64_t s = 0; s < M; ++s)
+      for (int64_t r = 0; r < M; ++r)
+        for (int k = 0; k < 6; ++k, ++e) {
+          int64_t v[4][3];
+          for (int a = 0; a < 4; ++a) {
+            v[a][0] = sc * (r + kHexTets[k][a][0]);
+            v[a][1] = sc * (s + kHexTets[k][a][1]);
+            v[a][2] = sc * (t + kHexTets[k][a][2]);
+            elems[e * npe + a] = (int32_t)(v[a][0] + v[a][1] * P + v[a][2] * P * P);
+          }
+          if (fe == 2)
+            for (int m = 0; m < 6; ++m) {
+              const int64_t* p0 = v[kTetEdges[m][0]];
+              const int64_t* p1 = v[kTetEdges[m][1]];
+              elems[e * npe + 4 + m] = (int32_t)((p0[0] + p1[0]) / 2 + (p0[1] + p1[1]) / 2 * P +
+                                                 (p0[2] + p1[2]) / 2 * P * P);
+            }
+        }
+  return FEDD_OK;
+}
+
+// Mesh::correctElementOrientation + flipElement (Mesh_def.hpp:543-578, 648-726):
+// elements with det(B) < 0 get local nodes 1 and 2 swapped (and the P2 edge nodes
+// permuted accordingly).  Returns the number of flipped elements in *flipped.
+extern "C" int fedd_mesh_correct_orientation(const double* pts, int64_t npts, int32_t* elems,
+                                             int64_t nelem, int npe, int64_t* flipped) {
+  if (!pts || !elems || (npe != 4 && npe != 10)) return FEDD_ERR_ARG;
+  int64_t nf = 0;
+  for (int64_t e = 0; e < nelem; ++e) {
+    int32_t* n = elems + e * npe;
+    double a[3][3];
+    for (int j = 0; j < 3; ++j)
+      for (int i = 0; i < 3; ++i) {
+        if (n[j + 1] < 0 || n[j + 1] >= npts || n[0] < 0 || n[0] >= npts) return FEDD_ERR_ARG;
+        a[i][j] = pts[3 * (int64_t)n[j + 1] + i] - pts[3 * (int64_t)n[0] + i];
+      }
+    double det = a[0][0] * a[1][1] * a[2][2] + a[0][1] * a[1][2] * a[2][0] + a[0][2] * a[1][0] * a[2][1] -
+                 a[2][0] * a[1][1] * a[0][2] - a[2][1] * a[1][2] * a[0][0] - a[2][2] * a[1][0] * a[0][1];
+    if (det < 0) {
+      ++nf;
+      std::swap(n[1], n[2]);
+      if (npe == 10) {
+        std::swap(n[4], n[6]);
+        std::swap(n[8], n[9]);
+      }
+    }
+  }
+  if (flipped) *flipped = nf;
+  return FEDD_OK;
+}
+
+struct fedd_ctx : public fedd::Ctx {};
+
+// LDS budget of one row-gather chunk, in doubles (40 KiB).
+static const int kLdsBudget = 5120;
+
+extern "C" {
+
+int fedd_device_count(int* count) {
+  if (!count) return FEDD_ERR_ARG;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  *count = n;
+  return FEDD_OK;
+}
+
+int fedd_ctx_create(int device, fedd_ctx** out) {
+  if (!out) return FEDD_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return FEDD_ERR_NODEVICE;
+  if (device < 0 || device >= n) return FEDD_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return FEDD_ERR_HIP;
+  fedd_ctx* c = new fedd_ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return FEDD_ERR_HIP;
+  }
+  for (auto& e : c->ev) (void)hipEventCreate(&e);
+  *out = c;
+  return FEDD_OK;
+}
+
+void fedd_ctx_destroy(fedd_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  ktimer_resolve(c);
+  for (auto e : c->kpool) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* fedd_last_error(const fedd_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int fedd_synchronize(fedd_ctx* c) {
+  if (!c) return FEDD_ERR_ARG;
+  FEDD_HIP(c, hipStreamSynchronize(c->stream));
+  return FEDD_OK;
+}
+
+int fedd_mesh_upload(fedd_ctx* c, int slot, const double* pts, int64_t npts,
+                     const int32_t* elems, int64_t nelem, int npe) {
+  if (!c) return FEDD_ERR_ARG;
+  if (slot < 0 || slot > 1) return fail(c, FEDD_ERR_ARG, "mesh slot must be 0 or 1");
+  if (!pts || !elems || npts <= 0 || nelem <= 0)
+    return fail(c, FEDD_ERR_ARG, "empty mesh");
+  if (npe != 4 && npe != 10) return fail(c, FEDD_ERR_ARG, "nodes_per_elem must be 4 (P1) or 10 (P2)");
+  if (slot == 1 && npe != 4) return fail(c, FEDD_ERR_ARG, "the pressure mesh must be P1");
+  if (npts > INT32_MAX) return fail(c, FEDD_ERR_LIMIT, "more than 2^31-1 nodes");
+  for (int64_t k = 0; k < nelem * npe; ++k)
+    if (elems[k] < 0 || elems[k] >= npts) return fail(c, FEDD_ERR_ARG, "element node id out of range");
+  (void)hipSetDevice(c->device);
+  Mesh& m = c->mesh[slot];
+  m.npts = npts;
+  m.nelem = nelem;
+  m.npe = npe;
+  m.pts.assign(pts, pts + npts * 3);
+  m.elems.assign(elems, elems + nelem * npe);
+  std::vector<double> soa(npts * 3);
+  for (int64_t i = 0; i < npts; ++i)
+    for (int d = 0; d < 3; ++d) soa[d * npts + i] = pts[3 * i + d];
+  std::vector<int32_t> conn(nelem * npe);
+  for (int64_t e = 0; e < nelem; ++e)
+    for (int k = 0; k < npe; ++k) conn[k * nelem + e] = elems[e * npe + k];
+  int rc;
+  if ((rc = upload(c, m.d_xyz, soa.data(), soa.size() * 8))) return rc;
+  if ((rc = upload(c, m.d_conn, conn.data(), conn.size() * 4))) return rc;
+  FEDD_HIP(c, hipStreamSynchronize(c->stream));
+  c->graph_kind = GK_NONE;
+  c->colour_ptr.clear();
+  c->colour_perm.clear();
+  c->newton_step = 0;
+  return FEDD_OK;
+}
+
+int fedd_graph_build(fedd_ctx* c, int kind) {
+  if (!c) return FEDD_ERR_ARG;
+  if (kind < FEDD_GRAPH_SCALAR || kind > FEDD_GRAPH_NS) return fail(c, FEDD_ERR_ARG, "unknown graph kind");
+  Mesh& m = c->mesh[0];
+  if (!m.valid()) return fail(c, FEDD_ERR_STATE, "no primary mesh");
+  if (kind == FEDD_GRAPH_NS) {
+    Mesh& pm = c->mesh[1];
+    if (!pm.valid()) return fail(c, FEDD_ERR_STATE, "no pressure mesh");
+    if (m.npe != 10) return fail(c, FEDD_ERR_ARG, "Navier-Stokes needs a P2 velocity mesh");
+    if (pm.nelem != m.nelem)
+      return fail(c, FEDD_ERR_ARG, "Number Elements not the same as number assembleFE elements.");
+  }
+  (void)hipSetDevice(c->device);
+  c->graph_kind = GK_NONE;
+  for (auto& b : c->blk) {
+    b.nrows = b.ncols = b.nnz = 0;
+    b.d_rowptr.release();
+    b.d_colind.release();
+    b.d_vals.release();
+  }
+  Incidence inc;
+  build_incidence(m, inc);
+  GraphHost g;
+  build_adjacency(inc, m.npts, m, g.a);
+  int rc;
+  const int64_t N = m.npts;
+  if (kind == FEDD_GRAPH_SCALAR) {
+    c->blk[0].nrows = N;
+    c->blk[0].ncols = N;
+    c->blk[0].nnz = g.a.ptr[N];
+  } else {
+    c->blk[0].nrows = 3 * N;
+    c->blk[0].ncols = 3 * N;
+    c->blk[0].nnz = 9 * g.a.ptr[N];
+  }
+  Incidence pinc;
+  if (kind == FEDD_GRAPH_NS) {
+    Mesh& pm = c->mesh[1];
+    build_adjacency(inc, N, pm, g.bt);
+    build_incidence(pm, pinc);
+    build_adjacency(pinc, pm.npts, m, g.b);
+    c->blk[1].nrows = 3 * N;
+    c->blk[1].ncols = pm.npts;
+    c->blk[1].nnz = 3 * g.bt.ptr[N];
+    c->blk[2].nrows = pm.npts;
+    c->blk[2].ncols = 3 * N;
+    c->blk[2].nnz = 3 * g.b.ptr[pm.npts];
+    c->blk[3].nrows = pm.npts;
+    c->blk[3].ncols = pm.npts;
+    c->blk[3].nnz = 0;
+  }
+  // node-level pointers and adjacency on the device, expanded by a kernel
+  if ((rc = upload(c, c->d_nptr, g.a.ptr.data(), g.a.ptr.size() * 8))) return rc;
+  DevBuf d_aidx, d_btidx, d_bidx;
+  if ((rc = upload(c, d_aidx, g.a.idx.data(), g.a.idx.size() * 4))) return rc;
+  if (kind == FEDD_GRAPH_NS) {
+    if ((rc = upload(c, c->d_pptr, g.bt.ptr.data(), g.bt.ptr.size() * 8))) return rc;
+    if ((rc = upload(c, c->d_bptr, g.b.ptr.data(), g.b.ptr.size() * 8))) return rc;
+    if ((rc = upload(c, d_btidx, g.bt.idx.data(), g.bt.idx.size() * 4))) return rc;
+    if ((rc = upload(c, d_bidx, g.b.idx.data(), g.b.idx.size() * 4))) return rc;
+  }
+  int nb = kind == FEDD_GRAPH_NS ? 4 : 1;
+  for (int k = 0; k < nb; ++k) {
+    Block& b = c->blk[k];
+    if ((rc = alloc(c, b.d_rowptr, (b.nrows + 1) * 8))) return rc;
+    if ((rc = alloc(c, b.d_colind, std::max<int64_t>(b.nnz, 1) * 4))) return rc;
+    if ((rc = alloc(c, b.d_vals, std::max<int64_t>(b.nnz, 1) * 8))) return rc;
+    FEDD_HIP(c, hipMemsetAsync(b.d_vals.p, 0, std::max<int64_t>(b.nnz, 1) * 8, c->stream));
+  }
+  c->graph_kind = kind;
+  rc = expand_graph(c, kind, d_aidx.as<int32_t>(), d_btidx.as<int32_t>(), d_bidx.as<int32_t>());
+  if (rc) {
+    c->graph_kind = GK_NONE;
+    return rc;
+  }
+  // gather schedules
+  PlanSpec vs{};
+  vs.main = &g.a;
+  vs.cols_main = &m;
+  vs.lds_budget_doubles = kLdsBudget;
+  if (kind == FEDD_GRAPH_SCALAR) {
+    vs.rows_per_node = 1;
+    vs.comps_per_col = 1;
+  } else {
+    vs.rows_per_node = 3;
+    vs.comps_per_col = 3;
+  }
+  if (kind == FEDD_GRAPH_NS) {
+    vs.second = &g.bt;
+    vs.second_rows = 3;
+    vs.cols_second = &c->mesh[1];
+  }
+  if ((rc = build_plan(c, inc, N, m, vs, c->vplan))) {
+    c->graph_kind = GK_NONE;
+    return rc;
+  }
+  if (kind == FEDD_GRAPH_NS) {
+    PlanSpec ps{};
+    ps.rows_per_node = 1;
+    ps.comps_per_col = 3;
+    ps.main = &g.b;
+    ps.cols_main = &m;
+    ps.lds_budget_doubles = kLdsBudget;
+    if ((rc = build_plan(c, pinc, c->mesh[1].npts, c->mesh[1], ps, c->pplan))) {
+      c->graph_kind = GK_NONE;
+      return rc;
+    }
+  }
+  FEDD_HIP(c, hipStreamSynchronize(c->stream));
+  c->newton_step = 0;
+  return FEDD_OK;
+}
+
+int fedd_block_info(fedd_ctx* c, int block, int64_t* nrows, int64_t* ncols, int64_t* nnz) {
+  if (!c || block < 0 || block > 3) return FEDD_ERR_ARG;
+  if (c->graph_kind == GK_NONE) return fail(c, FEDD_ERR_STATE, "no graph");
+  const Block& b = c->blk[block];
+  if (nrows) *nrows = b.nrows;
+  if (ncols) *ncols = b.ncols;
+  if (nnz) *nnz = b.nnz;
+  return FEDD_OK;
+}
+
+int fedd_block_graph(fedd_ctx* c, int block, int64_t* rowptr, int32_t* colind) {
+  if (!c || block < 0 || block > 3) return FEDD_ERR_ARG;
+  if (c->graph_kind == GK_NONE) return fail(c, FEDD_ERR_STATE, "no graph");
+  const Block& b = c->blk[block];
+  if (b.nrows == 0) return fail(c, FEDD_ERR_ARG, "block not present in this graph");
+  (void)hipSetDevice(c->device);
+  if (rowptr)
+    FEDD_HIP(c, hipMemcpyAsync(rowptr, b.d_rowptr.p, (b.nrows + 1) * 8, hipMemcpyDeviceToHost, c->stream));
+  if (colind && b.nnz)
+    FEDD_HIP(c, hipMemcpyAsync(colind, b.d_colind.p, b.nnz * 4, hipMemcpyDeviceToHost, c->stream));
+  FEDD_HIP(c, hipStreamSynchronize(c->stream));
+  return FEDD_OK;
+}
+
+int fedd_block_values(fedd_ctx* c, int block, double* values) {
+  if (!c || block < 0 || block > 3 || !values) return FEDD_ERR_ARG;
+  if (c->graph_kind == GK_NONE) return fail(c, FEDD_ERR_STATE, "no graph");
+  const Block& b = c->blk[block];
+  (void)hipSetDevice(c->device);
+  if (b.nnz)
+    FEDD_HIP(c, hipMemcpyAsync(values, b.d_vals.p, b.nnz * 8, hipMemcpyDeviceToHost, c->stream));
+  FEDD_HIP(c, hipStreamSynchronize(c->stream));
+  return FEDD_OK;
+}
+
+int fedd_block_device_values(fedd_ctx* c, int block, double** d) {
+  if (!c || block < 0 || block > 3 || !d) return FEDD_ERR_ARG;
+  if (c->graph_kind == GK_NONE) return fail(c, FEDD_ERR_STATE, "no graph");
+  *d = c->blk[block].d_vals.as<double>();
+  return FEDD_OK;
+}
+
+int fedd_colour_build(fedd_ctx* c, int* ncolours) {
+  if (!c) return FEDD_ERR_ARG;
+  Mesh& m = c->mesh[0];
+  if (!m.valid()) return fail(c, FEDD_ERR_STATE, "no primary mesh");
+  // Greedy first-fit colouring in ascending element order: an element takes the
+  // smallest colour not used by any element sharing a node with it.
+  Incidence inc;
+  build_incidence(m, inc);
+  std::vector<int32_t> col(m.nelem, -1);
+  std::vector<int64_t> used;  // used[colour] == e marks colour as taken for element e
+  int nc = 0;
+  for (int64_t e = 0; e < m.nelem; ++e) {
+    for (int k = 0; k < m.npe; ++k) {
+      int32_t n = m.elems[e * m.npe + k];
+      for (int64_t q = inc.ptr[n]; q < inc.ptr[n + 1]; ++q) {
+        int32_t f = inc.elem[q];
+        if (f < e && col[f] >= 0) {
+          if ((int)used.size() <= col[f]) used.resize(col[f] + 1, -1);
+          used[col[f]] = e;
+        }
+      }
+    }
+    int k = 0;
+    while (k < (int)used.size() && used[k] == e) ++k;
+    col[e] = k;
+    nc = std::max(nc, k + 1);
+  }
+  c->colour_ptr.assign(nc + 1, 0);
+  for (int64_t e = 0; e < m.nelem; ++e) c->colour_ptr[col[e] + 1]++;
+  for (int k = 0; k < nc; ++k) c->colour_ptr[k + 1] += c->colour_ptr[k];
+  c->colour_perm.resize(m.nelem);
+  std::vector<int64_t> fill(c->colour_ptr.begin(), c->colour_ptr.end() - 1);
+  for (int64_t e = 0; e < m.nelem; ++e) c->colour_perm[fill[col[e]]++] = (int32_t)e;
+  if (ncolours) *ncolours = nc;
+  return FEDD_OK;
+}
+
+int fedd_colour_export(fedd_ctx* c, int64_t* colour_ptr, int32_t* elem_perm) {
+  if (!c) return FEDD_ERR_ARG;
+  if (c->colour_ptr.empty()) return fail(c, FEDD_ERR_STATE, "no colouring");
+  if (colour_ptr) std::copy(c->colour_ptr.begin(), c->colour_ptr.end(), colour_ptr);
+  if (elem_perm) std::copy(c->colour_perm.begin(), c->colour_perm.end(), elem_perm);
+  return FEDD_OK;
+}
+
+static int64_t vec_len(fedd_ctx* c, int which) {
+  switch (which) {
+    case FEDD_VEC_U:
+    case FEDD_VEC_RES_U: return 3 * c->mesh[0].npts;
+    case FEDD_VEC_P:
+    case FEDD_VEC_RES_P: return c->mesh[1].npts;
+    case FEDD_VEC_RHS: return c->mesh[0].npts;
+  }
+  return -1;
+}
+
+int fedd_vector_upload(fedd_ctx* c, int which, const double* host, int64_t n) {
+  if (!c || which < 0 || which > 4 || !host) return FEDD_ERR_ARG;
+  int64_t want = vec_len(c, which);
+  if (n != want) return fail(c, FEDD_ERR_ARG, "vector length does not match the mesh");
+  (void)hipSetDevice(c->device);
+  int rc = upload(c, c->vec[which], host, n * 8);
+  if (rc) return rc;
+  c->vec_n[which] = n;
+  FEDD_HIP(c, hipStreamSynchronize(c->stream));
+  return FEDD_OK;
+}
+
+int fedd_vector_download(fedd_ctx* c, int which, double* host, int64_t n) {
+  if (!c || which < 0 || which > 4 || !host) return FEDD_ERR_ARG;
+  if (c->vec_n[which] != n) return fail(c, FEDD_ERR_ARG, "vector length mismatch or vector absent");
+  (void)hipSetDevice(c->device);
+  FEDD_HIP(c, hipMemcpyAsync(host, c->vec[which].p, n * 8, hipMemcpyDeviceToHost, c->stream));
+  FEDD_HIP(c, hipStreamSynchronize(c->stream));
+  return FEDD_OK;
+}
+
+int fedd_vector_device_ptr(fedd_ctx* c, int which, double** d, int64_t* n) {
+  if (!c || which < 0 || which > 4 || !d) return FEDD_ERR_ARG;
+  *d = c->vec[which].as<double>();
+  if (n) *n = c->vec_n[which];
+  return FEDD_OK;
+}
+
+int fedd_assembly_laplace(fedd_ctx* c, int precision) {
+  if (!c) return FEDD_ERR_ARG;
+  if (c->graph_kind != FEDD_GRAPH_SCALAR) return fail(c, FEDD_ERR_STATE, "needs a scalar graph");
+  if (precision != FEDD_FP64 && precision != FEDD_FP32_LOCAL) return fail(c, FEDD_ERR_ARG, "bad precision");
+  (void)hipSetDevice(c->device);
+  return launch_laplace(c, precision);
+}
+
+int fedd_assembly_rhs(fedd_ctx* c, double f) {
+  if (!c) return FEDD_ERR_ARG;
+  if (c->graph_kind != FEDD_GRAPH_SCALAR) return fail(c, FEDD_ERR_STATE, "needs a scalar graph");
+  (void)hipSetDevice(c->device);
+  int64_t n = c->mesh[0].npts;
+  if (c->vec_n[FEDD_VEC_RHS] != n) {
+    int rc = alloc(c, c->vec[FEDD_VEC_RHS], n * 8);
+    if (rc) return rc;
+    c->vec_n[FEDD_VEC_RHS] = n;
+  }
+  return launch_rhs(c, f);
+}
+
+int fedd_assembly_navier_stokes(fedd_ctx* c, const fedd_ns_params* p, int mode, int reassemble,
+                                int precision) {
+  if (!c || !p) return FEDD_ERR_ARG;
+  if (c->graph_kind != FEDD_GRAPH_NS) return fail(c, FEDD_ERR_STATE, "needs a Navier-Stokes graph");
+  if (mode < FEDD_NS_JACOBIAN || mode > FEDD_NS_RHS) return fail(c, FEDD_ERR_ARG, "unknown assembleMode");
+  if (precision != FEDD_FP64 && precision != FEDD_FP32_LOCAL) return fail(c, FEDD_ERR_ARG, "bad precision");
+  if (c->vec_n[FEDD_VEC_U] != 3 * c->mesh[0].npts) return fail(c, FEDD_ERR_STATE, "velocity iterate not uploaded");
+  if (c->vec_n[FEDD_VEC_P] != c->mesh[1].npts) return fail(c, FEDD_ERR_STATE, "pressure iterate not uploaded");
+  (void)hipSetDevice(c->device);
+  if (mode == FEDD_NS_RHS) {
+    for (int w : {FEDD_VEC_RES_U, FEDD_VEC_RES_P}) {
+      int64_t n = vec_len(c, w);
+      if (c->vec_n[w] != n) {
+        int rc = alloc(c, c->vec[w], n * 8);
+        if (rc) return rc;
+        c->vec_n[w] = n;
+      }
+    }
+  }
+  int rc = launch_ns_assembly(c, p, mode, reassemble, precision);
+  if (rc) return rc;
+  if (mode != FEDD_NS_RHS) c->newton_step++;  // advanceNewtonStep (FE_def.hpp:1171,1196)
+  return FEDD_OK;
+}
+
+int fedd_newton_step(fedd_ctx* c, int* step) {
+  if (!c || !step) return FEDD_ERR_ARG;
+  *step = c->newton_step;
+  return FEDD_OK;
+}
+
+int fedd_event_record(fedd_ctx* c, int slot) {
+  if (!c || slot < 0 || slot >= 16) return FEDD_ERR_ARG;
+  FEDD_HIP(c, hipEventRecord(c->ev[slot], c->stream));
+  return FEDD_OK;
+}
+
+int fedd_event_elapsed(fedd_ctx* c, int a, int b, float* ms) {
+  if (!c || a < 0 || a >= 16 || b < 0 || b >= 16 || !ms) return FEDD_ERR_ARG;
+  FEDD_HIP(c, hipEventSynchronize(c->ev[b]));
+  FEDD_HIP(c, hipEventElapsedTime(ms, c->ev[a], c->ev[b]));
+  return FEDD_OK;
+}
+
+int fedd_kernel_timing(fedd_ctx* c, int enable) {
+  if (!c) return FEDD_ERR_ARG;
+  ktimer_resolve(c);
+  c->timing = enable != 0;
+  c->ktime.clear();
+  return FEDD_OK;
+}
+
+int fedd_kernel_time(fedd_ctx* c, const char* name, double* total_ms, int64_t* launches) {
+  if (!c || !name) return FEDD_ERR_ARG;
+  ktimer_resolve(c);
+  auto it = c->ktime.find(name);
+  if (total_ms) *total_ms = it == c->ktime.end() ? 0.0 : it->second.first;
+  if (launches) *launches = it == c->ktime.end() ? 0 : it->second.second;
+  return FEDD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,134 @@
+// Internal types shared by the host code and the kernels of libfeddgpu.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/feddgpu.h"
+
+namespace fedd {
+
+// Device buffer owned by a context (hipMalloc / hipFree).
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+struct Mesh {
+  int64_t npts = 0, nelem = 0;
+  int npe = 0;
+  std::vector<double> pts;     // host, node-major
+  std::vector<int32_t> elems;  // host, element-major
+  DevBuf d_xyz;                // SoA: x[npts] y[npts] z[npts]
+  DevBuf d_conn;               // SoA: conn[k*nelem + e]
+  bool valid() const { return npts > 0; }
+};
+
+// Node -> element incidence: entries (element, local index), ascending element id.
+struct Incidence {
+  std::vector<int64_t> ptr;
+  std::vector<int32_t> elem;
+  std::vector<uint8_t> loc;
+};
+
+// Node adjacency (sorted unique neighbour node ids per node).
+struct Adjacency {
+  std::vector<int64_t> ptr;
+  std::vector<int32_t> idx;
+  int64_t deg(int64_t i) const { return ptr[i + 1] - ptr[i]; }
+};
+
+// One CRS block: Tpetra-identical graph + values, on the device.
+struct Block {
+  int64_t nrows = 0, ncols = 0, nnz = 0;
+  DevBuf d_rowptr;  // int64, nrows+1
+  DevBuf d_colind;  // int32, nnz
+  DevBuf d_vals;    // double, nnz
+};
+
+// Row-gather schedule over node chunks (see fedd_ns.hip for the layout).
+struct GatherPlan {
+  int64_t nchunks = 0, npairs = 0;
+  int lds_doubles = 0;          // max LDS doubles of any chunk
+  DevBuf d_chunk_node;          // int32 [nchunks+1]: first node of each chunk
+  DevBuf d_chunk_pair;          // int32 [nchunks+1]: first pair of each chunk
+  DevBuf d_chunk_split;         // int32 [nchunks]: first edge-class pair of each chunk
+  DevBuf d_pair_elem;           // int32 [npairs]
+  DevBuf d_pair_meta;           // uint4 [npairs]: loc, node-in-chunk, 14 slots
+};
+
+enum GraphKind { GK_NONE = -1 };
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  Mesh mesh[2];
+  int graph_kind = GK_NONE;
+  Block blk[4];
+  // node-level structures on the device
+  DevBuf d_nptr, d_pptr, d_bptr;  // int64 node-level row pointers (A, B^T, B)
+  GatherPlan vplan, pplan;        // velocity (primary) and pressure row gathers
+  // colouring
+  std::vector<int64_t> colour_ptr;
+  std::vector<int32_t> colour_perm;
+  // vectors
+  DevBuf vec[5];
+  int64_t vec_n[5] = {0, 0, 0, 0, 0};
+  int newton_step = 0;
+  // timing
+  hipEvent_t ev[16] = {};
+  bool timing = false;
+  std::map<std::string, std::pair<double, int64_t>> ktime;
+  struct KEv {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<KEv> kpending;       // recorded, not yet resolved
+  std::vector<hipEvent_t> kpool;   // free events
+  hipEvent_t kopen = nullptr;      // start event of the launch being timed
+};
+
+// Error helpers --------------------------------------------------------------
+int fail(Ctx* c, int code, const std::string& msg);
+#define FEDD_HIP(c, call)                                                        \
+  do {                                                                           \
+    hipError_t _e = (call);                                                      \
+    if (_e != hipSuccess)                                                        \
+      return ::fedd::fail((c), FEDD_ERR_HIP,                                     \
+                          std::string(#call) + ": " + hipGetErrorString(_e));    \
+  } while (0)
+
+int alloc(Ctx* c, DevBuf& b, size_t bytes);
+int upload(Ctx* c, DevBuf& b, const void* host, size_t bytes);
+
+// Kernel-timing bracket: events recorded on the context stream around a launch
+// and resolved lazily (no synchronisation inside the timed region).
+void ktimer_begin(Ctx* c);
+void ktimer_end(Ctx* c, const char* name);
+void ktimer_resolve(Ctx* c);
+
+// Launchers implemented in the .hip files -------------------------------------
+// Expands node-level adjacency into the dof-level CRS rowptr/colind of each block.
+int expand_graph(Ctx* c, int kind, const int32_t* aidx, const int32_t* btidx, const int32_t* bidx);
+int launch_ns_assembly(Ctx* c, const fedd_ns_params* p, int mode, int reassemble, int precision);
+int launch_laplace(Ctx* c, int precision);
+int launch_rhs(Ctx* c, double f);
+
+}  // namespace fedd

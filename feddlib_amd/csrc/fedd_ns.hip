@@ -1,0 +1,799 @@
+// Element-assembly kernels for gfx950: CRS graph expansion, the P2-P1
+// Navier-Stokes Jacobian / fixed-point / residual row gathers, and the P1/P2
+// Laplace stiffness and load vector.
+//
+// Row gather ("owner computes"): the rows of a contiguous chunk of nodes live in
+// LDS (FP64) while the chunk's (node, element) pairs add their element-matrix
+// rows into them; the chunk's values are then written to HBM exactly once, in
+// CRS order.  A pair relabels its element so that the row node sits at local
+// position 0 (vertex) or 4 (edge); the element integrals are exact reference
+// tensors (fedd_tensors.h), constant-folded into the two row kernels.
+//
+// Reference semantics (FEDDLib):
+//   element matrix  AssembleFENavierStokes_def.hpp:67-146 (Jacobian, FixedPoint)
+//   A  = nu*rho*|det| int grad phi_i . grad phi_j                     (:150-196, :81-82)
+//   N  = rho*|det| int (u . grad phi_j) phi_i, diagonal 3x3 blocks     (:228-298)
+//   W  = rho*|det| int phi_i phi_j du_d1/dx_d2, full 3x3 blocks        (:302-369)
+//   B  = -|det| int psi_m d phi_j/dx_d, placed as B and B^T            (:374-461, :88)
+//   residual = (constantMatrix + rho N) [u;p] (.) coeff, no W          (:200-224)
+//   scatter  = addFeBlock / addFeBlockMatrix                           (FE_def.hpp:982-1062, 1419-1442)
+#include "fedd_internal.h"
+#include "fedd_perm.h"
+#include "fedd_tensors.h"
+
+namespace fedd {
+
+__constant__ int8_t c_perm2[10][10] = FEDD_PERM_P2_INIT;
+__constant__ int8_t c_perm1[4][4] = FEDD_PERM_P1_INIT;
+
+namespace ft = fedd_tensors;
+
+// ---------------------------------------------------------------------------
+// CRS expansion from node adjacency
+// ---------------------------------------------------------------------------
+
+// A(0,0) of a 3-dof field: rows 3i+d1 hold the 3x3 node blocks of all neighbours
+// j (sorted), columns 3j+d2 (addFeBlock inserts full node blocks, FE_def.hpp:1430-1441).
+__global__ void k_expand_vec3(int64_t N, const int64_t* __restrict__ nptr,
+                              const int32_t* __restrict__ adj, int64_t* __restrict__ rowptr,
+                              int32_t* __restrict__ colind) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int64_t b = nptr[i], deg = nptr[i + 1] - b;
+  for (int d1 = 0; d1 < 3; ++d1) {
+    int64_t r = 9 * b + 3 * d1 * deg;
+    rowptr[3 * i + d1] = r;
+    for (int64_t s = 0; s < deg; ++s) {
+      int32_t j = adj[b + s];
+      colind[r + 3 * s + 0] = 3 * j + 0;
+      colind[r + 3 * s + 1] = 3 * j + 1;
+      colind[r + 3 * s + 2] = 3 * j + 2;
+    }
+  }
+  if (i == N - 1) rowptr[3 * N] = 9 * nptr[N];
+}
+
+// Scalar field: the node graph itself.
+__global__ void k_expand_scalar(int64_t N, const int64_t* __restrict__ nptr,
+                                const int32_t* __restrict__ adj, int64_t* __restrict__ rowptr,
+                                int32_t* __restrict__ colind) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int64_t b = nptr[i], deg = nptr[i + 1] - b;
+  rowptr[i] = b;
+  for (int64_t s = 0; s < deg; ++s) colind[b + s] = adj[b + s];
+  if (i == N - 1) rowptr[N] = nptr[N];
+}
+
+// B^T (0,1): rows 3i+d, columns the pressure neighbours of i (FE_def.hpp:1035-1046).
+__global__ void k_expand_bt(int64_t N, const int64_t* __restrict__ pptr,
+                            const int32_t* __restrict__ padj, int64_t* __restrict__ rowptr,
+                            int32_t* __restrict__ colind) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int64_t b = pptr[i], deg = pptr[i + 1] - b;
+  for (int d = 0; d < 3; ++d) {
+    int64_t r = 3 * b + d * deg;
+    rowptr[3 * i + d] = r;
+    for (int64_t s = 0; s < deg; ++s) colind[r + s] = padj[b + s];
+  }
+  if (i == N - 1) rowptr[3 * N] = 3 * pptr[N];
+}
+
+// B (1,0): rows p, columns 3j+d for the velocity neighbours j (FE_def.hpp:1048-1059).
+__global__ void k_expand_b(int64_t NP, const int64_t* __restrict__ bptr,
+                           const int32_t* __restrict__ badj, int64_t* __restrict__ rowptr,
+                           int32_t* __restrict__ colind) {
+  int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= NP) return;
+  int64_t b = bptr[p], deg = bptr[p + 1] - b;
+  rowptr[p] = 3 * b;
+  for (int64_t s = 0; s < deg; ++s) {
+    int32_t j = badj[b + s];
+    colind[3 * b + 3 * s + 0] = 3 * j + 0;
+    colind[3 * b + 3 * s + 1] = 3 * j + 1;
+    colind[3 * b + 3 * s + 2] = 3 * j + 2;
+  }
+  if (p == NP - 1) rowptr[NP] = 3 * bptr[NP];
+}
+
+__global__ void k_fill_i64(int64_t n, int64_t* __restrict__ a, int64_t v) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+static inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+int expand_graph(Ctx* c, int kind, const int32_t* aidx, const int32_t* btidx, const int32_t* bidx) {
+  const int64_t N = c->mesh[0].npts;
+  const int T = 256;
+  if (kind == FEDD_GRAPH_SCALAR) {
+    k_expand_scalar<<<nblk(N, T), T, 0, c->stream>>>(N, c->d_nptr.as<int64_t>(), aidx,
+                                                     c->blk[0].d_rowptr.as<int64_t>(),
+                                                     c->blk[0].d_colind.as<int32_t>());
+  } else {
+    k_expand_vec3<<<nblk(N, T), T, 0, c->stream>>>(N, c->d_nptr.as<int64_t>(), aidx,
+                                                   c->blk[0].d_rowptr.as<int64_t>(),
+                                                   c->blk[0].d_colind.as<int32_t>());
+  }
+  FEDD_HIP(c, hipGetLastError());
+  if (kind == FEDD_GRAPH_NS) {
+    const int64_t NP = c->mesh[1].npts;
+    k_expand_bt<<<nblk(N, T), T, 0, c->stream>>>(N, c->d_pptr.as<int64_t>(), btidx,
+                                                 c->blk[1].d_rowptr.as<int64_t>(),
+                                                 c->blk[1].d_colind.as<int32_t>());
+    k_expand_b<<<nblk(NP, T), T, 0, c->stream>>>(NP, c->d_bptr.as<int64_t>(), bidx,
+                                                 c->blk[2].d_rowptr.as<int64_t>(),
+                                                 c->blk[2].d_colind.as<int32_t>());
+    k_fill_i64<<<nblk(NP + 1, T), T, 0, c->stream>>>(NP + 1, c->blk[3].d_rowptr.as<int64_t>(), 0);
+    FEDD_HIP(c, hipGetLastError());
+  }
+  FEDD_HIP(c, hipStreamSynchronize(c->stream));
+  return FEDD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Element geometry (Helper::buildTransformation + SmallMatrix::computeInverse,
+// Helper.cpp:196-222, SmallMatrix.hpp:317-368), always FP64.
+// ---------------------------------------------------------------------------
+struct Geo {
+  double Bi[3][3];
+  double adet;
+};
+
+__device__ __forceinline__ void geometry(const double* __restrict__ xyz, int64_t npts,
+                                         const int32_t nodes[4], Geo& g) {
+  double x[4][3];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) x[k][d] = xyz[d * npts + nodes[k]];
+  double a[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) a[i][j] = x[j + 1][i] - x[0][i];
+  double det = a[0][0] * a[1][1] * a[2][2] + a[0][1] * a[1][2] * a[2][0] + a[0][2] * a[1][0] * a[2][1] -
+               a[2][0] * a[1][1] * a[0][2] - a[2][1] * a[1][2] * a[0][0] - a[2][2] * a[1][0] * a[0][1];
+  double id = 1.0 / det;
+  g.Bi[0][0] = (a[1][1] * a[2][2] - a[1][2] * a[2][1]) * id;
+  g.Bi[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) * id;
+  g.Bi[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) * id;
+  g.Bi[1][0] = (a[1][2] * a[2][0] - a[1][0] * a[2][2]) * id;
+  g.Bi[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) * id;
+  g.Bi[1][2] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) * id;
+  g.Bi[2][0] = (a[1][0] * a[2][1] - a[1][1] * a[2][0]) * id;
+  g.Bi[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) * id;
+  g.Bi[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) * id;
+  g.adet = fabs(det);
+}
+
+// Symmetric metric G_ab = sum_d Bi[a][d] Bi[b][d] in the pair order of KS_*.
+template <typename T>
+__device__ __forceinline__ void metric(const T Bi[3][3], T Gs[6]) {
+  const int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+  for (int s = 0; s < 6; ++s)
+    Gs[s] = Bi[pa[s]][0] * Bi[pb[s]][0] + Bi[pa[s]][1] * Bi[pb[s]][1] + Bi[pa[s]][2] * Bi[pb[s]][2];
+}
+
+template <int CLS>
+struct NsTensors;
+template <>
+struct NsTensors<0> {
+  static constexpr const double (&KS)[6][10] = ft::KS_V;
+  static constexpr const double (&TT)[3][10][10] = ft::T_V;
+  static constexpr const double (&L)[4][10] = ft::L_V;
+  static constexpr const double (&RV)[3][4] = ft::RV_V;
+  static constexpr int row = 0;
+};
+template <>
+struct NsTensors<1> {
+  static constexpr const double (&KS)[6][10] = ft::KS_E;
+  static constexpr const double (&TT)[3][10][10] = ft::T_E;
+  static constexpr const double (&L)[4][10] = ft::L_E;
+  static constexpr const double (&RV)[3][4] = ft::RV_E;
+  static constexpr int row = 4;
+};
+
+// ---------------------------------------------------------------------------
+// Navier-Stokes velocity rows
+// ---------------------------------------------------------------------------
+struct NsArgs {
+  // meshes
+  const double* xyz;       // P2 coordinates SoA
+  int64_t npts;
+  const int32_t* conn;     // P2 connectivity SoA
+  const int32_t* pconn;    // P1 connectivity SoA
+  int64_t nelem;
+  const double* u;         // velocity iterate, 3 per node
+  const double* p;         // pressure iterate
+  // node-level row pointers
+  const int64_t* nptr;     // A node graph
+  const int64_t* pptr;     // B^T node graph (velocity -> pressure)
+  const int64_t* bptr;     // B node graph (pressure -> velocity)
+  // plan
+  const int32_t* chunk_node;
+  const int32_t* chunk_pair;
+  const int32_t* chunk_split;
+  const int32_t* pair_elem;
+  const uint4* pair_meta;
+  // outputs
+  double* valsA;
+  double* valsBT;
+  double* valsB;
+  double* res_u;
+  double* res_p;
+  // parameters
+  double nurho, rho;
+  double coeff[4];
+  int newton;
+  int with_bt;
+};
+
+template <typename T, int CLS>
+struct VelRow {
+  // Computes the canonical row of one pair and adds it into LDS.
+  static __device__ __forceinline__ void run(const NsArgs& a, int32_t e, const uint8_t* m,
+                                             double* __restrict__ rowA, int deg,
+                                             double* __restrict__ rowBT, int pdeg) {
+    using NT = NsTensors<CLS>;
+    const int li = m[0];
+    int32_t nodes[10];
+#pragma unroll
+    for (int c = 0; c < 10; ++c) nodes[c] = a.conn[(int64_t)c_perm2[li][c] * a.nelem + e];
+    Geo g;
+    geometry(a.xyz, a.npts, nodes, g);
+    T Bi[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) Bi[i][j] = (T)g.Bi[i][j];
+    T Gs[6];
+    metric<T>(Bi, Gs);
+    // S_c = nu rho A_rc + rho N_rc   (without |det|)
+    T S[10];
+#pragma unroll
+    for (int c = 0; c < 10; ++c) {
+      T acc = 0;
+#pragma unroll
+      for (int s = 0; s < 6; ++s)
+        if (NT::KS[s][c] != 0.0) acc += Gs[s] * (T)NT::KS[s][c];
+      S[c] = (T)a.nurho * acc;
+    }
+    // grad-hat u at the vertices (P1 representation of grad u): gh[m][d1][axis]
+    T gh[4][3][3];
+#pragma unroll
+    for (int mm = 0; mm < 4; ++mm)
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) gh[mm][d][ax] = 0;
+    T Nacc[10];
+#pragma unroll
+    for (int c = 0; c < 10; ++c) Nacc[c] = 0;
+#pragma unroll
+    for (int k = 0; k < 10; ++k) {
+      T uk[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) uk[d] = (T)a.u[3 * (int64_t)nodes[k] + d];
+      T ut[3];  // ut_a = sum_d Bi[a][d] u_kd
+#pragma unroll
+      for (int ax = 0; ax < 3; ++ax) ut[ax] = Bi[ax][0] * uk[0] + Bi[ax][1] * uk[1] + Bi[ax][2] * uk[2];
+#pragma unroll
+      for (int c = 0; c < 10; ++c)
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax)
+          if (NT::TT[ax][k][c] != 0.0) Nacc[c] += ut[ax] * (T)NT::TT[ax][k][c];
+      if (a.newton) {
+#pragma unroll
+        for (int mm = 0; mm < 4; ++mm)
+#pragma unroll
+          for (int ax = 0; ax < 3; ++ax)
+            if (ft::DV[ax][k][mm] != 0.0)
+#pragma unroll
+              for (int d = 0; d < 3; ++d) gh[mm][d][ax] += uk[d] * (T)ft::DV[ax][k][mm];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 10; ++c) S[c] += (T)a.rho * Nacc[c];
+    const double adet = g.adet;
+    if (a.newton) {
+      // g[m][d1][d2] = sum_ax Bi[ax][d2] gh[m][d1][ax], scaled by rho
+      T gw[4][3][3];
+#pragma unroll
+      for (int mm = 0; mm < 4; ++mm)
+#pragma unroll
+        for (int d1 = 0; d1 < 3; ++d1)
+#pragma unroll
+          for (int d2 = 0; d2 < 3; ++d2)
+            gw[mm][d1][d2] = (T)a.rho * (Bi[0][d2] * gh[mm][d1][0] + Bi[1][d2] * gh[mm][d1][1] +
+                                         Bi[2][d2] * gh[mm][d1][2]);
+#pragma unroll
+      for (int c = 0; c < 10; ++c) {
+        double* r = rowA + 3 * m[2 + c];
+#pragma unroll
+        for (int d1 = 0; d1 < 3; ++d1)
+#pragma unroll
+          for (int d2 = 0; d2 < 3; ++d2) {
+            T w = 0;
+#pragma unroll
+            for (int mm = 0; mm < 4; ++mm)
+              if (NT::L[mm][c] != 0.0) w += gw[mm][d1][d2] * (T)NT::L[mm][c];
+            if (d1 == d2) w += S[c];
+            atomicAdd(r + d1 * 3 * deg + d2, (double)w * adet);
+          }
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 10; ++c) {
+        double v = (double)S[c] * adet;
+        double* r = rowA + 3 * m[2 + c];
+        atomicAdd(r + 0 * 3 * deg + 0, v);
+        atomicAdd(r + 1 * 3 * deg + 1, v);
+        atomicAdd(r + 2 * 3 * deg + 2, v);
+      }
+    }
+    if (rowBT) {
+      // B^T entries of row (r, d1): -|det| sum_ax Bi[ax][d1] RV[ax][m]
+#pragma unroll
+      for (int mm = 0; mm < 4; ++mm)
+#pragma unroll
+        for (int d1 = 0; d1 < 3; ++d1) {
+          T v = 0;
+#pragma unroll
+          for (int ax = 0; ax < 3; ++ax)
+            if (NT::RV[ax][mm] != 0.0) v += Bi[ax][d1] * (T)NT::RV[ax][mm];
+          atomicAdd(rowBT + d1 * pdeg + m[12 + mm], -(double)v * adet);
+        }
+    }
+  }
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_ns_vel_rows(NsArgs a) {
+  extern __shared__ double lds[];
+  const int ch = blockIdx.x;
+  const int n0 = a.chunk_node[ch], n1 = a.chunk_node[ch + 1];
+  const int q0 = a.chunk_pair[ch], q1 = a.chunk_pair[ch + 1], qs = a.chunk_split[ch];
+  const int64_t na0 = a.nptr[n0], na1 = a.nptr[n1];
+  const int aLen = (int)(9 * (na1 - na0));
+  int64_t nb0 = 0;
+  int btLen = 0;
+  if (a.with_bt) {
+    nb0 = a.pptr[n0];
+    btLen = (int)(3 * (a.pptr[n1] - nb0));
+  }
+  for (int k = threadIdx.x; k < aLen + btLen; k += blockDim.x) lds[k] = 0.0;
+  __syncthreads();
+  for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+    const int32_t e = a.pair_elem[q];
+    const uint4 mv = a.pair_meta[q];
+    const uint8_t* m = reinterpret_cast<const uint8_t*>(&mv);
+    const int i = n0 + m[1];
+    const int64_t ai = a.nptr[i];
+    const int deg = (int)(a.nptr[i + 1] - ai);
+    double* rowA = lds + 9 * (ai - na0);
+    double* rowBT = nullptr;
+    int pdeg = 0;
+    if (a.with_bt) {
+      const int64_t bi = a.pptr[i];
+      pdeg = (int)(a.pptr[i + 1] - bi);
+      rowBT = lds + aLen + 3 * (bi - nb0);
+    }
+    if (q < qs)
+      VelRow<T, 0>::run(a, e, m, rowA, deg, rowBT, pdeg);
+    else
+      VelRow<T, 1>::run(a, e, m, rowA, deg, rowBT, pdeg);
+  }
+  __syncthreads();
+  double* outA = a.valsA + 9 * na0;
+  for (int k = threadIdx.x; k < aLen; k += blockDim.x) __builtin_nontemporal_store(lds[k], outA + k);
+  if (a.with_bt) {
+    double* outB = a.valsBT + 3 * nb0;
+    for (int k = threadIdx.x; k < btLen; k += blockDim.x)
+      __builtin_nontemporal_store(lds[aLen + k], outB + k);
+  }
+}
+
+// Pressure rows of B (1,0): row p of element matrix rows 30+m (already scaled by -1).
+template <typename T>
+__global__ __launch_bounds__(256) void k_ns_pres_rows(NsArgs a) {
+  extern __shared__ double lds[];
+  const int ch = blockIdx.x;
+  const int n0 = a.chunk_node[ch], n1 = a.chunk_node[ch + 1];
+  const int q0 = a.chunk_pair[ch], q1 = a.chunk_pair[ch + 1];
+  const int64_t nb0 = a.bptr[n0];
+  const int len = (int)(3 * (a.bptr[n1] - nb0));
+  for (int k = threadIdx.x; k < len; k += blockDim.x) lds[k] = 0.0;
+  __syncthreads();
+  for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+    const int32_t e = a.pair_elem[q];
+    const uint4 mv = a.pair_meta[q];
+    const uint8_t* m = reinterpret_cast<const uint8_t*>(&mv);
+    const int li = m[0];
+    const int pn = n0 + m[1];
+    int32_t nodes[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) nodes[c] = a.conn[(int64_t)c_perm2[li][c] * a.nelem + e];
+    Geo g;
+    geometry(a.xyz, a.npts, nodes, g);
+    double* row = lds + 3 * (a.bptr[pn] - nb0);
+#pragma unroll
+    for (int c = 0; c < 10; ++c)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        T v = 0;
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax)
+          if (ft::RP[ax][c] != 0.0) v += (T)g.Bi[ax][d] * (T)ft::RP[ax][c];
+        atomicAdd(row + 3 * m[2 + c] + d, -(double)v * g.adet);
+      }
+  }
+  __syncthreads();
+  double* out = a.valsB + 3 * nb0;
+  for (int k = threadIdx.x; k < len; k += blockDim.x) __builtin_nontemporal_store(lds[k], out + k);
+}
+
+// Residual (assembleRHS, :200-224): velocity rows r_(i,d) = c00 sum_j S_ij u_jd
+// - c01 sum_m B_m,(i,d) p_m ; pressure rows r_m = -c10 sum_(j,d) B_m,(j,d) u_jd.
+template <typename T, int CLS>
+__device__ __forceinline__ void res_row(const NsArgs& a, int32_t e, int li, double r[3]) {
+  using NT = NsTensors<CLS>;
+  int32_t nodes[10];
+#pragma unroll
+  for (int c = 0; c < 10; ++c) nodes[c] = a.conn[(int64_t)c_perm2[li][c] * a.nelem + e];
+  Geo g;
+  geometry(a.xyz, a.npts, nodes, g);
+  T Bi[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Bi[i][j] = (T)g.Bi[i][j];
+  T Gs[6];
+  metric<T>(Bi, Gs);
+  T S[10];
+#pragma unroll
+  for (int c = 0; c < 10; ++c) {
+    T acc = 0;
+#pragma unroll
+    for (int s = 0; s < 6; ++s)
+      if (NT::KS[s][c] != 0.0) acc += Gs[s] * (T)NT::KS[s][c];
+    S[c] = (T)a.nurho * acc;
+  }
+  T uk[10][3];
+#pragma unroll
+  for (int k = 0; k < 10; ++k)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) uk[k][d] = (T)a.u[3 * (int64_t)nodes[k] + d];
+#pragma unroll
+  for (int k = 0; k < 10; ++k) {
+    T ut[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) ut[ax] = Bi[ax][0] * uk[k][0] + Bi[ax][1] * uk[k][1] + Bi[ax][2] * uk[k][2];
+#pragma unroll
+    for (int c = 0; c < 10; ++c)
+#pragma unroll
+      for (int ax = 0; ax < 3; ++ax)
+        if (NT::TT[ax][k][c] != 0.0) S[c] += (T)a.rho * ut[ax] * (T)NT::TT[ax][k][c];
+  }
+  T ru[3] = {0, 0, 0};
+#pragma unroll
+  for (int c = 0; c < 10; ++c)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) ru[d] += S[c] * uk[c][d];
+  T rp[3] = {0, 0, 0};
+#pragma unroll
+  for (int mm = 0; mm < 4; ++mm) {
+    const T pm = (T)a.p[a.pconn[(int64_t)c_perm2[li][mm] * a.nelem + e]];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      T v = 0;
+#pragma unroll
+      for (int ax = 0; ax < 3; ++ax)
+        if (NT::RV[ax][mm] != 0.0) v += Bi[ax][d] * (T)NT::RV[ax][mm];
+      rp[d] -= v * pm;
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) r[d] = g.adet * ((double)((T)a.coeff[0] * ru[d]) + (double)((T)a.coeff[1] * rp[d]));
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_ns_res_vel(NsArgs a) {
+  extern __shared__ double lds[];
+  const int ch = blockIdx.x;
+  const int n0 = a.chunk_node[ch], n1 = a.chunk_node[ch + 1];
+  const int q0 = a.chunk_pair[ch], q1 = a.chunk_pair[ch + 1], qs = a.chunk_split[ch];
+  const int len = 3 * (n1 - n0);
+  for (int k = threadIdx.x; k < len; k += blockDim.x) lds[k] = 0.0;
+  __syncthreads();
+  for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+    const int32_t e = a.pair_elem[q];
+    const uint4 mv = a.pair_meta[q];
+    const uint8_t* m = reinterpret_cast<const uint8_t*>(&mv);
+    double r[3];
+    if (q < qs)
+      res_row<T, 0>(a, e, m[0], r);
+    else
+      res_row<T, 1>(a, e, m[0], r);
+    double* dst = lds + 3 * m[1];
+    atomicAdd(dst + 0, r[0]);
+    atomicAdd(dst + 1, r[1]);
+    atomicAdd(dst + 2, r[2]);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < len; k += blockDim.x) a.res_u[3 * (int64_t)n0 + k] = lds[k];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_ns_res_pres(NsArgs a) {
+  extern __shared__ double lds[];
+  const int ch = blockIdx.x;
+  const int n0 = a.chunk_node[ch], n1 = a.chunk_node[ch + 1];
+  const int q0 = a.chunk_pair[ch], q1 = a.chunk_pair[ch + 1];
+  const int len = n1 - n0;
+  for (int k = threadIdx.x; k < len; k += blockDim.x) lds[k] = 0.0;
+  __syncthreads();
+  for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+    const int32_t e = a.pair_elem[q];
+    const uint4 mv = a.pair_meta[q];
+    const uint8_t* m = reinterpret_cast<const uint8_t*>(&mv);
+    const int li = m[0];
+    int32_t nodes[10];
+#pragma unroll
+    for (int c = 0; c < 10; ++c) nodes[c] = a.conn[(int64_t)c_perm2[li][c] * a.nelem + e];
+    Geo g;
+    geometry(a.xyz, a.npts, nodes, g);
+    T acc = 0;
+#pragma unroll
+    for (int c = 0; c < 10; ++c)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        T v = 0;
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax)
+          if (ft::RP[ax][c] != 0.0) v += (T)g.Bi[ax][d] * (T)ft::RP[ax][c];
+        acc += v * (T)a.u[3 * (int64_t)nodes[c] + d];
+      }
+    atomicAdd(lds + m[1], -(double)((T)a.coeff[2] * acc) * g.adet);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < len; k += blockDim.x) a.res_p[n0 + k] = lds[k];
+}
+
+// ---------------------------------------------------------------------------
+// Scalar Laplace (FE::assemblyLaplace, FE_def.hpp:2437-2500) and load vector
+// (FE::assemblyRHS, FE_def.hpp:7303-7395), P1 or P2.
+// ---------------------------------------------------------------------------
+struct LapArgs {
+  const double* xyz;
+  int64_t npts;
+  const int32_t* conn;
+  int64_t nelem;
+  int npe;
+  const int64_t* nptr;
+  const int32_t* chunk_node;
+  const int32_t* chunk_pair;
+  const int32_t* chunk_split;
+  const int32_t* pair_elem;
+  const uint4* pair_meta;
+  double* vals;
+  double* rhs;
+  double f;
+};
+
+template <typename T, int CLS>
+__device__ __forceinline__ void lap_row(const LapArgs& a, int32_t e, const uint8_t* m,
+                                        double* __restrict__ row) {
+  const int li = m[0];
+  int32_t nodes[4];
+  if (a.npe == 4) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) nodes[c] = a.conn[(int64_t)c_perm1[li][c] * a.nelem + e];
+  } else {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) nodes[c] = a.conn[(int64_t)c_perm2[li][c] * a.nelem + e];
+  }
+  Geo g;
+  geometry(a.xyz, a.npts, nodes, g);
+  T Bi[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Bi[i][j] = (T)g.Bi[i][j];
+  T Gs[6];
+  metric<T>(Bi, Gs);
+  if (a.npe == 4) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      T acc = 0;
+#pragma unroll
+      for (int s = 0; s < 6; ++s)
+        if (ft::KS1[s][c] != 0.0) acc += Gs[s] * (T)ft::KS1[s][c];
+      atomicAdd(row + m[2 + c], (double)acc * g.adet);
+    }
+  } else {
+    using NT = NsTensors<CLS>;
+#pragma unroll
+    for (int c = 0; c < 10; ++c) {
+      T acc = 0;
+#pragma unroll
+      for (int s = 0; s < 6; ++s)
+        if (NT::KS[s][c] != 0.0) acc += Gs[s] * (T)NT::KS[s][c];
+      atomicAdd(row + m[2 + c], (double)acc * g.adet);
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_laplace_rows(LapArgs a) {
+  extern __shared__ double lds[];
+  const int ch = blockIdx.x;
+  const int n0 = a.chunk_node[ch], n1 = a.chunk_node[ch + 1];
+  const int q0 = a.chunk_pair[ch], q1 = a.chunk_pair[ch + 1], qs = a.chunk_split[ch];
+  const int64_t na0 = a.nptr[n0];
+  const int len = (int)(a.nptr[n1] - na0);
+  for (int k = threadIdx.x; k < len; k += blockDim.x) lds[k] = 0.0;
+  __syncthreads();
+  for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+    const int32_t e = a.pair_elem[q];
+    const uint4 mv = a.pair_meta[q];
+    const uint8_t* m = reinterpret_cast<const uint8_t*>(&mv);
+    double* row = lds + (a.nptr[n0 + m[1]] - na0);
+    if (q < qs)
+      lap_row<T, 0>(a, e, m, row);
+    else
+      lap_row<T, 1>(a, e, m, row);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < len; k += blockDim.x) a.vals[na0 + k] = lds[k];
+}
+
+__global__ __launch_bounds__(256) void k_rhs_rows(LapArgs a) {
+  extern __shared__ double lds[];
+  const int ch = blockIdx.x;
+  const int n0 = a.chunk_node[ch], n1 = a.chunk_node[ch + 1];
+  const int q0 = a.chunk_pair[ch], q1 = a.chunk_pair[ch + 1];
+  const int len = n1 - n0;
+  for (int k = threadIdx.x; k < len; k += blockDim.x) lds[k] = 0.0;
+  __syncthreads();
+  for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+    const int32_t e = a.pair_elem[q];
+    const uint4 mv = a.pair_meta[q];
+    const uint8_t* m = reinterpret_cast<const uint8_t*>(&mv);
+    const int li = m[0];
+    int32_t nodes[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      nodes[c] = a.conn[(int64_t)(a.npe == 4 ? c_perm1[li][c] : c_perm2[li][c]) * a.nelem + e];
+    Geo g;
+    geometry(a.xyz, a.npts, nodes, g);
+    // int phi_r over the reference tet (exact for the 1-pt P1 / 5-pt P2 rules)
+    const double mv0 = a.npe == 4 ? ft::MV1[0] : (li < 4 ? ft::MV2[0] : ft::MV2[4]);
+    atomicAdd(lds + m[1], mv0 * a.f * g.adet);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < len; k += blockDim.x) a.rhs[n0 + k] = lds[k];
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static NsArgs ns_args(Ctx* c, const fedd_ns_params* p, const GatherPlan& pl) {
+  NsArgs a{};
+  a.xyz = c->mesh[0].d_xyz.as<double>();
+  a.npts = c->mesh[0].npts;
+  a.conn = c->mesh[0].d_conn.as<int32_t>();
+  a.pconn = c->mesh[1].d_conn.as<int32_t>();
+  a.nelem = c->mesh[0].nelem;
+  a.u = c->vec[FEDD_VEC_U].as<double>();
+  a.p = c->vec[FEDD_VEC_P].as<double>();
+  a.nptr = c->d_nptr.as<int64_t>();
+  a.pptr = c->d_pptr.as<int64_t>();
+  a.bptr = c->d_bptr.as<int64_t>();
+  a.chunk_node = pl.d_chunk_node.as<int32_t>();
+  a.chunk_pair = pl.d_chunk_pair.as<int32_t>();
+  a.chunk_split = pl.d_chunk_split.as<int32_t>();
+  a.pair_elem = pl.d_pair_elem.as<int32_t>();
+  a.pair_meta = pl.d_pair_meta.as<uint4>();
+  a.valsA = c->blk[0].d_vals.as<double>();
+  a.valsBT = c->blk[1].d_vals.as<double>();
+  a.valsB = c->blk[2].d_vals.as<double>();
+  a.res_u = c->vec[FEDD_VEC_RES_U].as<double>();
+  a.res_p = c->vec[FEDD_VEC_RES_P].as<double>();
+  a.nurho = p->viscosity * p->density;
+  a.rho = p->density;
+  for (int k = 0; k < 4; ++k) a.coeff[k] = p->coeff[k];
+  a.newton = p->newton ? 1 : 0;
+  return a;
+}
+
+int launch_ns_assembly(Ctx* c, const fedd_ns_params* p, int mode, int reassemble, int precision) {
+  const bool f32 = precision == FEDD_FP32_LOCAL;
+  if (mode == FEDD_NS_JACOBIAN || mode == FEDD_NS_FIXEDPOINT) {
+    fedd_ns_params q = *p;
+    if (mode == FEDD_NS_FIXEDPOINT) q.newton = 0;  // the fixed-point matrix never holds W (:115-146)
+    NsArgs a = ns_args(c, &q, c->vplan);
+    a.with_bt = reassemble ? 0 : 1;
+    const size_t lds = (size_t)c->vplan.lds_doubles * 8;
+    ktimer_begin(c);
+    if (f32)
+      k_ns_vel_rows<float><<<(unsigned)c->vplan.nchunks, 256, lds, c->stream>>>(a);
+    else
+      k_ns_vel_rows<double><<<(unsigned)c->vplan.nchunks, 256, lds, c->stream>>>(a);
+    FEDD_HIP(c, hipGetLastError());
+    ktimer_end(c, "ns_vel_rows");
+    if (!reassemble) {
+      NsArgs b = ns_args(c, &q, c->pplan);
+      const size_t ldsb = (size_t)c->pplan.lds_doubles * 8;
+      ktimer_begin(c);
+      if (f32)
+        k_ns_pres_rows<float><<<(unsigned)c->pplan.nchunks, 256, ldsb, c->stream>>>(b);
+      else
+        k_ns_pres_rows<double><<<(unsigned)c->pplan.nchunks, 256, ldsb, c->stream>>>(b);
+      FEDD_HIP(c, hipGetLastError());
+      ktimer_end(c, "ns_pres_rows");
+    }
+  } else {
+    NsArgs a = ns_args(c, p, c->vplan);
+    ktimer_begin(c);
+    if (f32)
+      k_ns_res_vel<float><<<(unsigned)c->vplan.nchunks, 256, 3 * 256 * 8, c->stream>>>(a);
+    else
+      k_ns_res_vel<double><<<(unsigned)c->vplan.nchunks, 256, 3 * 256 * 8, c->stream>>>(a);
+    FEDD_HIP(c, hipGetLastError());
+    NsArgs b = ns_args(c, p, c->pplan);
+    if (f32)
+      k_ns_res_pres<float><<<(unsigned)c->pplan.nchunks, 256, 256 * 8, c->stream>>>(b);
+    else
+      k_ns_res_pres<double><<<(unsigned)c->pplan.nchunks, 256, 256 * 8, c->stream>>>(b);
+    FEDD_HIP(c, hipGetLastError());
+    ktimer_end(c, "ns_residual");
+  }
+  return FEDD_OK;
+}
+
+static LapArgs lap_args(Ctx* c) {
+  LapArgs a{};
+  const GatherPlan& pl = c->vplan;
+  a.xyz = c->mesh[0].d_xyz.as<double>();
+  a.npts = c->mesh[0].npts;
+  a.conn = c->mesh[0].d_conn.as<int32_t>();
+  a.nelem = c->mesh[0].nelem;
+  a.npe = c->mesh[0].npe;
+  a.nptr = c->d_nptr.as<int64_t>();
+  a.chunk_node = pl.d_chunk_node.as<int32_t>();
+  a.chunk_pair = pl.d_chunk_pair.as<int32_t>();
+  a.chunk_split = pl.d_chunk_split.as<int32_t>();
+  a.pair_elem = pl.d_pair_elem.as<int32_t>();
+  a.pair_meta = pl.d_pair_meta.as<uint4>();
+  a.vals = c->blk[0].d_vals.as<double>();
+  a.rhs = c->vec[FEDD_VEC_RHS].as<double>();
+  return a;
+}
+
+int launch_laplace(Ctx* c, int precision) {
+  LapArgs a = lap_args(c);
+  const size_t lds = (size_t)c->vplan.lds_doubles * 8;
+  ktimer_begin(c);
+  if (precision == FEDD_FP32_LOCAL)
+    k_laplace_rows<float><<<(unsigned)c->vplan.nchunks, 256, lds, c->stream>>>(a);
+  else
+    k_laplace_rows<double><<<(unsigned)c->vplan.nchunks, 256, lds, c->stream>>>(a);
+  FEDD_HIP(c, hipGetLastError());
+  ktimer_end(c, "laplace_rows");
+  return FEDD_OK;
+}
+
+int launch_rhs(Ctx* c, double f) {
+  LapArgs a = lap_args(c);
+  a.f = f;
+  ktimer_begin(c);
+  k_rhs_rows<<<(unsigned)c->vplan.nchunks, 256, 256 * 8, c->stream>>>(a);
+  FEDD_HIP(c, hipGetLastError());
+  ktimer_end(c, "rhs_rows");
+  return FEDD_OK;
+}
+
+}  // namespace fedd

@@ -1,0 +1,252 @@
+"""ctypes binding of libfeddgpu.so (include/feddgpu.h).
+
+The library is the product: this module only moves host arrays across the C ABI.
+It raises when the library is missing -- there is no fallback path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfeddgpu.so")
+
+FEDD_OK = 0
+MESH_PRIMARY, MESH_PRESSURE = 0, 1
+GRAPH_SCALAR, GRAPH_VECTOR, GRAPH_NS = 0, 1, 2
+BLOCK_A, BLOCK_BT, BLOCK_B, BLOCK_C = 0, 1, 2, 3
+VEC_U, VEC_P, VEC_RES_U, VEC_RES_P, VEC_RHS = 0, 1, 2, 3, 4
+FP64, FP32_LOCAL = 0, 1
+NS_JACOBIAN, NS_FIXEDPOINT, NS_RHS = 0, 1, 2
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_lp = ctypes.POINTER(ctypes.c_int64)
+_vp = ctypes.c_void_p
+
+
+class NsParams(ctypes.Structure):
+    _fields_ = [("viscosity", ctypes.c_double), ("density", ctypes.c_double),
+                ("newton", ctypes.c_int), ("coeff", ctypes.c_double * 4)]
+
+
+class FeddError(RuntimeError):
+    """Error raised by a libfeddgpu call (std::logic_error/runtime_error in FEDDLib)."""
+
+    def __init__(self, code, msg):
+        super().__init__("fedd error %d: %s" % (code, msg))
+        self.code = code
+
+
+_SIGS = {
+    "fedd_device_count": (ctypes.c_int, [_ip]),
+    "fedd_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
+    "fedd_ctx_destroy": (None, [_vp]),
+    "fedd_last_error": (ctypes.c_char_p, [_vp]),
+    "fedd_synchronize": (ctypes.c_int, [_vp]),
+    "fedd_mesh_structured": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _ip, _ip]),
+    "fedd_mesh_correct_orientation": (ctypes.c_int, [_dp, ctypes.c_int64, _ip, ctypes.c_int64,
+                                                      ctypes.c_int, _lp]),
+    "fedd_mesh_upload": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.c_int64, _ip, ctypes.c_int64,
+                                        ctypes.c_int]),
+    "fedd_graph_build": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "fedd_block_info": (ctypes.c_int, [_vp, ctypes.c_int, _lp, _lp, _lp]),
+    "fedd_block_graph": (ctypes.c_int, [_vp, ctypes.c_int, _lp, _ip]),
+    "fedd_block_values": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "fedd_block_device_values": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_dp)]),
+    "fedd_colour_build": (ctypes.c_int, [_vp, _ip]),
+    "fedd_colour_export": (ctypes.c_int, [_vp, _lp, _ip]),
+    "fedd_vector_upload": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.c_int64]),
+    "fedd_vector_download": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.c_int64]),
+    "fedd_vector_device_ptr": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_dp), _lp]),
+    "fedd_assembly_laplace": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "fedd_assembly_rhs": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "fedd_assembly_navier_stokes": (ctypes.c_int, [_vp, ctypes.POINTER(NsParams), ctypes.c_int,
+                                                   ctypes.c_int, ctypes.c_int]),
+    "fedd_newton_step": (ctypes.c_int, [_vp, _ip]),
+    "fedd_event_record": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "fedd_event_elapsed": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_float)]),
+    "fedd_kernel_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "fedd_kernel_time": (ctypes.c_int, [_vp, ctypes.c_char_p, _dp, _lp]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libfeddgpu.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libfeddgpu.so is not built; run `python -m feddlib_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def declared_symbols():
+    """Names of the entry points declared in include/feddgpu.h."""
+    import re
+    hdr = os.path.join(os.path.dirname(HERE), "include", "feddgpu.h")
+    txt = open(hdr).read()
+    return sorted(set(re.findall(r"\b(fedd_[a-z0-9_]+)\s*\(", txt)))
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def mesh_structured(M, fe):
+    """Unit-cube Kuhn mesh (MeshStructured::buildMesh3D): (pts[n,3], bc[n], elems[E,k])."""
+    P = M + 1 if fe == 1 else 2 * M + 1
+    n, E, k = P ** 3, 6 * M ** 3, 4 if fe == 1 else 10
+    pts = np.zeros((n, 3))
+    bc = np.zeros(n, np.int32)
+    el = np.zeros((E, k), np.int32)
+    rc = lib().fedd_mesh_structured(M, fe, _p(pts, _dp), _p(bc, _ip), _p(el, _ip))
+    if rc:
+        raise FeddError(rc, "fedd_mesh_structured")
+    return pts, bc, el
+
+
+def correct_orientation(pts, elems):
+    """Mesh::correctElementOrientation in place; returns the number of flipped elements."""
+    assert elems.flags.c_contiguous and elems.dtype == np.int32
+    f = ctypes.c_int64()
+    rc = lib().fedd_mesh_correct_orientation(_p(np.ascontiguousarray(pts, np.float64), _dp),
+                                             pts.shape[0], _p(elems, _ip), elems.shape[0],
+                                             elems.shape[1], ctypes.byref(f))
+    if rc:
+        raise FeddError(rc, "fedd_mesh_correct_orientation")
+    return f.value
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    lib().fedd_device_count(ctypes.byref(n))
+    return n.value
+
+
+class Context:
+    """One libfeddgpu context (device buffers + HIP stream)."""
+
+    def __init__(self, device=0):
+        self._L = lib()
+        h = _vp()
+        rc = self._L.fedd_ctx_create(device, ctypes.byref(h))
+        if rc:
+            raise FeddError(rc, "fedd_ctx_create failed (no HIP device?)")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self._L.fedd_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc):
+        if rc:
+            raise FeddError(rc, self._L.fedd_last_error(self.h).decode())
+
+    # mesh / graph
+    def mesh_upload(self, slot, pts, elems):
+        pts = np.ascontiguousarray(pts, np.float64)
+        elems = np.ascontiguousarray(elems, np.int32)
+        self._chk(self._L.fedd_mesh_upload(self.h, slot, _p(pts, _dp), pts.shape[0], _p(elems, _ip),
+                                           elems.shape[0], elems.shape[1]))
+
+    def graph_build(self, kind):
+        self._chk(self._L.fedd_graph_build(self.h, kind))
+
+    def block_info(self, block):
+        r, c, n = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        self._chk(self._L.fedd_block_info(self.h, block, ctypes.byref(r), ctypes.byref(c), ctypes.byref(n)))
+        return r.value, c.value, n.value
+
+    def block_graph(self, block):
+        nr, _, nnz = self.block_info(block)
+        rp = np.zeros(nr + 1, np.int64)
+        ci = np.zeros(max(nnz, 1), np.int32)
+        self._chk(self._L.fedd_block_graph(self.h, block, _p(rp, _lp), _p(ci, _ip)))
+        return rp, ci[:nnz]
+
+    def block_values(self, block):
+        _, _, nnz = self.block_info(block)
+        v = np.zeros(max(nnz, 1))
+        self._chk(self._L.fedd_block_values(self.h, block, _p(v, _dp)))
+        return v[:nnz]
+
+    def colour_build(self):
+        n = ctypes.c_int()
+        self._chk(self._L.fedd_colour_build(self.h, ctypes.byref(n)))
+        return n.value
+
+    def colour_export(self, ncolours, nelem):
+        cp = np.zeros(ncolours + 1, np.int64)
+        perm = np.zeros(nelem, np.int32)
+        self._chk(self._L.fedd_colour_export(self.h, _p(cp, _lp), _p(perm, _ip)))
+        return cp, perm
+
+    # vectors
+    def vector_upload(self, which, a):
+        a = np.ascontiguousarray(a, np.float64).ravel()
+        self._chk(self._L.fedd_vector_upload(self.h, which, _p(a, _dp), a.size))
+
+    def vector_download(self, which, n):
+        a = np.zeros(n)
+        self._chk(self._L.fedd_vector_download(self.h, which, _p(a, _dp), n))
+        return a
+
+    # assembly
+    def assembly_laplace(self, precision=FP64):
+        self._chk(self._L.fedd_assembly_laplace(self.h, precision))
+
+    def assembly_rhs(self, f):
+        self._chk(self._L.fedd_assembly_rhs(self.h, f))
+
+    def assembly_navier_stokes(self, viscosity, density, newton, mode, reassemble, precision=FP64,
+                               coeff=(1.0, 1.0, 1.0, 1.0)):
+        p = NsParams(viscosity, density, 1 if newton else 0, (ctypes.c_double * 4)(*coeff))
+        self._chk(self._L.fedd_assembly_navier_stokes(self.h, ctypes.byref(p), mode, 1 if reassemble else 0,
+                                                      precision))
+
+    def newton_step(self):
+        s = ctypes.c_int()
+        self._chk(self._L.fedd_newton_step(self.h, ctypes.byref(s)))
+        return s.value
+
+    # timing
+    def synchronize(self):
+        self._chk(self._L.fedd_synchronize(self.h))
+
+    def event_record(self, slot):
+        self._chk(self._L.fedd_event_record(self.h, slot))
+
+    def event_elapsed(self, a, b):
+        ms = ctypes.c_float()
+        self._chk(self._L.fedd_event_elapsed(self.h, a, b, ctypes.byref(ms)))
+        return ms.value
+
+    def kernel_timing(self, enable):
+        self._chk(self._L.fedd_kernel_timing(self.h, 1 if enable else 0))
+
+    def kernel_time(self, name):
+        t, n = ctypes.c_double(), ctypes.c_int64()
+        self._chk(self._L.fedd_kernel_time(self.h, name.encode(), ctypes.byref(t), ctypes.byref(n)))
+        return t.value, n.value

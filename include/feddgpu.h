@@ -1,0 +1,169 @@
+/*
+ * feddgpu.h -- C ABI of the MI355X (gfx950) element-assembly engine.
+ *
+ * Drop-in for FEDDLib's element-assembly path: the FE::assembly* loops
+ * (feddlib/core/FE/FE_decl.hpp:181-186, 453-459, 475-498, 560-595) with their
+ * AssembleFE element plugins (feddlib/core/AceFemAssembly/AssembleFE_decl.hpp:61-299),
+ * writing into CRS matrices whose graph is identical to the Tpetra graph the
+ * reference builds (insertGlobalValues + fillComplete,
+ * feddlib/core/LinearAlgebra/Matrix_def.hpp:88-92, 192-199).
+ *
+ * Conventions
+ *   - Plain pointers and sizes only.  Host arrays are caller-owned and only read
+ *     during the call.  Device buffers are owned by the context.
+ *   - Node-major coordinates (x0 y0 z0 x1 ...), element-major connectivity in
+ *     FEDDLib local order; node ids are the one-rank GIDs (repeated map = unique
+ *     map), as MeshStructured::buildMesh3D produces them
+ *     (feddlib/core/Mesh/MeshStructured_def.hpp:703-994).
+ *   - Dof numbering dof = 3*node + d (Map::buildVecFieldMap,
+ *     feddlib/core/LinearAlgebra/Map_def.hpp:95-107).
+ *   - Every entry point returns FEDD_OK (0) or a negative error class; the
+ *     message is in fedd_last_error().  No exception crosses the ABI.  The
+ *     reference raises std::logic_error / std::runtime_error through
+ *     TEUCHOS_TEST_FOR_EXCEPTION for the same conditions.
+ *   - One context per device and host thread; calls are ordered on the
+ *     context's HIP stream; a context is not thread-safe.
+ */
+#ifndef FEDDGPU_H
+#define FEDDGPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fedd_ctx fedd_ctx;
+
+enum fedd_status {
+  FEDD_OK = 0,
+  FEDD_ERR_ARG = -1,      /* invalid argument (std::logic_error in the reference) */
+  FEDD_ERR_HIP = -2,      /* HIP runtime failure */
+  FEDD_ERR_STATE = -3,    /* call order violated (e.g. assembly before graph) */
+  FEDD_ERR_LIMIT = -4,    /* a documented size limit is exceeded */
+  FEDD_ERR_NODEVICE = -5  /* no HIP device */
+};
+
+enum fedd_mesh_slot { FEDD_MESH_PRIMARY = 0, FEDD_MESH_PRESSURE = 1 };
+
+enum fedd_graph_kind {
+  FEDD_GRAPH_SCALAR = 0,  /* one dof per node: FE::assemblyLaplace */
+  FEDD_GRAPH_VECTOR = 1,  /* three dofs per node, full 3x3 node blocks: addFeBlock */
+  FEDD_GRAPH_NS = 2       /* P2-P1: A(0,0), B^T(0,1), B(1,0), C(1,1): addFeBlockMatrix */
+};
+
+enum fedd_block {
+  FEDD_BLOCK_A = 0,   /* scalar Laplace, vector / elasticity, or NS A(0,0) */
+  FEDD_BLOCK_BT = 1,  /* NS (0,1) */
+  FEDD_BLOCK_B = 2,   /* NS (1,0) */
+  FEDD_BLOCK_C = 3    /* NS (1,1): empty graph for P2-P1 (|v| > 1e-13 rule) */
+};
+
+enum fedd_vector {
+  FEDD_VEC_U = 0,      /* velocity / displacement iterate, 3 per primary node */
+  FEDD_VEC_P = 1,      /* pressure iterate, 1 per pressure node */
+  FEDD_VEC_RES_U = 2,  /* residual, velocity / displacement block */
+  FEDD_VEC_RES_P = 3,  /* residual, pressure block */
+  FEDD_VEC_RHS = 4     /* load vector of FE::assemblyRHS (1 per primary node) */
+};
+
+enum fedd_precision {
+  FEDD_FP64 = 0,       /* everything in FP64 */
+  FEDD_FP32_LOCAL = 1  /* element-local products in FP32, geometry (Binv, |det|) and the
+                          global accumulation in FP64 */
+};
+
+enum fedd_ns_mode { FEDD_NS_JACOBIAN = 0, FEDD_NS_FIXEDPOINT = 1, FEDD_NS_RHS = 2 };
+
+/* Parameter keys of AssembleFENavierStokes (AssembleFENavierStokes_def.hpp:44-53):
+ * Parameter/Viscosity, Parameter/Density, General/Linearization ("Newton" -> 1,
+ * "FixedPoint" -> 0), and the 2x2 coeff of setCoeff (:57-64), row-major. */
+typedef struct {
+  double viscosity;
+  double density;
+  int newton;
+  double coeff[4];
+} fedd_ns_params;
+
+/* ---------------------------------------------------------------- context */
+int fedd_device_count(int* count);
+/* Creates a context on `device` with its own HIP stream. */
+int fedd_ctx_create(int device, fedd_ctx** ctx);
+void fedd_ctx_destroy(fedd_ctx* ctx);
+const char* fedd_last_error(const fedd_ctx* ctx);
+int fedd_synchronize(fedd_ctx* ctx);
+
+/* ------------------------------------------------------------------- mesh */
+/* MeshStructured::buildMesh3D (MeshStructured_def.hpp:622-1009) on one rank for the
+ * unit cube: fe = 1 (P1, (M+1)^3 points) or 2 (P2, (2M+1)^3 points), 6*M^3 tets.
+ * pts: npts*3, bc: npts (1 on the boundary, may be NULL), elems: nelem*(4|10). */
+int fedd_mesh_structured(int M, int fe, double* pts, int32_t* bc, int32_t* elems);
+/* Mesh::correctElementOrientation + flipElement (Mesh_def.hpp:543-578, 648-726). */
+int fedd_mesh_correct_orientation(const double* pts, int64_t npts, int32_t* elems, int64_t nelem,
+                                  int nodes_per_elem, int64_t* flipped);
+
+/* Domain::buildMesh / getPointsRepeated / getElementsC (Domain_def.hpp:210-280).
+ * slot PRIMARY: the P1 (scalar Laplace) or P2 (velocity, displacement) mesh;
+ * slot PRESSURE: the P1 pressure mesh of a P2-P1 pair, element T of which is the
+ * pressure element of velocity element T (FE_def.hpp:1099-1101, 1157).  Replaces
+ * the mesh and drops graphs built on it. */
+int fedd_mesh_upload(fedd_ctx* ctx, int slot, const double* pts, int64_t npts,
+                     const int32_t* elems, int64_t nelem, int nodes_per_elem);
+
+/* ------------------------------------------------------------------ graph */
+/* Builds the CRS graph(s) exactly as insertGlobalValues + fillComplete would
+ * (FE_def.hpp:1419-1442 addFeBlock, :982-1062 addFeBlockMatrix, :2470-2496
+ * assemblyLaplace), plus the node incidence and scatter indices the kernels use.
+ * Rows never hold more than 255 distinct neighbour nodes (FEDD_ERR_LIMIT). */
+int fedd_graph_build(fedd_ctx* ctx, int kind);
+int fedd_block_info(fedd_ctx* ctx, int block, int64_t* nrows, int64_t* ncols, int64_t* nnz);
+/* Host copies: rowptr has nrows+1 entries, colind nnz (local = global ids on one rank). */
+int fedd_block_graph(fedd_ctx* ctx, int block, int64_t* rowptr, int32_t* colind);
+int fedd_block_values(fedd_ctx* ctx, int block, double* values);
+/* Device pointer of the value array (owned by the context, valid until the graph is rebuilt). */
+int fedd_block_device_values(fedd_ctx* ctx, int block, double** d_values);
+
+/* ------------------------------------------------------------- colouring */
+/* Element colouring of the primary mesh: elements of one colour share no node.
+ * No counterpart in the reference (it assembles serially per rank); used by the
+ * element-parallel scatter kernel.  colour_ptr has ncolours+1 entries,
+ * elem_perm nelem entries (elements sorted by colour, ascending id). */
+int fedd_colour_build(fedd_ctx* ctx, int* ncolours);
+int fedd_colour_export(fedd_ctx* ctx, int64_t* colour_ptr, int32_t* elem_perm);
+
+/* ---------------------------------------------------------------- vectors */
+/* Repeated-map vectors (one rank: repeated = unique).  Upload copies host data
+ * to the context's device vector; download copies back (stream-synchronous). */
+int fedd_vector_upload(fedd_ctx* ctx, int which, const double* host, int64_t n);
+int fedd_vector_download(fedd_ctx* ctx, int which, double* host, int64_t n);
+int fedd_vector_device_ptr(fedd_ctx* ctx, int which, double** d_ptr, int64_t* n);
+
+/* --------------------------------------------------------------- assembly */
+/* FE::assemblyLaplace (FE_def.hpp:2437-2500) for P1/P2 on the primary mesh,
+ * rule from determineDegree(dim,FEType,FEType,Grad,Grad).  Needs GRAPH_SCALAR. */
+int fedd_assembly_laplace(fedd_ctx* ctx, int precision);
+/* FE::assemblyRHS (FE_def.hpp:7303-7395), scalar field, constant f: writes VEC_RHS. */
+int fedd_assembly_rhs(fedd_ctx* ctx, double f);
+/* FE::assemblyNavierStokes (FE_def.hpp:1078-1239) with AssembleFENavierStokes
+ * (AssembleFENavierStokes_def.hpp:67-224).  mode: FEDD_NS_*.  reassemble = 1 fills
+ * only A(0,0) (addFeBlock), 0 fills A, B^T, B (and the empty C) (addFeBlockMatrix).
+ * Jacobian / FixedPoint read VEC_U (and VEC_P) and overwrite the block values;
+ * Rhs writes VEC_RES_U and VEC_RES_P.  Needs GRAPH_NS. */
+int fedd_assembly_navier_stokes(fedd_ctx* ctx, const fedd_ns_params* params, int mode,
+                                int reassemble, int precision);
+/* AssembleFE::advanceNewtonStep / getNewtonStep (AssembleFE_decl.hpp:146) for the
+ * NS elements: the counter the reference keeps per element. */
+int fedd_newton_step(fedd_ctx* ctx, int* step);
+
+/* ----------------------------------------------------------------- timing */
+/* HIP events recorded on the context stream.  Kernel timing, when enabled,
+ * brackets every launch of the assembly kernels with events on that stream. */
+int fedd_event_record(fedd_ctx* ctx, int slot);           /* slot < 16 */
+int fedd_event_elapsed(fedd_ctx* ctx, int a, int b, float* ms);
+int fedd_kernel_timing(fedd_ctx* ctx, int enable);
+/* Accumulated time and launch count of the named kernel since enabling. */
+int fedd_kernel_time(fedd_ctx* ctx, const char* name, double* total_ms, int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FEDDGPU_H */
