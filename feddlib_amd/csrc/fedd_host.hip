@@ -206,7 +206,9 @@ int build_plan(Ctx* c, const Incidence& inc, int64_t nnodes, const Mesh& rows_me
       for (int64_t q = inc.ptr[i]; q < inc.ptr[i + 1]; ++q) {
         int li = inc.loc[q];
         int cls = (rows_mesh.npe == 10 && li >= 4) ? 1 : 0;
-        int64_t key = ((int64_t)cls << 40) | ((int64_t)rank[cls]++ << 20) | (i - n0);
+        // class first; inside a class the k-th pairs of all nodes follow each
+        // other, so neighbouring lanes add into different rows
+        int64_t key = ((int64_t)cls << 60) | ((int64_t)rank[cls]++ << 20) | (i - n0);
         order.push_back({key, i, q});
       }
     }
@@ -214,7 +216,7 @@ int build_plan(Ctx* c, const Incidence& inc, int64_t nnodes, const Mesh& rows_me
     int32_t split = -1;
     for (size_t k = 0; k < order.size(); ++k) {
       int64_t i = order[k][1], q = order[k][2];
-      int cls = (int)(order[k][0] >> 40);
+      int cls = (int)(order[k][0] >> 60);
       if (cls == 1 && split < 0) split = (int32_t)(pair_out);
       int32_t e = inc.elem[q];
       int li = inc.loc[q];
@@ -356,8 +358,11 @@ extern "C" int fedd_mesh_correct_orientation(const double* pts, int64_t npts, in
 
 struct fedd_ctx : public fedd::Ctx {};
 
-// LDS budget of one row-gather chunk, in doubles (40 KiB).
-static const int kLdsBudget = 5120;
+// LDS budget of one row-gather chunk, in doubles.
+#ifndef FEDD_LDS_BUDGET
+#define FEDD_LDS_BUDGET 5120
+#endif
+static const int kLdsBudget = FEDD_LDS_BUDGET;
 
 extern "C" {
 
@@ -378,6 +383,11 @@ int fedd_ctx_create(int device, fedd_ctx** out) {
   if (hipSetDevice(device) != hipSuccess) return FEDD_ERR_HIP;
   fedd_ctx* c = new fedd_ctx();
   c->device = device;
+  {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+      c->num_cu = prop.multiProcessorCount;
+  }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return FEDD_ERR_HIP;
@@ -431,9 +441,13 @@ int fedd_mesh_upload(fedd_ctx* c, int slot, const double* pts, int64_t npts,
   std::vector<int32_t> conn(nelem * npe);
   for (int64_t e = 0; e < nelem; ++e)
     for (int k = 0; k < npe; ++k) conn[k * nelem + e] = elems[e * npe + k];
+  std::vector<int32_t> conn16(nelem * 16, 0);
+  for (int64_t e = 0; e < nelem; ++e)
+    for (int k = 0; k < npe; ++k) conn16[e * 16 + k] = elems[e * npe + k];
   int rc;
   if ((rc = upload(c, m.d_xyz, soa.data(), soa.size() * 8))) return rc;
   if ((rc = upload(c, m.d_conn, conn.data(), conn.size() * 4))) return rc;
+  if ((rc = upload(c, m.d_conn16, conn16.data(), conn16.size() * 4))) return rc;
   FEDD_HIP(c, hipStreamSynchronize(c->stream));
   c->graph_kind = GK_NONE;
   c->colour_ptr.clear();
@@ -549,6 +563,10 @@ int fedd_graph_build(fedd_ctx* c, int kind) {
       c->graph_kind = GK_NONE;
       return rc;
     }
+  }
+  if ((rc = launch_geometry(c))) {
+    c->graph_kind = GK_NONE;
+    return rc;
   }
   FEDD_HIP(c, hipStreamSynchronize(c->stream));
   c->newton_step = 0;
@@ -731,6 +749,12 @@ int fedd_newton_step(fedd_ctx* c, int* step) {
   if (!c || !step) return FEDD_ERR_ARG;
   *step = c->newton_step;
   return FEDD_OK;
+}
+
+int fedd_stream_copy(fedd_ctx* c, int64_t bytes, int iters, float* ms_per_copy) {
+  if (!c || bytes < 16 || iters < 1) return FEDD_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  return launch_stream_copy(c, bytes, iters, ms_per_copy);
 }
 
 int fedd_event_record(fedd_ctx* c, int slot) {

@@ -37,6 +37,7 @@ struct Mesh {
   std::vector<int32_t> elems;  // host, element-major
   DevBuf d_xyz;                // SoA: x[npts] y[npts] z[npts]
   DevBuf d_conn;               // SoA: conn[k*nelem + e]
+  DevBuf d_conn16;             // AoS: conn16[e*16 + k], k < npe (one 64-byte line per element)
   bool valid() const { return npts > 0; }
 };
 
@@ -77,6 +78,7 @@ enum GraphKind { GK_NONE = -1 };
 
 struct Ctx {
   int device = 0;
+  int num_cu = 256;  // compute units of the device (hipDeviceProp_t::multiProcessorCount)
   hipStream_t stream = nullptr;
   std::string err;
   Mesh mesh[2];
@@ -85,6 +87,8 @@ struct Ctx {
   // node-level structures on the device
   DevBuf d_nptr, d_pptr, d_bptr;  // int64 node-level row pointers (A, B^T, B)
   GatherPlan vplan, pplan;        // velocity (primary) and pressure row gathers
+  DevBuf d_geo;                   // per element: grad lambda_0..3 (12) + |det| + pad, 16 doubles
+  DevBuf d_upad;                  // velocity iterate padded to 4 doubles per node
   // colouring
   std::vector<int64_t> colour_ptr;
   std::vector<int32_t> colour_perm;
@@ -130,5 +134,7 @@ int expand_graph(Ctx* c, int kind, const int32_t* aidx, const int32_t* btidx, co
 int launch_ns_assembly(Ctx* c, const fedd_ns_params* p, int mode, int reassemble, int precision);
 int launch_laplace(Ctx* c, int precision);
 int launch_rhs(Ctx* c, double f);
+int launch_stream_copy(Ctx* c, int64_t bytes, int iters, float* ms);
+int launch_geometry(Ctx* c);
 
 }  // namespace fedd

@@ -21,12 +21,23 @@
 #include "fedd_perm.h"
 #include "fedd_tensors.h"
 
+#include <algorithm>
+
+#ifndef FEDD_ROW_BLOCK
+#define FEDD_ROW_BLOCK 256
+#endif
+#ifndef FEDD_ROW_WAVES
+#define FEDD_ROW_WAVES 2
+#endif
+
 namespace fedd {
 
 __constant__ int8_t c_perm2[10][10] = FEDD_PERM_P2_INIT;
 __constant__ int8_t c_perm1[4][4] = FEDD_PERM_P1_INIT;
 
 namespace ft = fedd_tensors;
+
+typedef double dbl2 __attribute__((ext_vector_type(2)));
 
 // ---------------------------------------------------------------------------
 // CRS expansion from node adjacency
@@ -205,6 +216,9 @@ struct NsArgs {
   int64_t npts;
   const int32_t* conn;     // P2 connectivity SoA
   const int32_t* pconn;    // P1 connectivity SoA
+  const int32_t* conn16;   // P2 connectivity AoS, 16 ints per element
+  const double* geo;       // element geometry cache (16 doubles per element)
+  const double* upad;      // velocity padded to 4 doubles per node
   int64_t nelem;
   const double* u;         // velocity iterate, 3 per node
   const double* p;         // pressure iterate
@@ -225,6 +239,7 @@ struct NsArgs {
   double* res_u;
   double* res_p;
   // parameters
+  int64_t nchunks;
   double nurho, rho;
   double coeff[4];
   int newton;
@@ -234,95 +249,102 @@ struct NsArgs {
 template <typename T, int CLS>
 struct VelRow {
   // Computes the canonical row of one pair and adds it into LDS.
+  //   e      element, m = pair meta (loc, node-in-chunk, 10 column slots, 4 pressure slots)
+  //   Binv'  rows grad lambda_{perm[1..3]} of the cached element geometry
+  //   u      the ten canonical nodes' velocities (padded copy, 2 loads per node)
   static __device__ __forceinline__ void run(const NsArgs& a, int32_t e, const uint8_t* m,
                                              double* __restrict__ rowA, int deg,
                                              double* __restrict__ rowBT, int pdeg) {
     using NT = NsTensors<CLS>;
     const int li = m[0];
+    const int8_t* pm = c_perm2[li];
+    const int32_t* cn = a.conn16 + 16 * (int64_t)e;
+    const double* ge = a.geo + 16 * (int64_t)e;
     int32_t nodes[10];
 #pragma unroll
-    for (int c = 0; c < 10; ++c) nodes[c] = a.conn[(int64_t)c_perm2[li][c] * a.nelem + e];
-    Geo g;
-    geometry(a.xyz, a.npts, nodes, g);
+    for (int c = 0; c < 10; ++c) nodes[c] = cn[pm[c]];
     T Bi[3][3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+    for (int ax = 0; ax < 3; ++ax) {
+      const double* gr = ge + 3 * pm[ax + 1];
 #pragma unroll
-      for (int j = 0; j < 3; ++j) Bi[i][j] = (T)g.Bi[i][j];
-    T Gs[6];
-    metric<T>(Bi, Gs);
-    // S_c = nu rho A_rc + rho N_rc   (without |det|)
-    T S[10];
-#pragma unroll
-    for (int c = 0; c < 10; ++c) {
-      T acc = 0;
-#pragma unroll
-      for (int s = 0; s < 6; ++s)
-        if (NT::KS[s][c] != 0.0) acc += Gs[s] * (T)NT::KS[s][c];
-      S[c] = (T)a.nurho * acc;
+      for (int d = 0; d < 3; ++d) Bi[ax][d] = (T)gr[d];
     }
-    // grad-hat u at the vertices (P1 representation of grad u): gh[m][d1][axis]
-    T gh[4][3][3];
-#pragma unroll
-    for (int mm = 0; mm < 4; ++mm)
-#pragma unroll
-      for (int d = 0; d < 3; ++d)
-#pragma unroll
-        for (int ax = 0; ax < 3; ++ax) gh[mm][d][ax] = 0;
-    T Nacc[10];
-#pragma unroll
-    for (int c = 0; c < 10; ++c) Nacc[c] = 0;
+    const double adet = ge[12];
+    T U[10][3];
 #pragma unroll
     for (int k = 0; k < 10; ++k) {
-      T uk[3];
+      const dbl2* up = reinterpret_cast<const dbl2*>(a.upad + 4 * (int64_t)nodes[k]);
+      const dbl2 v0 = up[0];
+      const dbl2 v1 = up[1];
+      U[k][0] = (T)v0.x;
+      U[k][1] = (T)v0.y;
+      U[k][2] = (T)v1.x;
+    }
+    // S_c = nu rho A_rc + rho N_rc   (without |det|)
+    T S[10];
+    {
+      T Gs[6];
+      metric<T>(Bi, Gs);
 #pragma unroll
-      for (int d = 0; d < 3; ++d) uk[d] = (T)a.u[3 * (int64_t)nodes[k] + d];
-      T ut[3];  // ut_a = sum_d Bi[a][d] u_kd
+      for (int c = 0; c < 10; ++c) {
+        T acc = 0;
 #pragma unroll
-      for (int ax = 0; ax < 3; ++ax) ut[ax] = Bi[ax][0] * uk[0] + Bi[ax][1] * uk[1] + Bi[ax][2] * uk[2];
+        for (int s = 0; s < 6; ++s)
+          if (NT::KS[s][c] != 0.0) acc += Gs[s] * (T)NT::KS[s][c];
+        S[c] = (T)a.nurho * acc;
+      }
+    }
+    {
+      T Nacc[10];
 #pragma unroll
-      for (int c = 0; c < 10; ++c)
+      for (int c = 0; c < 10; ++c) Nacc[c] = 0;
 #pragma unroll
-        for (int ax = 0; ax < 3; ++ax)
-          if (NT::TT[ax][k][c] != 0.0) Nacc[c] += ut[ax] * (T)NT::TT[ax][k][c];
-      if (a.newton) {
+      for (int k = 0; k < 10; ++k) {
+        T ut[3];  // ut_ax = sum_d Bi[ax][d] u_kd
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) ut[ax] = Bi[ax][0] * U[k][0] + Bi[ax][1] * U[k][1] + Bi[ax][2] * U[k][2];
+#pragma unroll
+        for (int c = 0; c < 10; ++c)
+#pragma unroll
+          for (int ax = 0; ax < 3; ++ax)
+            if (NT::TT[ax][k][c] != 0.0) Nacc[c] += ut[ax] * (T)NT::TT[ax][k][c];
+      }
+#pragma unroll
+      for (int c = 0; c < 10; ++c) S[c] += (T)a.rho * Nacc[c];
+    }
+    if (a.newton) {
+      // grad u is P1: du_d1/dxhat_ax = sum_m lambda_m gh[m][ax], gh[m][ax] = sum_k u_k,d1 DV[ax][k][m]
+#pragma unroll
+      for (int d1 = 0; d1 < 3; ++d1) {
+        T gh[4][3];
 #pragma unroll
         for (int mm = 0; mm < 4; ++mm)
 #pragma unroll
-          for (int ax = 0; ax < 3; ++ax)
-            if (ft::DV[ax][k][mm] != 0.0)
+          for (int ax = 0; ax < 3; ++ax) gh[mm][ax] = 0;
 #pragma unroll
-              for (int d = 0; d < 3; ++d) gh[mm][d][ax] += uk[d] * (T)ft::DV[ax][k][mm];
-      }
-    }
+        for (int k = 0; k < 10; ++k)
 #pragma unroll
-    for (int c = 0; c < 10; ++c) S[c] += (T)a.rho * Nacc[c];
-    const double adet = g.adet;
-    if (a.newton) {
-      // g[m][d1][d2] = sum_ax Bi[ax][d2] gh[m][d1][ax], scaled by rho
-      T gw[4][3][3];
+          for (int mm = 0; mm < 4; ++mm)
 #pragma unroll
-      for (int mm = 0; mm < 4; ++mm)
+            for (int ax = 0; ax < 3; ++ax)
+              if (ft::DV[ax][k][mm] != 0.0) gh[mm][ax] += U[k][d1] * (T)ft::DV[ax][k][mm];
 #pragma unroll
-        for (int d1 = 0; d1 < 3; ++d1)
+        for (int d2 = 0; d2 < 3; ++d2) {
+          T gw[4];  // rho * du_d1/dx_d2 at vertex m
 #pragma unroll
-          for (int d2 = 0; d2 < 3; ++d2)
-            gw[mm][d1][d2] = (T)a.rho * (Bi[0][d2] * gh[mm][d1][0] + Bi[1][d2] * gh[mm][d1][1] +
-                                         Bi[2][d2] * gh[mm][d1][2]);
+          for (int mm = 0; mm < 4; ++mm)
+            gw[mm] = (T)a.rho * (Bi[0][d2] * gh[mm][0] + Bi[1][d2] * gh[mm][1] + Bi[2][d2] * gh[mm][2]);
 #pragma unroll
-      for (int c = 0; c < 10; ++c) {
-        double* r = rowA + 3 * m[2 + c];
-#pragma unroll
-        for (int d1 = 0; d1 < 3; ++d1)
-#pragma unroll
-          for (int d2 = 0; d2 < 3; ++d2) {
+          for (int c = 0; c < 10; ++c) {
             T w = 0;
 #pragma unroll
             for (int mm = 0; mm < 4; ++mm)
-              if (NT::L[mm][c] != 0.0) w += gw[mm][d1][d2] * (T)NT::L[mm][c];
+              if (NT::L[mm][c] != 0.0) w += gw[mm] * (T)NT::L[mm][c];
             if (d1 == d2) w += S[c];
-            atomicAdd(r + d1 * 3 * deg + d2, (double)w * adet);
+            atomicAdd(rowA + d1 * 3 * deg + 3 * m[2 + c] + d2, (double)w * adet);
           }
+        }
       }
     } else {
 #pragma unroll
@@ -350,10 +372,15 @@ struct VelRow {
   }
 };
 
+// One workgroup per chunk (the hardware keeps several chunks per CU in
+// flight, each in a different phase: zeroing, gathering, writing).  Inside a
+// chunk, vertex-class pairs occupy whole waves before the edge-class pairs start
+// (nv is padded to a multiple of 64), so each wave runs one specialised row kernel.
 template <typename T>
-__global__ __launch_bounds__(256) void k_ns_vel_rows(NsArgs a) {
+__global__ __launch_bounds__(FEDD_ROW_BLOCK) __attribute__((amdgpu_waves_per_eu(FEDD_ROW_WAVES))) void k_ns_vel_rows(NsArgs a) {
   extern __shared__ double lds[];
-  const int ch = blockIdx.x;
+  __shared__ int4 tab[256];  // per chunk node: A-row offset, deg, B^T-row offset, pdeg
+  const int64_t ch = blockIdx.x;
   const int n0 = a.chunk_node[ch], n1 = a.chunk_node[ch + 1];
   const int q0 = a.chunk_pair[ch], q1 = a.chunk_pair[ch + 1], qs = a.chunk_split[ch];
   const int64_t na0 = a.nptr[n0], na1 = a.nptr[n1];
@@ -364,27 +391,39 @@ __global__ __launch_bounds__(256) void k_ns_vel_rows(NsArgs a) {
     nb0 = a.pptr[n0];
     btLen = (int)(3 * (a.pptr[n1] - nb0));
   }
+  for (int t = threadIdx.x; t < n1 - n0; t += blockDim.x) {
+    const int64_t ai = a.nptr[n0 + t], ai1 = a.nptr[n0 + t + 1];
+    int4 v;
+    v.x = (int)(9 * (ai - na0));
+    v.y = (int)(ai1 - ai);
+    if (a.with_bt) {
+      const int64_t bi = a.pptr[n0 + t], bi1 = a.pptr[n0 + t + 1];
+      v.z = aLen + (int)(3 * (bi - nb0));
+      v.w = (int)(bi1 - bi);
+    } else {
+      v.z = 0;
+      v.w = 0;
+    }
+    tab[t] = v;
+  }
   for (int k = threadIdx.x; k < aLen + btLen; k += blockDim.x) lds[k] = 0.0;
   __syncthreads();
-  for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+  const int nv = qs - q0, ne = q1 - qs;
+  const int nvpad = (nv + 63) & ~63;
+  for (int t = threadIdx.x; t < nvpad + ne; t += blockDim.x) {
+    const bool vert = t < nvpad;  // uniform across the wave
+    if (vert && t >= nv) continue;
+    const int q = vert ? q0 + t : qs + (t - nvpad);
     const int32_t e = a.pair_elem[q];
     const uint4 mv = a.pair_meta[q];
     const uint8_t* m = reinterpret_cast<const uint8_t*>(&mv);
-    const int i = n0 + m[1];
-    const int64_t ai = a.nptr[i];
-    const int deg = (int)(a.nptr[i + 1] - ai);
-    double* rowA = lds + 9 * (ai - na0);
-    double* rowBT = nullptr;
-    int pdeg = 0;
-    if (a.with_bt) {
-      const int64_t bi = a.pptr[i];
-      pdeg = (int)(a.pptr[i + 1] - bi);
-      rowBT = lds + aLen + 3 * (bi - nb0);
-    }
-    if (q < qs)
-      VelRow<T, 0>::run(a, e, m, rowA, deg, rowBT, pdeg);
+    const int4 nd = tab[m[1]];
+    double* rowA = lds + nd.x;
+    double* rowBT = a.with_bt ? lds + nd.z : nullptr;
+    if (vert)
+      VelRow<T, 0>::run(a, e, m, rowA, nd.y, rowBT, nd.w);
     else
-      VelRow<T, 1>::run(a, e, m, rowA, deg, rowBT, pdeg);
+      VelRow<T, 1>::run(a, e, m, rowA, nd.y, rowBT, nd.w);
   }
   __syncthreads();
   double* outA = a.valsA + 9 * na0;
@@ -394,6 +433,50 @@ __global__ __launch_bounds__(256) void k_ns_vel_rows(NsArgs a) {
     for (int k = threadIdx.x; k < btLen; k += blockDim.x)
       __builtin_nontemporal_store(lds[aLen + k], outB + k);
   }
+}
+
+// Element geometry cache: grad lambda_0..3 (rows of Binv and minus their sum) and
+// |det B| per element, from the first four (vertex) nodes.
+__global__ __launch_bounds__(256) void k_geometry(const double* __restrict__ xyz, int64_t npts,
+                                                   const int32_t* __restrict__ conn, int64_t nelem,
+                                                   double* __restrict__ geo) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= nelem) return;
+  int32_t nodes[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) nodes[k] = conn[(int64_t)k * nelem + e];
+  Geo g;
+  geometry(xyz, npts, nodes, g);
+  double* o = geo + 16 * e;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    o[3 + d] = g.Bi[0][d];
+    o[6 + d] = g.Bi[1][d];
+    o[9 + d] = g.Bi[2][d];
+    o[d] = -(g.Bi[0][d] + g.Bi[1][d] + g.Bi[2][d]);
+  }
+  o[12] = g.adet;
+  o[13] = o[14] = o[15] = 0.0;
+}
+
+// Velocity iterate padded to 4 doubles per node (two 16-byte loads per node).
+__global__ __launch_bounds__(256) void k_pad_u(int64_t n, const double* __restrict__ u,
+                                                double* __restrict__ up) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  dbl2* o = reinterpret_cast<dbl2*>(up + 4 * i);
+  o[0] = dbl2{u[3 * i], u[3 * i + 1]};
+  o[1] = dbl2{u[3 * i + 2], 0.0};
+}
+
+int launch_geometry(Ctx* c) {
+  const Mesh& m = c->mesh[0];
+  int rc = alloc(c, c->d_geo, (size_t)m.nelem * 16 * 8);
+  if (rc) return rc;
+  k_geometry<<<nblk(m.nelem, 256), 256, 0, c->stream>>>(m.d_xyz.as<double>(), m.npts, m.d_conn.as<int32_t>(),
+                                                       m.nelem, c->d_geo.as<double>());
+  FEDD_HIP(c, hipGetLastError());
+  return FEDD_OK;
 }
 
 // Pressure rows of B (1,0): row p of element matrix rows 30+m (already scaled by -1).
@@ -678,6 +761,39 @@ __global__ __launch_bounds__(256) void k_rhs_rows(LapArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// STREAM-style copy (attainable HBM bandwidth; byte-counter calibration)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_stream_copy(int64_t n2, const dbl2* __restrict__ src,
+                                                     dbl2* __restrict__ dst) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n2; i += (int64_t)gridDim.x * blockDim.x)
+    __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+}
+
+int launch_stream_copy(Ctx* c, int64_t bytes, int iters, float* ms) {
+  DevBuf a, b;
+  int rc;
+  const int64_t n2 = bytes / 16;
+  if ((rc = alloc(c, a, n2 * 16)) || (rc = alloc(c, b, n2 * 16))) return rc;
+  FEDD_HIP(c, hipMemsetAsync(a.p, 0, n2 * 16, c->stream));
+  const unsigned grid = (unsigned)std::min<int64_t>((n2 + 255) / 256, (int64_t)c->num_cu * 32);
+  k_stream_copy<<<grid, 256, 0, c->stream>>>(n2, a.as<dbl2>(), b.as<dbl2>());
+  FEDD_HIP(c, hipGetLastError());
+  hipEvent_t e0, e1;
+  FEDD_HIP(c, hipEventCreate(&e0));
+  FEDD_HIP(c, hipEventCreate(&e1));
+  FEDD_HIP(c, hipEventRecord(e0, c->stream));
+  for (int k = 0; k < iters; ++k) k_stream_copy<<<grid, 256, 0, c->stream>>>(n2, a.as<dbl2>(), b.as<dbl2>());
+  FEDD_HIP(c, hipEventRecord(e1, c->stream));
+  FEDD_HIP(c, hipEventSynchronize(e1));
+  float t = 0.f;
+  FEDD_HIP(c, hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (ms) *ms = t / std::max(iters, 1);
+  return FEDD_OK;
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 static NsArgs ns_args(Ctx* c, const fedd_ns_params* p, const GatherPlan& pl) {
@@ -686,6 +802,9 @@ static NsArgs ns_args(Ctx* c, const fedd_ns_params* p, const GatherPlan& pl) {
   a.npts = c->mesh[0].npts;
   a.conn = c->mesh[0].d_conn.as<int32_t>();
   a.pconn = c->mesh[1].d_conn.as<int32_t>();
+  a.conn16 = c->mesh[0].d_conn16.as<int32_t>();
+  a.geo = c->d_geo.as<double>();
+  a.upad = c->d_upad.as<double>();
   a.nelem = c->mesh[0].nelem;
   a.u = c->vec[FEDD_VEC_U].as<double>();
   a.p = c->vec[FEDD_VEC_P].as<double>();
@@ -697,6 +816,7 @@ static NsArgs ns_args(Ctx* c, const fedd_ns_params* p, const GatherPlan& pl) {
   a.chunk_split = pl.d_chunk_split.as<int32_t>();
   a.pair_elem = pl.d_pair_elem.as<int32_t>();
   a.pair_meta = pl.d_pair_meta.as<uint4>();
+  a.nchunks = pl.nchunks;
   a.valsA = c->blk[0].d_vals.as<double>();
   a.valsBT = c->blk[1].d_vals.as<double>();
   a.valsB = c->blk[2].d_vals.as<double>();
@@ -714,14 +834,24 @@ int launch_ns_assembly(Ctx* c, const fedd_ns_params* p, int mode, int reassemble
   if (mode == FEDD_NS_JACOBIAN || mode == FEDD_NS_FIXEDPOINT) {
     fedd_ns_params q = *p;
     if (mode == FEDD_NS_FIXEDPOINT) q.newton = 0;  // the fixed-point matrix never holds W (:115-146)
+    const int64_t n2 = c->mesh[0].npts;
+    if (c->d_upad.bytes < (size_t)n2 * 32) {
+      int rc = alloc(c, c->d_upad, (size_t)n2 * 32);
+      if (rc) return rc;
+    }
     NsArgs a = ns_args(c, &q, c->vplan);
     a.with_bt = reassemble ? 0 : 1;
     const size_t lds = (size_t)c->vplan.lds_doubles * 8;
+    const unsigned grid = (unsigned)c->vplan.nchunks;
+    ktimer_begin(c);
+    k_pad_u<<<nblk(n2, 256), 256, 0, c->stream>>>(n2, a.u, c->d_upad.as<double>());
+    FEDD_HIP(c, hipGetLastError());
+    ktimer_end(c, "pad_u");
     ktimer_begin(c);
     if (f32)
-      k_ns_vel_rows<float><<<(unsigned)c->vplan.nchunks, 256, lds, c->stream>>>(a);
+      k_ns_vel_rows<float><<<grid, FEDD_ROW_BLOCK, lds, c->stream>>>(a);
     else
-      k_ns_vel_rows<double><<<(unsigned)c->vplan.nchunks, 256, lds, c->stream>>>(a);
+      k_ns_vel_rows<double><<<grid, FEDD_ROW_BLOCK, lds, c->stream>>>(a);
     FEDD_HIP(c, hipGetLastError());
     ktimer_end(c, "ns_vel_rows");
     if (!reassemble) {

@@ -64,6 +64,7 @@ _SIGS = {
     "fedd_assembly_navier_stokes": (ctypes.c_int, [_vp, ctypes.POINTER(NsParams), ctypes.c_int,
                                                    ctypes.c_int, ctypes.c_int]),
     "fedd_newton_step": (ctypes.c_int, [_vp, _ip]),
+    "fedd_stream_copy": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]),
     "fedd_event_record": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fedd_event_elapsed": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int,
                                           ctypes.POINTER(ctypes.c_float)]),
@@ -234,6 +235,12 @@ class Context:
     # timing
     def synchronize(self):
         self._chk(self._L.fedd_synchronize(self.h))
+
+    def stream_copy(self, nbytes, iters):
+        """ms per copy of nbytes (read nbytes + write nbytes)."""
+        ms = ctypes.c_float()
+        self._chk(self._L.fedd_stream_copy(self.h, nbytes, iters, ctypes.byref(ms)))
+        return ms.value
 
     def event_record(self, slot):
         self._chk(self._L.fedd_event_record(self.h, slot))

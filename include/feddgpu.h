@@ -155,6 +155,11 @@ int fedd_assembly_navier_stokes(fedd_ctx* ctx, const fedd_ns_params* params, int
 int fedd_newton_step(fedd_ctx* ctx, int* step);
 
 /* ----------------------------------------------------------------- timing */
+/* STREAM-style copy of `bytes` (read + write) repeated `iters` times on the context
+ * stream: the attainable HBM bandwidth next to the 8 TB/s datasheet figure, and the
+ * calibration kernel for byte counters.  No counterpart in the reference. */
+int fedd_stream_copy(fedd_ctx* ctx, int64_t bytes, int iters, float* ms_per_copy);
+
 /* HIP events recorded on the context stream.  Kernel timing, when enabled,
  * brackets every launch of the assembly kernels with events on that stream. */
 int fedd_event_record(fedd_ctx* ctx, int slot);           /* slot < 16 */
