@@ -1,0 +1,486 @@
+// Hyperelastic P2 tet assembly for gfx950: quadrature-point kernel + row gather.
+//
+// Reference: AssembleFE_{LinElas,NonLinElas,NonLinElas2}_def.hpp call the AceGen
+// tet-10 elements skr1/skr2/skr3 (NeoHookQuadraticTets{,2,3}.f) and take
+// element matrix = -s, rhs = p (AssembleFE_NonLinElas_def.hpp:119, 171).  With the
+// AceGen isoparametric determinant detJ_ace = -det J_F (J_F in FEDDLib's
+// reference frame), that is
+//     element = sum_q w_q det J_F(q) K_q ,   rhs = -sum_q w_q det J_F(q) f_int,q
+// where K_q = dP/dF contracted with the material gradients and f_int,q = P grad N.
+// Laws (mu = E/(2(1+nu)), lam = 2 mu nu/(1-2 nu), K = E/(3(1-2 nu))):
+//   model 0 (skr1, 4-pt rule):  W = mu/2(I1-3) - mu lnJ + lam/4 (J^2-1-2 lnJ)
+//   model 1 (skr2, 5-pt rule):  W = mu/2(I1-3) - mu lnJ + lam/2 (lnJ)^2
+//   model 2 (skr3, 5-pt rule):  W = mu/2(I1 I3^(-1/3) - 3) + K/2 (J-1)^2
+//
+// The quadrature-point kernel writes, per (element, point), a 52-double record:
+// F (9), S (6 Voigt), the material tangent CC = dS/dE as a symmetric 6x6 Voigt
+// matrix (21), w_q det J_F (1), and the inverse isoparametric Jacobian in
+// barycentric-row form (12: rows dxi_a/dX for a = 1..3 and minus their sum), so
+// that a relabelled element's inverse Jacobian is a selection of three rows.  The tangent slot is where the learned surrogate writes
+// instead of the exact law.  The row gather then builds each node's 3 rows of
+// the tangent and its 3 residual entries in LDS and writes them once.
+#include "fedd_internal.h"
+#include "fedd_perm.h"
+
+#include <algorithm>
+
+#ifndef FEDD_ELAS_WAVES
+#define FEDD_ELAS_WAVES 2
+#endif
+
+namespace fedd {
+
+__constant__ int8_t c_eperm2[10][10] = FEDD_PERM_P2_INIT;
+
+// Voigt order (11, 22, 33, 12, 13, 23); strains engineering, stresses tensorial.
+__constant__ int8_t c_vA[6] = {0, 1, 2, 0, 0, 1};
+__constant__ int8_t c_vB[6] = {0, 1, 2, 1, 2, 2};
+
+// Quadrature rules (SMSUtility.f icode 18 / 19), as barycentric orbits: the rule
+// is symmetric, so point m (m < 4) is the one whose barycentric coordinate
+// lambda_m is the distinguished value; the 5-point rule adds the centre (m = 4).
+struct ElasRule {
+  int n;
+  double a, b, wa;  // orbit points: lambda_m = a, others b, weight wa
+  double wc;        // centre weight (5-point rule)
+};
+__host__ __device__ inline ElasRule elas_rule(int model) {
+  if (model == 0) return {4, 0.585410196624968454, 0.138196601125010515, 0.0416666666666666667, 0.0};
+  return {5, 0.5, 1.0 / 6.0, 0.075, -0.133333333333333333};
+}
+
+// Reference point of orbit point m in FEDDLib coordinates (x, y, z) = (l1, l2, l3).
+__device__ inline void qp_ref(const ElasRule& R, int m, double xi[3]) {
+  if (m == 4) {
+    xi[0] = xi[1] = xi[2] = 0.25;
+    return;
+  }
+  double l[4] = {R.b, R.b, R.b, R.b};
+  l[m] = R.a;
+  xi[0] = l[1];
+  xi[1] = l[2];
+  xi[2] = l[3];
+}
+
+// P2 reference gradients (Helper.cpp:545-592).
+__device__ inline void p2_dphi(const double p[3], double g[10][3]) {
+  const double s = 4. * (p[0] + p[1] + p[2]);
+  g[0][0] = g[0][1] = g[0][2] = -3. + s;
+  g[1][0] = 4. * p[0] - 1.; g[1][1] = 0.; g[1][2] = 0.;
+  g[2][0] = 0.; g[2][1] = 4. * p[1] - 1.; g[2][2] = 0.;
+  g[3][0] = 0.; g[3][1] = 0.; g[3][2] = 4. * p[2] - 1.;
+  g[4][0] = 4. - 8. * p[0] - 4. * p[1] - 4. * p[2]; g[4][1] = -4. * p[0]; g[4][2] = -4. * p[0];
+  g[5][0] = 4. * p[1]; g[5][1] = 4. * p[0]; g[5][2] = 0.;
+  g[6][0] = -4. * p[1]; g[6][1] = 4. - 4. * p[0] - 8. * p[1] - 4. * p[2]; g[6][2] = -4. * p[1];
+  g[7][0] = -4. * p[2]; g[7][1] = -4. * p[2]; g[7][2] = 4. - 4. * p[0] - 4. * p[1] - 8. * p[2];
+  g[8][0] = 4. * p[2]; g[8][1] = 0.; g[8][2] = 4. * p[0];
+  g[9][0] = 0.; g[9][1] = 4. * p[2]; g[9][2] = 4. * p[1];
+}
+
+// Physical (material) gradients of the ten basis functions at reference point xi
+// for element nodes x[10][3]; returns the signed det J_F.
+__device__ inline double p2_grad(const double x[10][3], const double xi[3], double G[10][3],
+                                 double Ji[3][3]) {
+  double g[10][3];
+  p2_dphi(xi, g);
+  double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // J[i][a] = sum_k x_k,i dphi_k/dxi_a
+#pragma unroll
+  for (int k = 0; k < 10; ++k)
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) J[i][a] += x[k][i] * g[k][a];
+  const double det = J[0][0] * J[1][1] * J[2][2] + J[0][1] * J[1][2] * J[2][0] + J[0][2] * J[1][0] * J[2][1] -
+                     J[2][0] * J[1][1] * J[0][2] - J[2][1] * J[1][2] * J[0][0] - J[2][2] * J[1][0] * J[0][1];
+  const double id = 1.0 / det;
+  // Ji[a][i] = dxi_a/dX_i
+  Ji[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) * id;
+  Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+  Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+  Ji[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * id;
+  Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+  Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+  Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * id;
+  Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+  Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+#pragma unroll
+  for (int k = 0; k < 10; ++k)
+#pragma unroll
+    for (int A = 0; A < 3; ++A) G[k][A] = g[k][0] * Ji[0][A] + g[k][1] * Ji[1][A] + g[k][2] * Ji[2][A];
+  return det;
+}
+
+// Exact S and CC (Voigt) for the three laws, from C = F^T F.
+__device__ inline void exact_law(int model, double Emod, double nu, const double F[3][3], double S6[6],
+                                 double CC21[21]) {
+  double C[3][3];
+#pragma unroll
+  for (int A = 0; A < 3; ++A)
+#pragma unroll
+    for (int B = 0; B < 3; ++B) C[A][B] = F[0][A] * F[0][B] + F[1][A] * F[1][B] + F[2][A] * F[2][B];
+  const double mu = Emod / (2. * (1. + nu));
+  const double lam = 2. * mu * nu / (1. - 2. * nu);
+  const double I3 = C[0][0] * (C[1][1] * C[2][2] - C[1][2] * C[2][1]) -
+                    C[0][1] * (C[1][0] * C[2][2] - C[1][2] * C[2][0]) +
+                    C[0][2] * (C[1][0] * C[2][1] - C[1][1] * C[2][0]);
+  double Ci[3][3];
+  Ci[0][0] = (C[1][1] * C[2][2] - C[1][2] * C[2][1]) / I3;
+  Ci[0][1] = (C[0][2] * C[2][1] - C[0][1] * C[2][2]) / I3;
+  Ci[0][2] = (C[0][1] * C[1][2] - C[0][2] * C[1][1]) / I3;
+  Ci[1][0] = Ci[0][1];
+  Ci[1][1] = (C[0][0] * C[2][2] - C[0][2] * C[2][0]) / I3;
+  Ci[1][2] = (C[0][2] * C[1][0] - C[0][0] * C[1][2]) / I3;
+  Ci[2][0] = Ci[0][2];
+  Ci[2][1] = Ci[1][2];
+  Ci[2][2] = (C[0][0] * C[1][1] - C[0][1] * C[1][0]) / I3;
+  const double J = sqrt(I3);
+  const double I1 = C[0][0] + C[1][1] + C[2][2];
+  double a_id, a_ci, c_cc, c_ii, c_ic = 0.0;
+  if (model == 0) {
+    a_id = mu;
+    a_ci = lam / 2. * I3 - mu - lam / 2.;
+    c_cc = lam * I3;
+    c_ii = 2. * mu + lam - lam * I3;
+  } else if (model == 1) {
+    const double lnJ = log(J);
+    a_id = mu;
+    a_ci = lam * lnJ - mu;
+    c_cc = lam;
+    c_ii = 2. * (mu - lam * lnJ);
+  } else {
+    const double K = Emod / (3. * (1. - 2. * nu));
+    const double f = pow(I3, -1. / 3.);
+    a_id = mu * f;
+    a_ci = -mu / 3. * I1 * f + K * (J - 1.) * J;
+    c_cc = 2. / 9. * mu * f * I1 + K * (2. * J - 1.) * J;
+    c_ii = 2. / 3. * mu * f * I1 - 2. * K * (J - 1.) * J;
+    c_ic = -2. / 3. * mu * f;
+  }
+  const int vA[6] = {0, 1, 2, 0, 0, 1}, vB[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+  for (int I = 0; I < 6; ++I) S6[I] = (vA[I] == vB[I] ? a_id : 0.0) + a_ci * Ci[vA[I]][vB[I]];
+  int k = 0;
+#pragma unroll
+  for (int I = 0; I < 6; ++I)
+#pragma unroll
+    for (int Jv = I; Jv < 6; ++Jv) {
+      const int A = vA[I], B = vB[I], Cc = vA[Jv], D = vB[Jv];
+      CC21[k++] = c_cc * Ci[A][B] * Ci[Cc][D] + c_ii * 0.5 * (Ci[A][Cc] * Ci[B][D] + Ci[A][D] * Ci[B][Cc]) +
+                  c_ic * ((A == B ? Ci[Cc][D] : 0.0) + (Cc == D ? Ci[A][B] : 0.0));
+    }
+}
+
+struct ElasArgs {
+  const double* xyz;     // SoA coordinates
+  int64_t npts;
+  const int32_t* conn16;  // AoS connectivity
+  int64_t nelem;
+  const double* u;        // displacement, 3 per node
+  double* qp;             // per (element, point): 52-double record (see above)
+  int nq;
+  int model;
+  double Emod, nu;
+  int exact_tangent;      // 1: write CC from the law; 0: leave CC (surrogate wrote it)
+  // plan
+  const int64_t* nptr;
+  const int32_t* chunk_node;
+  const int32_t* chunk_pair;
+  const int32_t* pair_elem;
+  const uint4* pair_meta;
+  double* vals;
+  double* res;
+  int with_res;
+};
+
+// One thread per (element, quadrature point), original labelling.
+__global__ __launch_bounds__(256) void k_elas_qp(ElasArgs a) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= a.nelem * a.nq) return;
+  const int64_t e = t / a.nq;
+  const int m = (int)(t - e * a.nq);
+  const ElasRule R = elas_rule(a.model);
+  const int32_t* cn = a.conn16 + 16 * e;
+  double x[10][3], uu[10][3];
+#pragma unroll
+  for (int k = 0; k < 10; ++k) {
+    const int32_t n = cn[k];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      x[k][d] = a.xyz[d * a.npts + n];
+      uu[k][d] = a.u[3 * (int64_t)n + d];
+    }
+  }
+  double xi[3];
+  qp_ref(R, m, xi);
+  double G[10][3], Ji[3][3];
+  const double det = p2_grad(x, xi, G, Ji);
+  double F[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int A = 0; A < 3; ++A) {
+      double v = (i == A) ? 1.0 : 0.0;
+#pragma unroll
+      for (int k = 0; k < 10; ++k) v += uu[k][i] * G[k][A];
+      F[i][A] = v;
+    }
+  double* o = a.qp + 52 * t;
+  double S6[6], CC[21];
+  exact_law(a.model, a.Emod, a.nu, F, S6, CC);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int A = 0; A < 3; ++A) o[3 * i + A] = F[i][A];
+#pragma unroll
+  for (int I = 0; I < 6; ++I) o[9 + I] = S6[I];
+  if (a.exact_tangent)
+#pragma unroll
+    for (int k = 0; k < 21; ++k) o[15 + k] = CC[k];
+  o[36] = (m == 4 ? R.wc : R.wa) * det;
+#pragma unroll
+  for (int A = 0; A < 3; ++A) {
+    o[37 + A] = -(Ji[0][A] + Ji[1][A] + Ji[2][A]);
+    o[40 + A] = Ji[0][A];
+    o[43 + A] = Ji[1][A];
+    o[46 + A] = Ji[2][A];
+  }
+  o[49] = o[50] = o[51] = 0.0;
+}
+
+// Canonical-frame P2 reference gradients at the canonical points (compile-time).
+template <int M, int MODEL>
+struct DPhiQ {
+  static __device__ __forceinline__ void get(double g[10][3]) {
+    const ElasRule R = elas_rule(MODEL);
+    double xi[3];
+    if (M == 4) {
+      xi[0] = xi[1] = xi[2] = 0.25;
+    } else {
+      double l[4] = {R.b, R.b, R.b, R.b};
+      l[M] = R.a;
+      xi[0] = l[1];
+      xi[1] = l[2];
+      xi[2] = l[3];
+    }
+    p2_dphi(xi, g);
+  }
+};
+
+// Adds the contribution of canonical point M to node r's rows.
+template <typename T, int M, int MODEL>
+__device__ __forceinline__ void elas_point(const ElasArgs& a, int32_t e, const int8_t* pm, int r,
+                                           const uint8_t* mm, double* __restrict__ row, int deg,
+                                           double fres[3]) {
+  const int morig = M < 4 ? pm[M] : 4;
+  const double* rec = a.qp + 52 * ((int64_t)e * a.nq + morig);
+  double dphi[10][3];
+  DPhiQ<M, MODEL>::get(dphi);
+  double Jd[3][3];  // canonical inverse Jacobian: rows pm[1..3] of the barycentric form
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    const double* jr = rec + 37 + 3 * pm[ax + 1];
+#pragma unroll
+    for (int A = 0; A < 3; ++A) Jd[ax][A] = jr[A];
+  }
+  T Ji[3][3];
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+    for (int A = 0; A < 3; ++A) Ji[ax][A] = (T)Jd[ax][A];
+  T G[10][3];
+#pragma unroll
+  for (int c = 0; c < 10; ++c)
+#pragma unroll
+    for (int A = 0; A < 3; ++A) {
+      T v = 0;
+#pragma unroll
+      for (int ax = 0; ax < 3; ++ax)
+        if (dphi[c][ax] != 0.0) v += (T)dphi[c][ax] * Ji[ax][A];
+      G[c][A] = v;
+    }
+  T g[3];
+#pragma unroll
+  for (int A = 0; A < 3; ++A) g[A] = r == 0 ? G[0][A] : G[4][A];
+  const double wdet = rec[36];
+  double Fd[3][3];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) Fd[k / 3][k % 3] = rec[k];
+  double Sd[6];
+#pragma unroll
+  for (int I = 0; I < 6; ++I) Sd[I] = rec[9 + I];
+  {  // residual in FP64: f_a += w det sum_A (F S)_aA g_A, with the row gradient in FP64
+    double gd[3];
+#pragma unroll
+    for (int A = 0; A < 3; ++A) {
+      const double v0 = dphi[0][0] * Jd[0][A] + dphi[0][1] * Jd[1][A] + dphi[0][2] * Jd[2][A];
+      const double v4 = dphi[4][0] * Jd[0][A] + dphi[4][1] * Jd[1][A] + dphi[4][2] * Jd[2][A];
+      gd[A] = r == 0 ? v0 : v4;
+    }
+    const double Sg0 = Sd[0] * gd[0] + Sd[3] * gd[1] + Sd[4] * gd[2];
+    const double Sg1 = Sd[3] * gd[0] + Sd[1] * gd[1] + Sd[5] * gd[2];
+    const double Sg2 = Sd[4] * gd[0] + Sd[5] * gd[1] + Sd[2] * gd[2];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) fres[i] += wdet * (Fd[i][0] * Sg0 + Fd[i][1] * Sg1 + Fd[i][2] * Sg2);
+  }
+  T F[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) F[i][j] = (T)Fd[i][j];
+  const T w = (T)wdet;
+  T Bg[3][6];  // d E_I / d u_(r,a), engineering shear
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    Bg[i][0] = F[i][0] * g[0];
+    Bg[i][1] = F[i][1] * g[1];
+    Bg[i][2] = F[i][2] * g[2];
+    Bg[i][3] = F[i][0] * g[1] + F[i][1] * g[0];
+    Bg[i][4] = F[i][0] * g[2] + F[i][2] * g[0];
+    Bg[i][5] = F[i][1] * g[2] + F[i][2] * g[1];
+  }
+  T BC[3][6];  // w * Bg CC
+  {
+    T CC[21];
+#pragma unroll
+    for (int k = 0; k < 21; ++k) CC[k] = (T)rec[15 + k];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int J = 0; J < 6; ++J) {
+        T s = 0;
+#pragma unroll
+        for (int I = 0; I < 6; ++I) {
+          const int lo = I < J ? I : J, hi = I < J ? J : I;
+          s += Bg[i][I] * CC[lo * 6 - lo * (lo - 1) / 2 + (hi - lo)];
+        }
+        BC[i][J] = w * s;
+      }
+  }
+  T Sg[3];
+  Sg[0] = w * ((T)Sd[0] * g[0] + (T)Sd[3] * g[1] + (T)Sd[4] * g[2]);
+  Sg[1] = w * ((T)Sd[3] * g[0] + (T)Sd[1] * g[1] + (T)Sd[5] * g[2]);
+  Sg[2] = w * ((T)Sd[4] * g[0] + (T)Sd[5] * g[1] + (T)Sd[2] * g[2]);
+#pragma unroll
+  for (int c = 0; c < 10; ++c) {
+    const T geo = Sg[0] * G[c][0] + Sg[1] * G[c][1] + Sg[2] * G[c][2];
+    double* rc = row + 3 * mm[2 + c];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      T Bh[6];
+      Bh[0] = F[j][0] * G[c][0];
+      Bh[1] = F[j][1] * G[c][1];
+      Bh[2] = F[j][2] * G[c][2];
+      Bh[3] = F[j][0] * G[c][1] + F[j][1] * G[c][0];
+      Bh[4] = F[j][0] * G[c][2] + F[j][2] * G[c][0];
+      Bh[5] = F[j][1] * G[c][2] + F[j][2] * G[c][1];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        T s = (i == j) ? geo : (T)0;
+#pragma unroll
+        for (int J = 0; J < 6; ++J) s += BC[i][J] * Bh[J];
+        atomicAdd(rc + i * 3 * deg + j, (double)s);
+      }
+    }
+  }
+}
+
+template <typename T, int MODEL>
+__device__ __forceinline__ void elas_pair(const ElasArgs& a, int32_t e, const uint8_t* mm,
+                                          double* __restrict__ row, int deg, double fres[3]) {
+  const int li = mm[0];
+  const int8_t* pm = c_eperm2[li];
+  const int r = li < 4 ? 0 : 4;
+  elas_point<T, 0, MODEL>(a, e, pm, r, mm, row, deg, fres);
+  elas_point<T, 1, MODEL>(a, e, pm, r, mm, row, deg, fres);
+  elas_point<T, 2, MODEL>(a, e, pm, r, mm, row, deg, fres);
+  elas_point<T, 3, MODEL>(a, e, pm, r, mm, row, deg, fres);
+  if (MODEL != 0) elas_point<T, 4, MODEL>(a, e, pm, r, mm, row, deg, fres);
+}
+
+// Row gather: pair (node i, element e, local li) adds node i's 3 tangent rows and
+// 3 residual entries of element e into LDS; the chunk is then written once.
+template <typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FEDD_ELAS_WAVES))) void k_elas_rows(ElasArgs a) {
+  extern __shared__ double lds[];
+  __shared__ int2 tab[256];
+  const int64_t ch = blockIdx.x;
+  const int n0 = a.chunk_node[ch], n1 = a.chunk_node[ch + 1];
+  const int q0 = a.chunk_pair[ch], q1 = a.chunk_pair[ch + 1];
+  const int64_t na0 = a.nptr[n0], na1 = a.nptr[n1];
+  const int aLen = (int)(9 * (na1 - na0));
+  const int rLen = a.with_res ? 3 * (n1 - n0) : 0;
+  for (int t = threadIdx.x; t < n1 - n0; t += blockDim.x) {
+    const int64_t ai = a.nptr[n0 + t];
+    tab[t] = make_int2((int)(9 * (ai - na0)), (int)(a.nptr[n0 + t + 1] - ai));
+  }
+  for (int k = threadIdx.x; k < aLen + rLen; k += blockDim.x) lds[k] = 0.0;
+  __syncthreads();
+  for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+    const int32_t e = a.pair_elem[q];
+    const uint4 mv = a.pair_meta[q];
+    const uint8_t* mm = reinterpret_cast<const uint8_t*>(&mv);
+    const int2 nd = tab[mm[1]];
+    double fres[3] = {0.0, 0.0, 0.0};
+    if (a.model == 0)
+      elas_pair<T, 0>(a, e, mm, lds + nd.x, nd.y, fres);
+    else
+      elas_pair<T, 1>(a, e, mm, lds + nd.x, nd.y, fres);
+    if (a.with_res) {
+      double* rr = lds + aLen + 3 * mm[1];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) atomicAdd(rr + i, -fres[i]);
+    }
+  }
+  __syncthreads();
+  double* out = a.vals + 9 * na0;
+  for (int k = threadIdx.x; k < aLen; k += blockDim.x) __builtin_nontemporal_store(lds[k], out + k);
+  if (a.with_res)
+    for (int k = threadIdx.x; k < rLen; k += blockDim.x) a.res[3 * (int64_t)n0 + k] = lds[aLen + k];
+}
+
+int launch_elasticity(Ctx* c, int model, double Emod, double nu, int precision, int with_res,
+                      int exact_tangent) {
+  const Mesh& m = c->mesh[0];
+  ElasArgs a{};
+  a.xyz = m.d_xyz.as<double>();
+  a.npts = m.npts;
+  a.conn16 = m.d_conn16.as<int32_t>();
+  a.nelem = m.nelem;
+  a.u = c->vec[FEDD_VEC_U].as<double>();
+  a.nq = elas_rule(model).n;
+  a.model = model;
+  a.Emod = Emod;
+  a.nu = nu;
+  a.exact_tangent = exact_tangent;
+  const size_t qbytes = (size_t)m.nelem * 5 * 52 * 8;
+  if (c->d_qp.bytes < qbytes) {
+    int rc = alloc(c, c->d_qp, qbytes);
+    if (rc) return rc;
+  }
+  a.qp = c->d_qp.as<double>();
+  const GatherPlan& pl = c->vplan;
+  a.nptr = c->d_nptr.as<int64_t>();
+  a.chunk_node = pl.d_chunk_node.as<int32_t>();
+  a.chunk_pair = pl.d_chunk_pair.as<int32_t>();
+  a.pair_elem = pl.d_pair_elem.as<int32_t>();
+  a.pair_meta = pl.d_pair_meta.as<uint4>();
+  a.vals = c->blk[0].d_vals.as<double>();
+  a.res = c->vec[FEDD_VEC_RES_U].as<double>();
+  a.with_res = with_res;
+  const int64_t nqp = m.nelem * a.nq;
+  ktimer_begin(c);
+  k_elas_qp<<<(unsigned)((nqp + 255) / 256), 256, 0, c->stream>>>(a);
+  FEDD_HIP(c, hipGetLastError());
+  ktimer_end(c, "elas_qp");
+  const size_t lds = (size_t)(pl.lds_doubles + 3 * 256) * 8;
+  ktimer_begin(c);
+  if (precision == FEDD_FP32_LOCAL)
+    k_elas_rows<float><<<(unsigned)pl.nchunks, 256, lds, c->stream>>>(a);
+  else
+    k_elas_rows<double><<<(unsigned)pl.nchunks, 256, lds, c->stream>>>(a);
+  FEDD_HIP(c, hipGetLastError());
+  ktimer_end(c, "elas_rows");
+  return FEDD_OK;
+}
+
+}  // namespace fedd

@@ -543,6 +543,8 @@ int fedd_graph_build(fedd_ctx* c, int kind) {
     vs.rows_per_node = 3;
     vs.comps_per_col = 3;
   }
+  if (kind == FEDD_GRAPH_VECTOR && m.npe != 10)
+    return fail(c, FEDD_ERR_ARG, "the vector graph needs a P2 mesh (tet-10 elasticity)");
   if (kind == FEDD_GRAPH_NS) {
     vs.second = &g.bt;
     vs.second_rows = 3;
@@ -743,6 +745,33 @@ int fedd_assembly_navier_stokes(fedd_ctx* c, const fedd_ns_params* p, int mode, 
   if (rc) return rc;
   if (mode != FEDD_NS_RHS) c->newton_step++;  // advanceNewtonStep (FE_def.hpp:1171,1196)
   return FEDD_OK;
+}
+
+static int elasticity_common(fedd_ctx* c, int model, double E, double nu, int precision, int with_res) {
+  if (!c) return FEDD_ERR_ARG;
+  if (c->graph_kind != FEDD_GRAPH_VECTOR) return fail(c, FEDD_ERR_STATE, "needs a vector (P2) graph");
+  if (precision != FEDD_FP64 && precision != FEDD_FP32_LOCAL) return fail(c, FEDD_ERR_ARG, "bad precision");
+  if (!(E > 0) || !(nu > -1.0 && nu < 0.5)) return fail(c, FEDD_ERR_ARG, "E must be > 0 and -1 < nu < 0.5");
+  if (c->vec_n[FEDD_VEC_U] != 3 * c->mesh[0].npts) return fail(c, FEDD_ERR_STATE, "displacement not uploaded");
+  (void)hipSetDevice(c->device);
+  if (with_res) {
+    int64_t n = 3 * c->mesh[0].npts;
+    if (c->vec_n[FEDD_VEC_RES_U] != n) {
+      int rc = alloc(c, c->vec[FEDD_VEC_RES_U], n * 8);
+      if (rc) return rc;
+      c->vec_n[FEDD_VEC_RES_U] = n;
+    }
+  }
+  return launch_elasticity(c, model, E, nu, precision, with_res, 1);
+}
+
+int fedd_assembly_linear_elasticity(fedd_ctx* c, double E, double nu, int precision) {
+  return elasticity_common(c, 0, E, nu, precision, 0);
+}
+
+int fedd_assembly_nonlinear_elasticity(fedd_ctx* c, int neo_hooke_model, double E, double nu, int precision) {
+  // "Neo-Hooke Modell" == 1 -> NonLinearElasticity (skr2), otherwise NonLinearElasticity2 (skr3)
+  return elasticity_common(c, neo_hooke_model == 1 ? 1 : 2, E, nu, precision, 1);
 }
 
 int fedd_newton_step(fedd_ctx* c, int* step) {

@@ -89,6 +89,7 @@ struct Ctx {
   GatherPlan vplan, pplan;        // velocity (primary) and pressure row gathers
   DevBuf d_geo;                   // per element: grad lambda_0..3 (12) + |det| + pad, 16 doubles
   DevBuf d_upad;                  // velocity iterate padded to 4 doubles per node
+  DevBuf d_qp;                    // elasticity quadrature-point records (40 doubles each)
   // colouring
   std::vector<int64_t> colour_ptr;
   std::vector<int32_t> colour_perm;
@@ -136,5 +137,7 @@ int launch_laplace(Ctx* c, int precision);
 int launch_rhs(Ctx* c, double f);
 int launch_stream_copy(Ctx* c, int64_t bytes, int iters, float* ms);
 int launch_geometry(Ctx* c);
+int launch_elasticity(Ctx* c, int model, double Emod, double nu, int precision, int with_res,
+                      int exact_tangent);
 
 }  // namespace fedd

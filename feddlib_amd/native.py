@@ -64,6 +64,9 @@ _SIGS = {
     "fedd_assembly_navier_stokes": (ctypes.c_int, [_vp, ctypes.POINTER(NsParams), ctypes.c_int,
                                                    ctypes.c_int, ctypes.c_int]),
     "fedd_newton_step": (ctypes.c_int, [_vp, _ip]),
+    "fedd_assembly_linear_elasticity": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_int]),
+    "fedd_assembly_nonlinear_elasticity": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                          ctypes.c_int]),
     "fedd_stream_copy": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]),
     "fedd_event_record": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fedd_event_elapsed": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int,
@@ -226,6 +229,12 @@ class Context:
         p = NsParams(viscosity, density, 1 if newton else 0, (ctypes.c_double * 4)(*coeff))
         self._chk(self._L.fedd_assembly_navier_stokes(self.h, ctypes.byref(p), mode, 1 if reassemble else 0,
                                                       precision))
+
+    def assembly_linear_elasticity(self, E, nu, precision=FP64):
+        self._chk(self._L.fedd_assembly_linear_elasticity(self.h, E, nu, precision))
+
+    def assembly_nonlinear_elasticity(self, neo_hooke_model, E, nu, precision=FP64):
+        self._chk(self._L.fedd_assembly_nonlinear_elasticity(self.h, neo_hooke_model, E, nu, precision))
 
     def newton_step(self):
         s = ctypes.c_int()

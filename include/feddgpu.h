@@ -150,6 +150,16 @@ int fedd_assembly_rhs(fedd_ctx* ctx, double f);
  * Rhs writes VEC_RES_U and VEC_RES_P.  Needs GRAPH_NS. */
 int fedd_assembly_navier_stokes(fedd_ctx* ctx, const fedd_ns_params* params, int mode,
                                 int reassemble, int precision);
+/* FE::assemblyLinearElasticity (FE_def.hpp:362-434): the "LinearElasticity"
+ * AssembleFE element, i.e. AceGen skr1 (NeoHookQuadraticTets.f, 4-point rule), at
+ * the displacement VEC_U; element matrix = -s (AssembleFE_LinElas_def.hpp:109-119).
+ * Fills A of a GRAPH_VECTOR graph on a P2 mesh.  Parameter/E, Parameter/Poisson Ratio. */
+int fedd_assembly_linear_elasticity(fedd_ctx* ctx, double E, double nu, int precision);
+/* FE::assemblyNonLinearElasticity (FE_def.hpp:449-523): tangent into A and residual
+ * (rhs = p of the element, AssembleFE_NonLinElas_def.hpp:131-173) into VEC_RES_U.
+ * neo_hooke_model = Parameter/"Neo-Hooke Modell": 1 -> skr2, otherwise skr3. */
+int fedd_assembly_nonlinear_elasticity(fedd_ctx* ctx, int neo_hooke_model, double E, double nu,
+                                       int precision);
 /* AssembleFE::advanceNewtonStep / getNewtonStep (AssembleFE_decl.hpp:146) for the
  * NS elements: the counter the reference keeps per element. */
 int fedd_newton_step(fedd_ctx* ctx, int* step);
