@@ -438,8 +438,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FEDD_ELAS_W
     for (int k = threadIdx.x; k < rLen; k += blockDim.x) a.res[3 * (int64_t)n0 + k] = lds[aLen + k];
 }
 
-int launch_elasticity(Ctx* c, int model, double Emod, double nu, int precision, int with_res,
-                      int exact_tangent) {
+int launch_elasticity(Ctx* c, int model, double Emod, double nu, int precision, int with_res, int tangent) {
+  const int exact_tangent = tangent == 2 ? 0 : 1;
   const Mesh& m = c->mesh[0];
   ElasArgs a{};
   a.xyz = m.d_xyz.as<double>();
@@ -472,6 +472,10 @@ int launch_elasticity(Ctx* c, int model, double Emod, double nu, int precision, 
   k_elas_qp<<<(unsigned)((nqp + 255) / 256), 256, 0, c->stream>>>(a);
   FEDD_HIP(c, hipGetLastError());
   ktimer_end(c, "elas_qp");
+  if (!exact_tangent) {
+    int rc = launch_mlp_tangent(c, a.nq);
+    if (rc) return rc;
+  }
   const size_t lds = (size_t)(pl.lds_doubles + 3 * 256) * 8;
   ktimer_begin(c);
   if (precision == FEDD_FP32_LOCAL)

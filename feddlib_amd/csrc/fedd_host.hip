@@ -356,7 +356,6 @@ extern "C" int fedd_mesh_correct_orientation(const double* pts, int64_t npts, in
   return FEDD_OK;
 }
 
-struct fedd_ctx : public fedd::Ctx {};
 
 // LDS budget of one row-gather chunk, in doubles.
 #ifndef FEDD_LDS_BUDGET
@@ -669,12 +668,15 @@ static int64_t vec_len(fedd_ctx* c, int which) {
     case FEDD_VEC_P:
     case FEDD_VEC_RES_P: return c->mesh[1].npts;
     case FEDD_VEC_RHS: return c->mesh[0].npts;
+    case FEDD_VEC_FEXT:
+    case FEDD_VEC_X:
+    case FEDD_VEC_B: return 3 * c->mesh[0].npts;
   }
   return -1;
 }
 
 int fedd_vector_upload(fedd_ctx* c, int which, const double* host, int64_t n) {
-  if (!c || which < 0 || which > 4 || !host) return FEDD_ERR_ARG;
+  if (!c || which < 0 || which >= FEDD_NUM_VECTORS || !host) return FEDD_ERR_ARG;
   int64_t want = vec_len(c, which);
   if (n != want) return fail(c, FEDD_ERR_ARG, "vector length does not match the mesh");
   (void)hipSetDevice(c->device);
@@ -686,7 +688,7 @@ int fedd_vector_upload(fedd_ctx* c, int which, const double* host, int64_t n) {
 }
 
 int fedd_vector_download(fedd_ctx* c, int which, double* host, int64_t n) {
-  if (!c || which < 0 || which > 4 || !host) return FEDD_ERR_ARG;
+  if (!c || which < 0 || which >= FEDD_NUM_VECTORS || !host) return FEDD_ERR_ARG;
   if (c->vec_n[which] != n) return fail(c, FEDD_ERR_ARG, "vector length mismatch or vector absent");
   (void)hipSetDevice(c->device);
   FEDD_HIP(c, hipMemcpyAsync(host, c->vec[which].p, n * 8, hipMemcpyDeviceToHost, c->stream));
@@ -695,7 +697,7 @@ int fedd_vector_download(fedd_ctx* c, int which, double* host, int64_t n) {
 }
 
 int fedd_vector_device_ptr(fedd_ctx* c, int which, double** d, int64_t* n) {
-  if (!c || which < 0 || which > 4 || !d) return FEDD_ERR_ARG;
+  if (!c || which < 0 || which >= FEDD_NUM_VECTORS || !d) return FEDD_ERR_ARG;
   *d = c->vec[which].as<double>();
   if (n) *n = c->vec_n[which];
   return FEDD_OK;

@@ -94,8 +94,17 @@ struct Ctx {
   std::vector<int64_t> colour_ptr;
   std::vector<int32_t> colour_perm;
   // vectors
-  DevBuf vec[5];
-  int64_t vec_n[5] = {0, 0, 0, 0, 0};
+  DevBuf vec[FEDD_NUM_VECTORS];
+  int64_t vec_n[FEDD_NUM_VECTORS] = {};
+  // Dirichlet constraints and Krylov work space (elasticity Newton solve)
+  DevBuf d_bcflag;      // uint8 per dof of the primary field
+  int64_t nbc = 0;
+  DevBuf d_kw[5];       // r, z, p, Ap and the block-Jacobi inverse (9 per node)
+  DevBuf d_part;        // per-block partial sums
+  DevBuf d_scal;        // device scalars of the Krylov loop
+  // learned tangent surrogate (FP32 master weights + BF16 copies)
+  DevBuf d_mlp;         // parameters, gradients and Adam moments
+  bool mlp_ready = false;
   int newton_step = 0;
   // timing
   hipEvent_t ev[16] = {};
@@ -137,7 +146,10 @@ int launch_laplace(Ctx* c, int precision);
 int launch_rhs(Ctx* c, double f);
 int launch_stream_copy(Ctx* c, int64_t bytes, int iters, float* ms);
 int launch_geometry(Ctx* c);
-int launch_elasticity(Ctx* c, int model, double Emod, double nu, int precision, int with_res,
-                      int exact_tangent);
+// tangent: 1 = exact law, 2 = learned surrogate (the network writes the tangent slot)
+int launch_elasticity(Ctx* c, int model, double Emod, double nu, int precision, int with_res, int tangent);
+int launch_mlp_tangent(Ctx* c, int nq);
 
 }  // namespace fedd
+
+struct fedd_ctx : public fedd::Ctx {};

@@ -1,0 +1,510 @@
+// Newton solve of the P2 hyperelastic problem on gfx950: Dirichlet rows/columns,
+// traction loads, CSR SpMV, block-Jacobi preconditioned CG with device-resident
+// scalars, and the load-stepped Newton driver with exact-tangent fallback.
+//
+// Reference roles: NonLinearSolver::solveNewton (NonLinearSolver_def.hpp:356-414)
+// drives NonLinElasAssFE (NonLinElasAssFE_def.hpp:111-152, 262-377) with
+// BCBuilder::setSystem (BCBuilder_def.hpp:590-618) and Belos/Amesos2 linear solves
+// (LinearSolver_def.hpp:23-137).  Here the whole loop stays on the device.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "fedd_internal.h"
+
+namespace fedd {
+
+static inline unsigned nb(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+// ---------------------------------------------------------------- Dirichlet
+// Constrained rows and columns become identity rows/columns (symmetric
+// elimination keeps the tangent SPD; the increment of a constrained dof is 0).
+__global__ __launch_bounds__(256) void k_bc_matrix(int64_t nrows, const int64_t* __restrict__ rowptr,
+                                                   const int32_t* __restrict__ colind, double* __restrict__ vals,
+                                                   const uint8_t* __restrict__ flag) {
+  const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t row = gid >> 4;
+  const int lane = (int)(gid & 15);
+  if (row >= nrows) return;
+  const bool fr = flag[row] != 0;
+  for (int64_t k = rowptr[row] + lane; k < rowptr[row + 1]; k += 16) {
+    const int32_t col = colind[k];
+    if (fr || flag[col]) vals[k] = (col == row) ? 1.0 : 0.0;
+  }
+}
+
+// R = -res - lam fext, zero on constrained dofs (res = assembled element rhs p).
+__global__ __launch_bounds__(256) void k_newton_residual(int64_t n, const double* __restrict__ res,
+                                                         const double* __restrict__ fext, double lam,
+                                                         const uint8_t* __restrict__ flag,
+                                                         double* __restrict__ R, double* __restrict__ part) {
+  __shared__ double red[256];
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (i < n) {
+    v = flag[i] ? 0.0 : -res[i] - lam * (fext ? fext[i] : 0.0);
+    R[i] = v;
+  }
+  red[threadIdx.x] = v * v;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// ------------------------------------------------------------------- Krylov
+// Single-block sum of n partials into out[slot]; optional CG bookkeeping:
+// mode 1: beta = sum / rz (stored in scal[5]) and rz = sum.
+__global__ __launch_bounds__(1024) void k_reduce(const double* __restrict__ part, int64_t n,
+                                                 double* __restrict__ scal, int slot, int mode) {
+  __shared__ double red[1024];
+  double v = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) v += part[i];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = 512; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (mode == 1) {
+      scal[5] = red[0] / scal[0];
+      scal[0] = red[0];
+    } else {
+      scal[slot] = red[0];
+    }
+  }
+}
+
+// y = A x with 16 lanes per row; partial sums of x.y per block.
+__global__ __launch_bounds__(256) void k_spmv_dot(int64_t nrows, const int64_t* __restrict__ rowptr,
+                                                  const int32_t* __restrict__ colind,
+                                                  const double* __restrict__ vals, const double* __restrict__ x,
+                                                  double* __restrict__ y, double* __restrict__ part) {
+  __shared__ double red[16];
+  const int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 4;
+  const int lane = threadIdx.x & 15;
+  double s = 0.0;
+  if (row < nrows)
+    for (int64_t k = rowptr[row] + lane; k < rowptr[row + 1]; k += 16) s += vals[k] * x[colind[k]];
+#pragma unroll
+  for (int o = 8; o > 0; o >>= 1) s += __shfl_down(s, o, 16);
+  if (lane == 0) {
+    double d = 0.0;
+    if (row < nrows) {
+      y[row] = s;
+      d = s * x[row];
+    }
+    red[threadIdx.x >> 4] = d;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) t += red[k];
+    part[blockIdx.x] = t;
+  }
+}
+
+// x += alpha p; r -= alpha Ap; z = Binv r (3x3 node blocks); partials of r.z and r.r.
+__global__ __launch_bounds__(256) void k_pcg_update(int64_t nnode, double* __restrict__ x, double* __restrict__ r,
+                                                    const double* __restrict__ p, const double* __restrict__ Ap,
+                                                    double* __restrict__ z, const double* __restrict__ binv,
+                                                    const double* __restrict__ scal, double* __restrict__ part_rz,
+                                                    double* __restrict__ part_rr, int first) {
+  __shared__ double red[2][256];
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  double rz = 0.0, rr = 0.0;
+  if (i < nnode) {
+    double rl[3];
+    if (first) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) rl[d] = r[3 * i + d];
+    } else {
+      const double alpha = scal[0] / scal[1];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        x[3 * i + d] += alpha * p[3 * i + d];
+        rl[d] = r[3 * i + d] - alpha * Ap[3 * i + d];
+        r[3 * i + d] = rl[d];
+      }
+    }
+    const double* B = binv + 9 * i;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double zv = B[3 * d] * rl[0] + B[3 * d + 1] * rl[1] + B[3 * d + 2] * rl[2];
+      z[3 * i + d] = zv;
+      rz += zv * rl[d];
+      rr += rl[d] * rl[d];
+    }
+  }
+  red[0][threadIdx.x] = rz;
+  red[1][threadIdx.x] = rr;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + s];
+      red[1][threadIdx.x] += red[1][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part_rz[blockIdx.x] = red[0][0];
+    part_rr[blockIdx.x] = red[1][0];
+  }
+}
+
+// p = z + beta p  (first: p = z)
+__global__ __launch_bounds__(256) void k_pcg_p(int64_t n, double* __restrict__ p, const double* __restrict__ z,
+                                               const double* __restrict__ scal, int first) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  p[i] = first ? z[i] : z[i] + scal[5] * p[i];
+}
+
+// Inverses of the 3x3 diagonal node blocks (block Jacobi).
+__global__ __launch_bounds__(256) void k_block_diag(int64_t nnode, const int64_t* __restrict__ rowptr,
+                                                    const int32_t* __restrict__ colind,
+                                                    const double* __restrict__ vals, double* __restrict__ binv) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nnode) return;
+  double a[3][3];
+#pragma unroll
+  for (int d1 = 0; d1 < 3; ++d1) {
+    const int64_t row = 3 * i + d1;
+    int64_t lo = rowptr[row], hi = rowptr[row + 1];
+    const int32_t target = (int32_t)(3 * i);
+    while (lo < hi) {  // first column >= 3i
+      const int64_t mid = (lo + hi) >> 1;
+      if (colind[mid] < target) lo = mid + 1;
+      else hi = mid;
+    }
+#pragma unroll
+    for (int d2 = 0; d2 < 3; ++d2) a[d1][d2] = vals[lo + d2];
+  }
+  const double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) -
+                     a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+                     a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+  const double id = 1.0 / det;
+  double* o = binv + 9 * i;
+  o[0] = (a[1][1] * a[2][2] - a[1][2] * a[2][1]) * id;
+  o[1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) * id;
+  o[2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) * id;
+  o[3] = (a[1][2] * a[2][0] - a[1][0] * a[2][2]) * id;
+  o[4] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) * id;
+  o[5] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) * id;
+  o[6] = (a[1][0] * a[2][1] - a[1][1] * a[2][0]) * id;
+  o[7] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) * id;
+  o[8] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) * id;
+}
+
+__global__ __launch_bounds__(256) void k_axpy(int64_t n, double a, const double* __restrict__ x,
+                                              double* __restrict__ y) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) y[i] += a * x[i];
+}
+
+__global__ __launch_bounds__(256) void k_scale_copy(int64_t n, double a, const double* __restrict__ x,
+                                                    double* __restrict__ y) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) y[i] = a * x[i];
+}
+
+__global__ __launch_bounds__(256) void k_fill(int64_t n, double* __restrict__ y, double v) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) y[i] = v;
+}
+
+static int ensure(Ctx* c, DevBuf& b, size_t bytes) {
+  if (b.bytes >= bytes) return FEDD_OK;
+  return alloc(c, b, bytes);
+}
+
+static int ensure_vec(Ctx* c, int w, int64_t n) {
+  if (c->vec_n[w] == n && c->vec[w].bytes >= (size_t)n * 8) return FEDD_OK;
+  int rc = alloc(c, c->vec[w], n * 8);
+  if (rc) return rc;
+  c->vec_n[w] = n;
+  return FEDD_OK;
+}
+
+int apply_dirichlet_matrix(Ctx* c) {
+  const Block& A = c->blk[0];
+  if (c->nbc == 0) return FEDD_OK;
+  k_bc_matrix<<<nb(A.nrows * 16, 256), 256, 0, c->stream>>>(A.nrows, A.d_rowptr.as<int64_t>(),
+                                                             A.d_colind.as<int32_t>(), A.d_vals.as<double>(),
+                                                             c->d_bcflag.as<uint8_t>());
+  FEDD_HIP(c, hipGetLastError());
+  return FEDD_OK;
+}
+
+// Preconditioned CG on block A with b = VEC_B, x = VEC_X (starting from 0).
+int pcg_solve(Ctx* c, double rtol, double atol, int maxit, int* iters, double* relres) {
+  const Block& A = c->blk[0];
+  const int64_t n = A.nrows, nnode = n / 3;
+  int rc;
+  for (int k = 0; k < 4; ++k)
+    if ((rc = ensure(c, c->d_kw[k], n * 8))) return rc;
+  if ((rc = ensure(c, c->d_kw[4], nnode * 9 * 8))) return rc;
+  const int64_t nblk_rows = nb(n * 16, 256);
+  const int64_t nblk_nodes = nb(nnode, 256);
+  if ((rc = ensure(c, c->d_part, (size_t)std::max(nblk_rows, 2 * nblk_nodes) * 8))) return rc;
+  if ((rc = ensure(c, c->d_scal, 16 * 8))) return rc;
+  if ((rc = ensure_vec(c, FEDD_VEC_X, n))) return rc;
+  double* r = c->d_kw[0].as<double>();
+  double* z = c->d_kw[1].as<double>();
+  double* p = c->d_kw[2].as<double>();
+  double* Ap = c->d_kw[3].as<double>();
+  double* binv = c->d_kw[4].as<double>();
+  double* x = c->vec[FEDD_VEC_X].as<double>();
+  double* scal = c->d_scal.as<double>();
+  double* part = c->d_part.as<double>();
+  double* part2 = part + nblk_nodes;
+  const int64_t* rp = A.d_rowptr.as<int64_t>();
+  const int32_t* ci = A.d_colind.as<int32_t>();
+  const double* va = A.d_vals.as<double>();
+  k_block_diag<<<nb(nnode, 256), 256, 0, c->stream>>>(nnode, rp, ci, va, binv);
+  FEDD_HIP(c, hipMemcpyAsync(r, c->vec[FEDD_VEC_B].p, n * 8, hipMemcpyDeviceToDevice, c->stream));
+  k_fill<<<nb(n, 256), 256, 0, c->stream>>>(n, x, 0.0);
+  k_pcg_update<<<nblk_nodes, 256, 0, c->stream>>>(nnode, x, r, p, Ap, z, binv, scal, part, part2, 1);
+  k_reduce<<<1, 1024, 0, c->stream>>>(part, nblk_nodes, scal, 0, 0);   // rz
+  k_reduce<<<1, 1024, 0, c->stream>>>(part2, nblk_nodes, scal, 3, 0);  // rr
+  k_pcg_p<<<nb(n, 256), 256, 0, c->stream>>>(n, p, z, scal, 1);
+  FEDD_HIP(c, hipGetLastError());
+  double h[4];
+  FEDD_HIP(c, hipMemcpyAsync(h, scal, 4 * 8, hipMemcpyDeviceToHost, c->stream));
+  FEDD_HIP(c, hipStreamSynchronize(c->stream));
+  const double bnorm = std::sqrt(h[3]);
+  double rnorm = bnorm;
+  int it = 0;
+  const double target = std::max(rtol * bnorm, atol);
+  if (bnorm > 0.0 && rnorm > target) {
+    while (it < maxit) {
+      k_spmv_dot<<<nblk_rows, 256, 0, c->stream>>>(n, rp, ci, va, p, Ap, part);
+      k_reduce<<<1, 1024, 0, c->stream>>>(part, nblk_rows, scal, 1, 0);  // pAp
+      k_pcg_update<<<nblk_nodes, 256, 0, c->stream>>>(nnode, x, r, p, Ap, z, binv, scal, part, part2, 0);
+      k_reduce<<<1, 1024, 0, c->stream>>>(part, nblk_nodes, scal, 0, 1);  // rz, beta
+      k_reduce<<<1, 1024, 0, c->stream>>>(part2, nblk_nodes, scal, 3, 0);  // rr
+      k_pcg_p<<<nb(n, 256), 256, 0, c->stream>>>(n, p, z, scal, 0);
+      ++it;
+      if ((it & 7) == 0 || it == maxit) {
+        FEDD_HIP(c, hipGetLastError());
+        FEDD_HIP(c, hipMemcpyAsync(h, scal, 4 * 8, hipMemcpyDeviceToHost, c->stream));
+        FEDD_HIP(c, hipStreamSynchronize(c->stream));
+        rnorm = std::sqrt(h[3]);
+        if (!(rnorm == rnorm)) break;  // NaN: let the caller reject the step
+        if (rnorm <= target) break;
+      }
+    }
+  }
+  FEDD_HIP(c, hipGetLastError());
+  if (iters) *iters = it;
+  if (relres) *relres = bnorm > 0 ? rnorm / bnorm : 0.0;
+  return FEDD_OK;
+}
+
+int newton_residual(Ctx* c, double lam, double* norm) {
+  const int64_t n = 3 * c->mesh[0].npts;
+  const int64_t nblk = nb(n, 256);
+  int rc;
+  if ((rc = ensure(c, c->d_part, nblk * 8))) return rc;
+  if ((rc = ensure(c, c->d_scal, 16 * 8))) return rc;
+  if ((rc = ensure_vec(c, FEDD_VEC_B, n))) return rc;
+  if ((rc = ensure(c, c->d_bcflag, n))) return rc;
+  if (c->nbc == 0) FEDD_HIP(c, hipMemsetAsync(c->d_bcflag.p, 0, n, c->stream));
+  const double* fext = c->vec_n[FEDD_VEC_FEXT] == n ? c->vec[FEDD_VEC_FEXT].as<double>() : nullptr;
+  k_newton_residual<<<nblk, 256, 0, c->stream>>>(n, c->vec[FEDD_VEC_RES_U].as<double>(), fext, lam,
+                                                 c->d_bcflag.as<uint8_t>(), c->vec[FEDD_VEC_B].as<double>(),
+                                                 c->d_part.as<double>());
+  k_reduce<<<1, 1024, 0, c->stream>>>(c->d_part.as<double>(), nblk, c->d_scal.as<double>(), 6, 0);
+  FEDD_HIP(c, hipGetLastError());
+  double h = 0.0;
+  FEDD_HIP(c, hipMemcpyAsync(&h, c->d_scal.as<double>() + 6, 8, hipMemcpyDeviceToHost, c->stream));
+  FEDD_HIP(c, hipStreamSynchronize(c->stream));
+  *norm = std::sqrt(h);
+  return FEDD_OK;
+}
+
+// Traction on the face {x_axis == value}: a uniform traction t on a 6-node
+// triangle gives |T|/3 t to each edge node and 0 to the vertices.
+int traction_load(Ctx* c, int axis, double value, const double t[3], double tol) {
+  const Mesh& m = c->mesh[0];
+  const int64_t n = 3 * m.npts;
+  std::vector<double> f(n, 0.0);
+  static const int faces[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+  static const int edge_of[4][4] = {{-1, 4, 6, 7}, {4, -1, 5, 8}, {6, 5, -1, 9}, {7, 8, 9, -1}};
+  for (int64_t e = 0; e < m.nelem; ++e) {
+    const int32_t* nd = &m.elems[e * m.npe];
+    for (auto& fc : faces) {
+      bool on = true;
+      for (int k = 0; k < 3 && on; ++k) on = std::fabs(m.pts[3 * (int64_t)nd[fc[k]] + axis] - value) <= tol;
+      if (!on) continue;
+      const double* a = &m.pts[3 * (int64_t)nd[fc[0]]];
+      const double* b = &m.pts[3 * (int64_t)nd[fc[1]]];
+      const double* cc = &m.pts[3 * (int64_t)nd[fc[2]]];
+      const double u1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+      const double u2[3] = {cc[0] - a[0], cc[1] - a[1], cc[2] - a[2]};
+      const double cr[3] = {u1[1] * u2[2] - u1[2] * u2[1], u1[2] * u2[0] - u1[0] * u2[2],
+                            u1[0] * u2[1] - u1[1] * u2[0]};
+      const double area = 0.5 * std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+      const int pairs[3][2] = {{fc[0], fc[1]}, {fc[1], fc[2]}, {fc[0], fc[2]}};
+      for (auto& pr : pairs) {
+        const int32_t en = nd[edge_of[pr[0]][pr[1]]];
+        for (int d = 0; d < 3; ++d) f[3 * (int64_t)en + d] += area / 3.0 * t[d];
+      }
+    }
+  }
+  int rc = upload(c, c->vec[FEDD_VEC_FEXT], f.data(), n * 8);
+  if (rc) return rc;
+  c->vec_n[FEDD_VEC_FEXT] = n;
+  FEDD_HIP(c, hipStreamSynchronize(c->stream));
+  return FEDD_OK;
+}
+
+}  // namespace fedd
+
+using namespace fedd;
+
+extern "C" {
+
+int fedd_dirichlet_set(fedd_ctx* cc, const int32_t* dofs, int64_t n) {
+  Ctx* c = cc;
+  if (!c || (n > 0 && !dofs)) return FEDD_ERR_ARG;
+  const int64_t nd = 3 * c->mesh[0].npts;
+  if (nd == 0) return fail(c, FEDD_ERR_STATE, "no primary mesh");
+  std::vector<uint8_t> flag(nd, 0);
+  for (int64_t k = 0; k < n; ++k) {
+    if (dofs[k] < 0 || dofs[k] >= nd) return fail(c, FEDD_ERR_ARG, "Dirichlet dof out of range");
+    flag[dofs[k]] = 1;
+  }
+  (void)hipSetDevice(c->device);
+  int rc = upload(c, c->d_bcflag, flag.data(), nd);
+  if (rc) return rc;
+  c->nbc = n;
+  FEDD_HIP(c, hipStreamSynchronize(c->stream));
+  return FEDD_OK;
+}
+
+int fedd_traction_load(fedd_ctx* cc, int axis, double value, const double* traction) {
+  Ctx* c = cc;
+  if (!c || axis < 0 || axis > 2 || !traction) return FEDD_ERR_ARG;
+  if (c->mesh[0].npe != 10) return fail(c, FEDD_ERR_STATE, "traction loads need a P2 primary mesh");
+  (void)hipSetDevice(c->device);
+  return traction_load(c, axis, value, traction, 1e-12);
+}
+
+int fedd_linear_solve(fedd_ctx* cc, double rtol, int maxit, int* iters, double* relres) {
+  Ctx* c = cc;
+  if (!c || !(rtol > 0) || maxit < 1) return FEDD_ERR_ARG;
+  if (c->graph_kind != FEDD_GRAPH_VECTOR) return fail(c, FEDD_ERR_STATE, "needs a vector graph");
+  if (c->vec_n[FEDD_VEC_B] != c->blk[0].nrows) return fail(c, FEDD_ERR_STATE, "right-hand side not set");
+  (void)hipSetDevice(c->device);
+  return pcg_solve(c, rtol, 0.0, maxit, iters, relres);
+}
+
+int fedd_apply_dirichlet(fedd_ctx* cc) {
+  Ctx* c = cc;
+  if (!c) return FEDD_ERR_ARG;
+  if (c->graph_kind != FEDD_GRAPH_VECTOR) return fail(c, FEDD_ERR_STATE, "needs a vector graph");
+  (void)hipSetDevice(c->device);
+  return apply_dirichlet_matrix(c);
+}
+
+int fedd_newton_elasticity(fedd_ctx* cc, const fedd_newton_params* P, fedd_newton_stats* st,
+                           double* history, int history_cap) {
+  Ctx* c = cc;
+  if (!c || !P || !st) return FEDD_ERR_ARG;
+  if (c->graph_kind != FEDD_GRAPH_VECTOR) return fail(c, FEDD_ERR_STATE, "needs a vector (P2) graph");
+  if (P->load_steps < 1 || P->max_newton < 1 || !(P->abs_tol > 0)) return fail(c, FEDD_ERR_ARG, "bad Newton parameters");
+  if (P->tangent == 1 && !c->mlp_ready) return fail(c, FEDD_ERR_STATE, "surrogate tangent requested but no network");
+  (void)hipSetDevice(c->device);
+  const int64_t n = 3 * c->mesh[0].npts;
+  int rc;
+  if ((rc = ensure_vec(c, FEDD_VEC_U, n))) return rc;
+  if ((rc = ensure_vec(c, FEDD_VEC_RES_U, n))) return rc;
+  std::memset(st, 0, sizeof(*st));
+  const int model = P->neo_hooke_model == 1 ? 1 : 2;
+  auto t0 = std::chrono::steady_clock::now();
+  hipEvent_t ea, eb;
+  FEDD_HIP(c, hipEventCreate(&ea));
+  FEDD_HIP(c, hipEventCreate(&eb));
+  DevBuf u_prev;
+  if ((rc = alloc(c, u_prev, n * 8))) return rc;
+  double* u = c->vec[FEDD_VEC_U].as<double>();
+  int hist = 0;
+  double final_norm = 0.0;
+  bool all_converged = true;
+  for (int step = 1; step <= P->load_steps; ++step) {
+    const double lam = (double)step / P->load_steps;
+    bool surrogate = P->tangent == 1;
+    double prev = INFINITY, norm = 0.0;
+    bool converged = false;
+    for (int it = 0; it <= P->max_newton; ++it) {
+      FEDD_HIP(c, hipEventRecord(ea, c->stream));
+      if ((rc = launch_elasticity(c, model, P->E, P->nu, P->precision, 1, surrogate ? 2 : 1))) return rc;
+      FEDD_HIP(c, hipEventRecord(eb, c->stream));
+      if ((rc = newton_residual(c, lam, &norm))) return rc;
+      float ms = 0.f;
+      FEDD_HIP(c, hipEventElapsedTime(&ms, ea, eb));
+      st->assembly_ms += ms;
+      if (history && hist < history_cap) history[hist++] = norm;
+      final_norm = norm;
+      if (norm < P->abs_tol) {
+        converged = true;
+        break;
+      }
+      if (it == P->max_newton) break;
+      if (surrogate && !(norm <= P->stall_ratio * prev)) {
+        // the surrogate tangent stalled (or diverged): undo the last update and
+        // continue this load step with the exact tangent
+        if (std::isfinite(prev)) {
+          FEDD_HIP(c, hipMemcpyAsync(u, u_prev.p, n * 8, hipMemcpyDeviceToDevice, c->stream));
+          norm = prev;
+        }
+        surrogate = false;
+        st->fallbacks++;
+        it--;  // the rejected step does not count as an iteration
+        prev = INFINITY;
+        if ((rc = launch_elasticity(c, model, P->E, P->nu, P->precision, 1, 1))) return rc;
+        if ((rc = newton_residual(c, lam, &norm))) return rc;
+      }
+      // Newton step: K du = -R  (B holds R)
+      if ((rc = apply_dirichlet_matrix(c))) return rc;
+      k_scale_copy<<<nb(n, 256), 256, 0, c->stream>>>(n, -1.0, c->vec[FEDD_VEC_B].as<double>(),
+                                                      c->vec[FEDD_VEC_B].as<double>());
+      // inexact Newton forcing term (Eisenstat-Walker), never tighter than needed
+      double eta = std::isfinite(prev) ? 0.9 * (norm / prev) * (norm / prev) : 0.1;
+      eta = std::min(0.1, std::max(eta, P->lin_rtol));
+      eta = std::max(eta, 0.1 * P->abs_tol / std::max(norm, 1e-300));
+      int iters = 0;
+      double relres = 0.0;
+      FEDD_HIP(c, hipEventRecord(ea, c->stream));
+      if ((rc = pcg_solve(c, eta, 0.0, P->lin_maxit, &iters, &relres))) return rc;
+      FEDD_HIP(c, hipEventRecord(eb, c->stream));
+      FEDD_HIP(c, hipEventSynchronize(eb));
+      FEDD_HIP(c, hipEventElapsedTime(&ms, ea, eb));
+      st->solve_ms += ms;
+      st->krylov_iters += iters;
+      FEDD_HIP(c, hipMemcpyAsync(u_prev.p, u, n * 8, hipMemcpyDeviceToDevice, c->stream));
+      k_axpy<<<nb(n, 256), 256, 0, c->stream>>>(n, 1.0, c->vec[FEDD_VEC_X].as<double>(), u);
+      FEDD_HIP(c, hipGetLastError());
+      st->newton_iters++;
+      prev = norm;
+    }
+    if (!converged) all_converged = false;
+  }
+  FEDD_HIP(c, hipStreamSynchronize(c->stream));
+  auto t1 = std::chrono::steady_clock::now();
+  (void)hipEventDestroy(ea);
+  (void)hipEventDestroy(eb);
+  st->seconds = std::chrono::duration<double>(t1 - t0).count();
+  st->final_residual = final_norm;
+  st->converged = all_converged ? 1 : 0;
+  st->history_len = hist;
+  return FEDD_OK;
+}
+
+}  // extern "C"

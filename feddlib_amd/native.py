@@ -15,7 +15,7 @@ FEDD_OK = 0
 MESH_PRIMARY, MESH_PRESSURE = 0, 1
 GRAPH_SCALAR, GRAPH_VECTOR, GRAPH_NS = 0, 1, 2
 BLOCK_A, BLOCK_BT, BLOCK_B, BLOCK_C = 0, 1, 2, 3
-VEC_U, VEC_P, VEC_RES_U, VEC_RES_P, VEC_RHS = 0, 1, 2, 3, 4
+VEC_U, VEC_P, VEC_RES_U, VEC_RES_P, VEC_RHS, VEC_FEXT, VEC_X, VEC_B = 0, 1, 2, 3, 4, 5, 6, 7
 FP64, FP32_LOCAL = 0, 1
 NS_JACOBIAN, NS_FIXEDPOINT, NS_RHS = 0, 1, 2
 
@@ -28,6 +28,22 @@ _vp = ctypes.c_void_p
 class NsParams(ctypes.Structure):
     _fields_ = [("viscosity", ctypes.c_double), ("density", ctypes.c_double),
                 ("newton", ctypes.c_int), ("coeff", ctypes.c_double * 4)]
+
+
+class NewtonParams(ctypes.Structure):
+    _fields_ = [("neo_hooke_model", ctypes.c_int), ("E", ctypes.c_double), ("nu", ctypes.c_double),
+                ("load_steps", ctypes.c_int), ("abs_tol", ctypes.c_double), ("max_newton", ctypes.c_int),
+                ("lin_rtol", ctypes.c_double), ("lin_maxit", ctypes.c_int), ("tangent", ctypes.c_int),
+                ("precision", ctypes.c_int), ("stall_ratio", ctypes.c_double)]
+
+
+class NewtonStats(ctypes.Structure):
+    _fields_ = [("newton_iters", ctypes.c_int), ("fallbacks", ctypes.c_int), ("krylov_iters", ctypes.c_int64),
+                ("seconds", ctypes.c_double), ("assembly_ms", ctypes.c_double), ("solve_ms", ctypes.c_double),
+                ("final_residual", ctypes.c_double), ("converged", ctypes.c_int), ("history_len", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class FeddError(RuntimeError):
@@ -67,6 +83,12 @@ _SIGS = {
     "fedd_assembly_linear_elasticity": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_int]),
     "fedd_assembly_nonlinear_elasticity": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                                           ctypes.c_int]),
+    "fedd_dirichlet_set": (ctypes.c_int, [_vp, _ip, ctypes.c_int64]),
+    "fedd_apply_dirichlet": (ctypes.c_int, [_vp]),
+    "fedd_traction_load": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, _dp]),
+    "fedd_linear_solve": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _ip, _dp]),
+    "fedd_newton_elasticity": (ctypes.c_int, [_vp, ctypes.POINTER(NewtonParams), ctypes.POINTER(NewtonStats),
+                                              _dp, ctypes.c_int]),
     "fedd_stream_copy": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]),
     "fedd_event_record": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fedd_event_elapsed": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int,
@@ -235,6 +257,33 @@ class Context:
 
     def assembly_nonlinear_elasticity(self, neo_hooke_model, E, nu, precision=FP64):
         self._chk(self._L.fedd_assembly_nonlinear_elasticity(self.h, neo_hooke_model, E, nu, precision))
+
+    def dirichlet_set(self, dofs):
+        d = np.ascontiguousarray(dofs, np.int32)
+        self._chk(self._L.fedd_dirichlet_set(self.h, _p(d, _ip), d.size))
+
+    def apply_dirichlet(self):
+        self._chk(self._L.fedd_apply_dirichlet(self.h))
+
+    def traction_load(self, axis, value, t):
+        t = np.ascontiguousarray(t, np.float64)
+        self._chk(self._L.fedd_traction_load(self.h, axis, value, _p(t, _dp)))
+
+    def linear_solve(self, rtol, maxit):
+        it, rr = ctypes.c_int(), ctypes.c_double()
+        self._chk(self._L.fedd_linear_solve(self.h, rtol, maxit, ctypes.byref(it), ctypes.byref(rr)))
+        return it.value, rr.value
+
+    def newton_elasticity(self, model=1, E=1000.0, nu=0.3, load_steps=5, abs_tol=1e-8, max_newton=30,
+                          lin_rtol=1e-10, lin_maxit=20000, tangent=0, precision=FP64, stall_ratio=0.9,
+                          history_cap=512):
+        P = NewtonParams(model, E, nu, load_steps, abs_tol, max_newton, lin_rtol, lin_maxit, tangent, precision,
+                         stall_ratio)
+        st = NewtonStats()
+        hist = np.zeros(history_cap)
+        self._chk(self._L.fedd_newton_elasticity(self.h, ctypes.byref(P), ctypes.byref(st), _p(hist, _dp),
+                                                 history_cap))
+        return st.as_dict(), hist[:st.history_len]
 
     def newton_step(self):
         s = ctypes.c_int()

@@ -63,8 +63,12 @@ enum fedd_vector {
   FEDD_VEC_P = 1,      /* pressure iterate, 1 per pressure node */
   FEDD_VEC_RES_U = 2,  /* residual, velocity / displacement block */
   FEDD_VEC_RES_P = 3,  /* residual, pressure block */
-  FEDD_VEC_RHS = 4     /* load vector of FE::assemblyRHS (1 per primary node) */
+  FEDD_VEC_RHS = 4,    /* load vector of FE::assemblyRHS (1 per primary node) */
+  FEDD_VEC_FEXT = 5,   /* external load of the elasticity problem, 3 per node */
+  FEDD_VEC_X = 6,      /* linear-solve solution / Newton increment, 3 per node */
+  FEDD_VEC_B = 7       /* linear-solve right-hand side, 3 per node */
 };
+#define FEDD_NUM_VECTORS 8
 
 enum fedd_precision {
   FEDD_FP64 = 0,       /* everything in FP64 */
@@ -160,6 +164,49 @@ int fedd_assembly_linear_elasticity(fedd_ctx* ctx, double E, double nu, int prec
  * neo_hooke_model = Parameter/"Neo-Hooke Modell": 1 -> skr2, otherwise skr3. */
 int fedd_assembly_nonlinear_elasticity(fedd_ctx* ctx, int neo_hooke_model, double E, double nu,
                                        int precision);
+/* ------------------------------------------------------- nonlinear solve */
+/* Dirichlet dofs of the primary field (BCBuilder::addBC / setSystem,
+ * BCBuilder_def.hpp:590-618): constrained rows and columns of the assembled tangent
+ * become identity rows/columns, their residual entries zero. */
+int fedd_dirichlet_set(fedd_ctx* ctx, const int32_t* dofs, int64_t n);
+int fedd_apply_dirichlet(fedd_ctx* ctx);
+/* Uniform traction t on the face {x_axis == value} of the P2 mesh into VEC_FEXT
+ * (surface term of the load-stepped source, NonLinElasAssFE_def.hpp:262-325). */
+int fedd_traction_load(fedd_ctx* ctx, int axis, double value, const double* traction);
+/* Block-Jacobi preconditioned CG on block A: solves A x = VEC_B into VEC_X. */
+int fedd_linear_solve(fedd_ctx* ctx, double rtol, int maxit, int* iters, double* relres);
+
+typedef struct {
+  int neo_hooke_model;  /* Parameter/"Neo-Hooke Modell": 1 -> skr2 law, otherwise skr3 */
+  double E, nu;
+  int load_steps;       /* lambda = k/load_steps, k = 1..load_steps */
+  double abs_tol;       /* converged when ||F_int - lambda F_ext||_2 < abs_tol (constrained dofs 0) */
+  int max_newton;       /* per load step */
+  double lin_rtol;      /* floor of the inexact-Newton forcing term */
+  int lin_maxit;
+  int tangent;          /* 0: exact tangent, 1: learned surrogate (exact residual always) */
+  int precision;        /* FEDD_FP64 or FEDD_FP32_LOCAL for the tangent */
+  double stall_ratio;   /* surrogate: fall back to the exact tangent when ||R_k+1|| > ratio ||R_k|| */
+} fedd_newton_params;
+
+typedef struct {
+  int newton_iters;
+  int fallbacks;
+  int64_t krylov_iters;
+  double seconds;        /* wall time of the whole solve */
+  double assembly_ms;    /* tangent + residual assembly (device time) */
+  double solve_ms;       /* Krylov solves (device time) */
+  double final_residual;
+  int converged;
+  int history_len;
+} fedd_newton_stats;
+
+/* NonLinearSolver::solveNewton (NonLinearSolver_def.hpp:356-414) for the hyperelastic
+ * problem on the device: VEC_U is the initial displacement and the solution;
+ * history (may be NULL) receives ||R|| at every Newton iterate. */
+int fedd_newton_elasticity(fedd_ctx* ctx, const fedd_newton_params* params, fedd_newton_stats* stats,
+                           double* history, int history_cap);
+
 /* AssembleFE::advanceNewtonStep / getNewtonStep (AssembleFE_decl.hpp:146) for the
  * NS elements: the counter the reference keeps per element. */
 int fedd_newton_step(fedd_ctx* ctx, int* step);
