@@ -111,8 +111,8 @@ __device__ inline double p2_grad(const double x[10][3], const double xi[3], doub
 }
 
 // Exact S and CC (Voigt) for the three laws, from C = F^T F.
-__device__ inline void exact_law(int model, double Emod, double nu, const double F[3][3], double S6[6],
-                                 double CC21[21]) {
+__host__ __device__ void exact_law(int model, double Emod, double nu, const double F[3][3], double S6[6],
+                                   double CC21[21]) {
   double C[3][3];
 #pragma unroll
   for (int A = 0; A < 3; ++A)
@@ -159,6 +159,7 @@ __device__ inline void exact_law(int model, double Emod, double nu, const double
   const int vA[6] = {0, 1, 2, 0, 0, 1}, vB[6] = {0, 1, 2, 1, 2, 2};
 #pragma unroll
   for (int I = 0; I < 6; ++I) S6[I] = (vA[I] == vB[I] ? a_id : 0.0) + a_ci * Ci[vA[I]][vB[I]];
+  if (!CC21) return;  // stress only (the surrogate supplies the tangent)
   int k = 0;
 #pragma unroll
   for (int I = 0; I < 6; ++I)
@@ -226,7 +227,7 @@ __global__ __launch_bounds__(256) void k_elas_qp(ElasArgs a) {
     }
   double* o = a.qp + 52 * t;
   double S6[6], CC[21];
-  exact_law(a.model, a.Emod, a.nu, F, S6, CC);
+  exact_law(a.model, a.Emod, a.nu, F, S6, a.exact_tangent ? CC : nullptr);
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll

@@ -400,6 +400,7 @@ void fedd_ctx_destroy(fedd_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  mlp_free(c);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   ktimer_resolve(c);

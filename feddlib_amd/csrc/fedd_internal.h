@@ -103,7 +103,7 @@ struct Ctx {
   DevBuf d_part;        // per-block partial sums
   DevBuf d_scal;        // device scalars of the Krylov loop
   // learned tangent surrogate (FP32 master weights + BF16 copies)
-  DevBuf d_mlp;         // parameters, gradients and Adam moments
+  struct MlpHost* mlp = nullptr;
   bool mlp_ready = false;
   int newton_step = 0;
   // timing
@@ -149,6 +149,14 @@ int launch_geometry(Ctx* c);
 // tangent: 1 = exact law, 2 = learned surrogate (the network writes the tangent slot)
 int launch_elasticity(Ctx* c, int model, double Emod, double nu, int precision, int with_res, int tangent);
 int launch_mlp_tangent(Ctx* c, int nq);
+void mlp_free(Ctx* c);
+// True when the loaded network was built for this law and these constants.
+bool mlp_matches(const Ctx* c, int model, double Emod, double nu);
+// Exact second Piola-Kirchhoff stress (Voigt 6) and tangent dS/dE (21, upper
+// triangle of the 6x6 Voigt matrix, row-major) of the three hyperelastic laws;
+// CC21 may be null (stress only).
+__host__ __device__ void exact_law(int model, double Emod, double nu, const double F[3][3], double S6[6],
+                                   double CC21[21]);
 
 }  // namespace fedd
 

@@ -419,14 +419,16 @@ int fedd_newton_elasticity(fedd_ctx* cc, const fedd_newton_params* P, fedd_newto
   if (!c || !P || !st) return FEDD_ERR_ARG;
   if (c->graph_kind != FEDD_GRAPH_VECTOR) return fail(c, FEDD_ERR_STATE, "needs a vector (P2) graph");
   if (P->load_steps < 1 || P->max_newton < 1 || !(P->abs_tol > 0)) return fail(c, FEDD_ERR_ARG, "bad Newton parameters");
+  const int model = P->neo_hooke_model == 1 ? 1 : 2;
   if (P->tangent == 1 && !c->mlp_ready) return fail(c, FEDD_ERR_STATE, "surrogate tangent requested but no network");
+  if (P->tangent == 1 && !mlp_matches(c, model, P->E, P->nu))
+    return fail(c, FEDD_ERR_STATE, "the surrogate was built for another law or other material constants");
   (void)hipSetDevice(c->device);
   const int64_t n = 3 * c->mesh[0].npts;
   int rc;
   if ((rc = ensure_vec(c, FEDD_VEC_U, n))) return rc;
   if ((rc = ensure_vec(c, FEDD_VEC_RES_U, n))) return rc;
   std::memset(st, 0, sizeof(*st));
-  const int model = P->neo_hooke_model == 1 ? 1 : 2;
   auto t0 = std::chrono::steady_clock::now();
   hipEvent_t ea, eb;
   FEDD_HIP(c, hipEventCreate(&ea));
@@ -457,19 +459,26 @@ int fedd_newton_elasticity(fedd_ctx* cc, const fedd_newton_params* P, fedd_newto
         break;
       }
       if (it == P->max_newton) break;
-      if (surrogate && !(norm <= P->stall_ratio * prev)) {
-        // the surrogate tangent stalled (or diverged): undo the last update and
-        // continue this load step with the exact tangent
-        if (std::isfinite(prev)) {
+      if (surrogate && std::isfinite(prev) && !(norm <= P->stall_ratio * prev)) {
+        // The surrogate tangent stalled: continue this load step with the exact
+        // tangent.  A step that increased the residual (or made it non-finite) is
+        // rejected first; a merely slow one is kept.
+        if (!(norm <= prev)) {
           FEDD_HIP(c, hipMemcpyAsync(u, u_prev.p, n * 8, hipMemcpyDeviceToDevice, c->stream));
-          norm = prev;
+          st->rejected++;
         }
         surrogate = false;
         st->fallbacks++;
-        it--;  // the rejected step does not count as an iteration
-        prev = INFINITY;
+        FEDD_HIP(c, hipEventRecord(ea, c->stream));
         if ((rc = launch_elasticity(c, model, P->E, P->nu, P->precision, 1, 1))) return rc;
+        FEDD_HIP(c, hipEventRecord(eb, c->stream));
         if ((rc = newton_residual(c, lam, &norm))) return rc;
+        FEDD_HIP(c, hipEventElapsedTime(&ms, ea, eb));
+        st->assembly_ms += ms;
+        if (norm < P->abs_tol) {
+          converged = true;
+          break;
+        }
       }
       // Newton step: K du = -R  (B holds R)
       if ((rc = apply_dirichlet_matrix(c))) return rc;

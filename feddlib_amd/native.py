@@ -23,6 +23,8 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
 _lp = ctypes.POINTER(ctypes.c_int64)
 _vp = ctypes.c_void_p
+_fp = ctypes.POINTER(ctypes.c_float)
+MLP_PARAMS = 7328
 
 
 class NsParams(ctypes.Structure):
@@ -38,7 +40,8 @@ class NewtonParams(ctypes.Structure):
 
 
 class NewtonStats(ctypes.Structure):
-    _fields_ = [("newton_iters", ctypes.c_int), ("fallbacks", ctypes.c_int), ("krylov_iters", ctypes.c_int64),
+    _fields_ = [("newton_iters", ctypes.c_int), ("fallbacks", ctypes.c_int), ("rejected", ctypes.c_int),
+                ("krylov_iters", ctypes.c_int64),
                 ("seconds", ctypes.c_double), ("assembly_ms", ctypes.c_double), ("solve_ms", ctypes.c_double),
                 ("final_residual", ctypes.c_double), ("converged", ctypes.c_int), ("history_len", ctypes.c_int)]
 
@@ -89,6 +92,16 @@ _SIGS = {
     "fedd_linear_solve": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _ip, _dp]),
     "fedd_newton_elasticity": (ctypes.c_int, [_vp, ctypes.POINTER(NewtonParams), ctypes.POINTER(NewtonStats),
                                               _dp, ctypes.c_int]),
+    "fedd_mlp_init": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_uint64]),
+    "fedd_mlp_add_samples_random": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_double, ctypes.c_uint64]),
+    "fedd_mlp_add_samples_qp": (ctypes.c_int, [_vp]),
+    "fedd_mlp_num_samples": (ctypes.c_int, [_vp, _lp]),
+    "fedd_mlp_train": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint64, _fp]),
+    "fedd_mlp_params": (ctypes.c_int, [_vp, _fp]),
+    "fedd_mlp_set_params": (ctypes.c_int, [_vp, _fp]),
+    "fedd_mlp_forward": (ctypes.c_int, [_vp, _fp, ctypes.c_int64, _fp]),
+    "fedd_mlp_gradient": (ctypes.c_int, [_vp, _fp, _fp, ctypes.c_int64, _fp, _fp]),
+    "fedd_mlp_norm": (ctypes.c_int, [_vp, _fp, _fp, _dp]),
     "fedd_stream_copy": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]),
     "fedd_event_record": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fedd_event_elapsed": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int,
@@ -275,7 +288,7 @@ class Context:
         return it.value, rr.value
 
     def newton_elasticity(self, model=1, E=1000.0, nu=0.3, load_steps=5, abs_tol=1e-8, max_newton=30,
-                          lin_rtol=1e-10, lin_maxit=20000, tangent=0, precision=FP64, stall_ratio=0.9,
+                          lin_rtol=1e-10, lin_maxit=20000, tangent=0, precision=FP64, stall_ratio=0.5,
                           history_cap=512):
         P = NewtonParams(model, E, nu, load_steps, abs_tol, max_newton, lin_rtol, lin_maxit, tangent, precision,
                          stall_ratio)
@@ -289,6 +302,57 @@ class Context:
         s = ctypes.c_int()
         self._chk(self._L.fedd_newton_step(self.h, ctypes.byref(s)))
         return s.value
+
+    # learned tangent surrogate
+    def mlp_init(self, model=1, E=1000.0, nu=0.3, seed=0):
+        self._chk(self._L.fedd_mlp_init(self.h, model, E, nu, seed))
+
+    def mlp_add_samples_random(self, n, sigma=0.1, seed=2):
+        self._chk(self._L.fedd_mlp_add_samples_random(self.h, n, sigma, seed))
+
+    def mlp_add_samples_qp(self):
+        self._chk(self._L.fedd_mlp_add_samples_qp(self.h))
+
+    def mlp_num_samples(self):
+        n = ctypes.c_int64()
+        self._chk(self._L.fedd_mlp_num_samples(self.h, ctypes.byref(n)))
+        return n.value
+
+    def mlp_train(self, steps, batch=4096, lr=1e-3, seed=0, history=True):
+        h = np.zeros(max(steps, 1), np.float32)
+        self._chk(self._L.fedd_mlp_train(self.h, steps, batch, lr, seed, _p(h, _fp) if history else None))
+        return h[:steps]
+
+    def mlp_params(self):
+        p = np.zeros(MLP_PARAMS, np.float32)
+        self._chk(self._L.fedd_mlp_params(self.h, _p(p, _fp)))
+        return p
+
+    def mlp_set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.shape == (MLP_PARAMS,)
+        self._chk(self._L.fedd_mlp_set_params(self.h, _p(p, _fp)))
+
+    def mlp_forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros((x.shape[0], 21), np.float32)
+        self._chk(self._L.fedd_mlp_forward(self.h, _p(x, _fp), x.shape[0], _p(y, _fp)))
+        return y
+
+    def mlp_gradient(self, x, t):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = np.zeros(1, np.float32)
+        self._chk(self._L.fedd_mlp_gradient(self.h, _p(x, _fp), _p(t, _fp), x.shape[0], _p(g, _fp),
+                                            _p(loss, _fp)))
+        return g, float(loss[0])
+
+    def mlp_norm(self):
+        cc0 = np.zeros(21, np.float32)
+        so, si = ctypes.c_float(), ctypes.c_double()
+        self._chk(self._L.fedd_mlp_norm(self.h, _p(cc0, _fp), ctypes.byref(so), ctypes.byref(si)))
+        return cc0, so.value, si.value
 
     # timing
     def synchronize(self):

@@ -186,12 +186,15 @@ typedef struct {
   int lin_maxit;
   int tangent;          /* 0: exact tangent, 1: learned surrogate (exact residual always) */
   int precision;        /* FEDD_FP64 or FEDD_FP32_LOCAL for the tangent */
-  double stall_ratio;   /* surrogate: fall back to the exact tangent when ||R_k+1|| > ratio ||R_k|| */
+  double stall_ratio;   /* surrogate: switch to the exact tangent for the rest of the load step
+                           when ||R_k+1|| > stall_ratio ||R_k|| (the step is kept), and reject
+                           the step first when ||R_k+1|| > ||R_k|| or is not finite */
 } fedd_newton_params;
 
 typedef struct {
-  int newton_iters;
-  int fallbacks;
+  int newton_iters;      /* linear solves (Newton updates), rejected ones included */
+  int fallbacks;         /* load steps switched from the surrogate to the exact tangent */
+  int rejected;          /* surrogate steps rejected (residual grew or became non-finite) */
   int64_t krylov_iters;
   double seconds;        /* wall time of the whole solve */
   double assembly_ms;    /* tangent + residual assembly (device time) */
@@ -206,6 +209,38 @@ typedef struct {
  * history (may be NULL) receives ||R|| at every Newton iterate. */
 int fedd_newton_elasticity(fedd_ctx* ctx, const fedd_newton_params* params, fedd_newton_stats* stats,
                            double* history, int history_cap);
+
+/* ------------------------------------------------- learned tangent surrogate
+ * None of these has a counterpart in the reference (FEDDLib always evaluates the
+ * AceGen tangent).  A 9 -> 64 -> 64 -> 21 GELU MLP maps x = (F - I)/s_in to
+ * y = (CC - CC0)/s_out, CC the 21 upper-triangle entries of the 6x6 Voigt tangent
+ * dS/dE (order 11,22,33,12,13,23), CC0 the tangent at F = I, s_out = lambda + 2 mu,
+ * s_in = 0.1.  BF16 MFMA with FP32 accumulation, FP32 master weights, Adam.
+ * Parameter vector (FEDD_MLP_PARAMS floats, row-major, zero padding):
+ *   W1[64][16] b1[64] W2[64][64] b2[64] W3[32][64] b3[32]
+ * (inputs 9 padded to 16, outputs 21 padded to 32). */
+#define FEDD_MLP_PARAMS 7328
+/* New network for law neo_hooke_model (as fedd_newton_params) with Xavier-normal
+ * weights from `seed`; drops any training set. */
+int fedd_mlp_init(fedd_ctx* ctx, int neo_hooke_model, double E, double nu, uint64_t seed);
+/* Training samples with exact-law targets: F = I + N(0, sigma^2) per entry (det F > 0.2). */
+int fedd_mlp_add_samples_random(fedd_ctx* ctx, int64_t n, double sigma, uint64_t seed);
+/* Training samples from the quadrature points of the last hyperelastic assembly
+ * (the solver's own iterates). */
+int fedd_mlp_add_samples_qp(fedd_ctx* ctx);
+int fedd_mlp_num_samples(fedd_ctx* ctx, int64_t* n);
+/* `steps` Adam steps (batches of `batch` from shuffled epochs, learning rate decayed
+ * from lr to lr/50 on a cosine) on the device;
+ * loss_history (may be NULL) gets the batch loss sum_o (y_o - t_o)^2 / batch. */
+int fedd_mlp_train(fedd_ctx* ctx, int steps, int batch, float lr, uint64_t seed, float* loss_history);
+int fedd_mlp_params(fedd_ctx* ctx, float* params);            /* read FEDD_MLP_PARAMS floats */
+int fedd_mlp_set_params(fedd_ctx* ctx, const float* params);  /* load (checkpoint / tests) */
+/* Network output y[n][21] for normalised inputs x[n][9]. */
+int fedd_mlp_forward(fedd_ctx* ctx, const float* x, int64_t n, float* y);
+/* Gradient of the batch loss over all n samples (as one batch) w.r.t. the parameters. */
+int fedd_mlp_gradient(fedd_ctx* ctx, const float* x, const float* t, int64_t n, float* grad, float* loss);
+/* Normalisation constants: cc0[21], s_out, s_in. */
+int fedd_mlp_norm(fedd_ctx* ctx, float* cc0, float* s_out, double* s_in);
 
 /* AssembleFE::advanceNewtonStep / getNewtonStep (AssembleFE_decl.hpp:146) for the
  * NS elements: the counter the reference keeps per element. */
