@@ -6,8 +6,14 @@ all blocks A(0,0), B^T(0,1), B(1,0) (C(1,1) is empty) -- on the unit cube with
 Linearization "Newton", velocity iterate = 8 seeded sine modes of amplitude 1.
 A "step" is one such assembly with the inputs resident in HBM.
 
+Second leg (the metric's other half): Neo-Hookean Newton time-to-||r||<1e-8 on the
+clamped unit cube 24^3 x 6 tets (P2, skr2 law, E = 1000, nu = 0.3), uniform traction
+(40, 0, 60) on z = 1 ramped in 5 load steps, with the exact tangent and with the
+learned surrogate (trained on the device beforehand; training time reported apart).
+
 Prints ONE JSON line (rank 0).  Usage:
   python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--M 40] [--cpu-sample 50000]
+                  [--newton-M 24] [--no-newton]
 """
 import argparse
 import json
@@ -45,10 +51,13 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=50000,
                     help="elements of the CPU-baseline sample (0 disables it)")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--newton-M", type=int, default=24)
+    ap.add_argument("--no-newton", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: every box here has a single MI355X and this build "
-              "has no multi-GPU path (see DESIGN.md, 'How it would shard').", file=sys.stderr)
+        print("bench.py measures one GPU only: this build has no multi-GPU path (the elements would be "
+              "partitioned into sub-cubes, one per GPU, with one interface-row exchange per assembly); "
+              "run it with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     from feddlib_amd import native as N
@@ -125,7 +134,7 @@ def main():
                          "insertion, Tpetra-style fillComplete); %.1f s on %s" % (n, E, secs, cpu_model())}
 
     out = {
-        "metric": "P2-P1 NS Jacobian elems/s",
+        "metric": "P2-P1 NS Jacobian elems/s; Neo-Hookean Newton time-to-1e-8, MLP vs exact tangent",
         "value": E / (ms64 * 1e-3),
         "unit": "elems/s",
         "n_gpus": 1,
@@ -162,8 +171,104 @@ def main():
         },
         "cpu_baseline": cpu,
     }
+    if not args.no_newton:
+        out["newton"] = newton_leg(ctx, N, args.newton_M)
     ctx.close()
     print(json.dumps(out))
+
+
+EMOD, POISSON = 1000.0, 0.3
+TRACTION = np.array([40.0, 0.0, 60.0])
+
+
+def _nh_problem(N, M):
+    pts, _, el = N.mesh_structured(M, 2)
+    el = np.ascontiguousarray(el)
+    N.correct_orientation(pts, el)
+    base = np.where(np.abs(pts[:, 2]) < 1e-12)[0]
+    dofs = np.sort((3 * base[:, None] + np.arange(3)).ravel()).astype(np.int32)
+    return pts, el, dofs
+
+
+def _nh_setup(ctx, N, prob):
+    pts, el, dofs = prob
+    ctx.mesh_upload(N.MESH_PRIMARY, pts, el)
+    ctx.graph_build(N.GRAPH_VECTOR)
+    ctx.vector_upload(N.VEC_U, np.zeros(3 * pts.shape[0]))
+    ctx.dirichlet_set(dofs)
+    ctx.traction_load(2, 1.0, TRACTION)
+
+
+def _nh_solve(ctx, N, prob, tangent, collect=False):
+    _nh_setup(ctx, N, prob)
+    st, hist = ctx.newton_elasticity(model=1, E=EMOD, nu=POISSON, load_steps=5, abs_tol=1e-8, max_newton=40,
+                                     tangent=tangent, collect_samples=collect)
+    return st, ctx.vector_download(N.VEC_U, 3 * prob[0].shape[0])
+
+
+def newton_leg(ctx, N, M):
+    """Time-to-tolerance of the load-stepped Newton solve, exact vs learned tangent."""
+    prob = _nh_problem(N, M)
+    nelem, nnode = prob[1].shape[0], prob[0].shape[0]
+    keys = ("seconds", "newton_iters", "krylov_iters", "assembly_ms", "solve_ms", "final_residual", "converged",
+            "fallbacks", "rejected")
+    _nh_solve(ctx, N, prob, 0)  # warm-up: allocations, code objects
+    ctx.kernel_timing(True)
+    st_e, u_e = _nh_solve(ctx, N, prob, 0)
+    k_qp, k_rows = ctx.kernel_time("elas_qp"), ctx.kernel_time("elas_rows")
+    ctx.kernel_timing(False)
+    print("# newton exact %.3fs (%d Newton, %d Krylov)" % (st_e["seconds"], st_e["newton_iters"],
+                                                          st_e["krylov_iters"]), file=sys.stderr, flush=True)
+    nnz = ctx.block_info(N.BLOCK_A)[2]
+
+    # surrogate: 10^6 samples F = I + N(0, 0.1) plus the iterates of the same
+    # problem solved on an 8^3 x 6 mesh, then Adam on the device
+    t0 = time.time()
+    ctx.mlp_init(1, EMOD, POISSON, 0)
+    ctx.mlp_add_samples_random(1000000, 0.1, 2)
+    _nh_solve(ctx, N, _nh_problem(N, 8), 0, collect=True)
+    nsamp = ctx.mlp_num_samples()
+    loss = ctx.mlp_train(5000, 8192, 3e-3, 0, history=True)
+    train_s = time.time() - t0
+    print("# surrogate trained on %d samples in %.2fs (loss %.2e)" % (nsamp, train_s, loss[-50:].mean()),
+          file=sys.stderr, flush=True)
+
+    _nh_solve(ctx, N, prob, 1)  # warm-up
+    ctx.kernel_timing(True)
+    st_m, u_m = _nh_solve(ctx, N, prob, 1)
+    k_mlp = ctx.kernel_time("mlp_tangent")
+    ctx.kernel_timing(False)
+    print("# newton mlp %.3fs (%d Newton, %d Krylov, %d fallbacks)" % (
+        st_m["seconds"], st_m["newton_iters"], st_m["krylov_iters"], st_m["fallbacks"]), file=sys.stderr,
+        flush=True)
+
+    # roofline of the solve: every Krylov iteration streams the matrix (8 B value +
+    # 4 B column) and ~8 vectors; every Newton iteration assembles once
+    n = 3 * nnode
+    krylov_bytes = 12 * nnz + 8 * 8 * n
+    asm_bytes = 8 * nnz + 4 * 10 * nelem + 24 * nnode + 8 * n + 8 * n
+    def frac(st):
+        b = st["krylov_iters"] * krylov_bytes + st["newton_iters"] * asm_bytes
+        return b / st["seconds"] / 1e9 / HBM_PEAK_GBS
+    nqp = 5 * nelem
+    mlp_ms = k_mlp[0] / max(k_mlp[1], 1)
+    return {
+        "workload": "Neo-Hookean (skr2) Newton, unit cube %d^3x6 = %d P2 tets, %d dofs, z=0 clamped, traction "
+                    "(40,0,60) on z=1 in 5 load steps, ||r||_2 < 1e-8, block-Jacobi PCG with Eisenstat-Walker "
+                    "forcing" % (M, nelem, n),
+        "metric": "Neo-Hookean Newton time-to-1e-8, MLP vs exact tangent",
+        "unit": "s", "higher_is_better": False,
+        "exact": dict({k: st_e[k] for k in keys}, roofline_frac=frac(st_e)),
+        "mlp": dict({k: st_m[k] for k in keys}, roofline_frac=frac(st_m)),
+        "mlp_over_exact_time": st_m["seconds"] / st_e["seconds"],
+        "solution_max_rel_diff": float(np.max(np.abs(u_m - u_e)) / np.max(np.abs(u_e))),
+        "train": {"seconds": train_s, "samples": nsamp, "adam_steps": 5000, "batch": 8192,
+                  "final_loss": float(loss[-50:].mean())},
+        "elas_kernels_avg_ms": {"qp": k_qp[0] / max(k_qp[1], 1), "rows": k_rows[0] / max(k_rows[1], 1)},
+        "mlp_tangent": {"avg_ms": mlp_ms, "qp_per_launch": nqp,
+                        "tflops": 12032.0 * nqp / (mlp_ms * 1e-3) / 1e12 if mlp_ms > 0 else None,
+                        "peak_bf16_tflops": 2500.0},
+    }
 
 
 if __name__ == "__main__":

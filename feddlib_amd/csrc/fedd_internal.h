@@ -89,7 +89,8 @@ struct Ctx {
   GatherPlan vplan, pplan;        // velocity (primary) and pressure row gathers
   DevBuf d_geo;                   // per element: grad lambda_0..3 (12) + |det| + pad, 16 doubles
   DevBuf d_upad;                  // velocity iterate padded to 4 doubles per node
-  DevBuf d_qp;                    // elasticity quadrature-point records (40 doubles each)
+  DevBuf d_qp;                    // elasticity quadrature-point records (52 doubles each)
+  int qp_nq = 0;                  // points per element of the records in d_qp (0: none yet)
   // colouring
   std::vector<int64_t> colour_ptr;
   std::vector<int32_t> colour_perm;
@@ -152,11 +153,8 @@ int launch_mlp_tangent(Ctx* c, int nq);
 void mlp_free(Ctx* c);
 // True when the loaded network was built for this law and these constants.
 bool mlp_matches(const Ctx* c, int model, double Emod, double nu);
-// Exact second Piola-Kirchhoff stress (Voigt 6) and tangent dS/dE (21, upper
-// triangle of the 6x6 Voigt matrix, row-major) of the three hyperelastic laws;
-// CC21 may be null (stress only).
-__host__ __device__ void exact_law(int model, double Emod, double nu, const double F[3][3], double S6[6],
-                                   double CC21[21]);
+// Appends the quadrature-point states in d_qp to the surrogate's training set.
+int mlp_collect_qp(Ctx* c);
 
 }  // namespace fedd
 

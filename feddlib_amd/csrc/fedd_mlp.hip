@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "fedd_internal.h"
+#include "fedd_law.h"
 
 namespace fedd {
 
@@ -446,11 +447,10 @@ __global__ __launch_bounds__(256) void k_mlp_refresh(const float* __restrict__ p
 struct MlpHost {
   DevBuf buf;        // p, g, m, v (P_TOTAL floats each) + wb (Q_TOTAL shorts)
   DevBuf loss;       // 1 float
-  DevBuf d_x, d_t;   // training set on the device
+  DevBuf d_x, d_t;   // training set on the device: x[cap][9], t[cap][21] (normalised)
   DevBuf d_idx;      // batch order
   DevBuf d_io;       // scratch for forward / gradient calls
-  int64_t dev_n = 0;
-  std::vector<float> xs, ts;  // training set (normalised), host staging
+  int64_t n = 0, cap = 0;
   MlpNorm nrm;
   int model = 1;
   double E = 1000, nu = 0.3;
@@ -497,12 +497,66 @@ static double gauss(uint64_t& s) {
   return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
 }
 
-static void add_sample(MlpHost* h, const double F[3][3]) {
+static void add_sample(const MlpHost* h, const double F[3][3], std::vector<float>& xs, std::vector<float>& ts) {
   double S6[6], CC[21];
   exact_law(h->model, h->E, h->nu, F, S6, CC);
   const double inv_s_in = h->nrm.inv_s_in;
-  for (int k = 0; k < 9; ++k) h->xs.push_back((float)((F[k / 3][k % 3] - (k % 4 == 0 ? 1.0 : 0.0)) * inv_s_in));
-  for (int k = 0; k < 21; ++k) h->ts.push_back((float)((CC[k] - h->nrm.cc0[k]) / h->nrm.s_out));
+  for (int k = 0; k < 9; ++k) xs.push_back((float)((F[k / 3][k % 3] - (k % 4 == 0 ? 1.0 : 0.0)) * inv_s_in));
+  for (int k = 0; k < 21; ++k) ts.push_back((float)((CC[k] - h->nrm.cc0[k]) / h->nrm.s_out));
+}
+
+// Grows the device training set to hold `need` samples (contents kept).
+static int reserve(Ctx* c, MlpHost* h, int64_t need) {
+  if (need <= h->cap) return FEDD_OK;
+  const int64_t cap = std::max(need, std::max<int64_t>(2 * h->cap, 65536));
+  DevBuf nx, nt;
+  int rc;
+  if ((rc = alloc(c, nx, cap * 9 * 4))) return rc;
+  if ((rc = alloc(c, nt, cap * 21 * 4))) return rc;
+  if (h->n) {
+    FEDD_HIP(c, hipMemcpyAsync(nx.p, h->d_x.p, h->n * 9 * 4, hipMemcpyDeviceToDevice, c->stream));
+    FEDD_HIP(c, hipMemcpyAsync(nt.p, h->d_t.p, h->n * 21 * 4, hipMemcpyDeviceToDevice, c->stream));
+    FEDD_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  std::swap(nx.p, h->d_x.p);
+  std::swap(nx.bytes, h->d_x.bytes);
+  std::swap(nt.p, h->d_t.p);
+  std::swap(nt.bytes, h->d_t.bytes);
+  h->cap = cap;
+  return FEDD_OK;
+}
+
+// Appends the deformation gradients of quadrature-point records (and their
+// exact-law tangents as targets) to the training set.
+__global__ __launch_bounds__(256) void k_mlp_collect(const double* __restrict__ qp, int64_t nrec, int model,
+                                                     double Emod, double nu, MlpNorm nrm, float* __restrict__ x,
+                                                     float* __restrict__ t) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nrec) return;
+  double F[3][3];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) F[k / 3][k % 3] = qp[52 * i + k];
+  double S6[6], CC[21];
+  exact_law(model, Emod, nu, F, S6, CC);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) x[9 * i + k] = (float)((F[k / 3][k % 3] - (k % 4 == 0 ? 1.0 : 0.0)) * nrm.inv_s_in);
+#pragma unroll
+  for (int k = 0; k < 21; ++k) t[21 * i + k] = (float)((CC[k] - (double)nrm.cc0[k]) / (double)nrm.s_out);
+}
+
+int mlp_collect_qp(Ctx* c) {
+  MlpHost* h = c->mlp;
+  if (!h) return fail(c, FEDD_ERR_STATE, "no surrogate network");
+  const int64_t nrec = c->mesh[0].nelem * c->qp_nq;
+  if (c->qp_nq == 0 || nrec == 0) return fail(c, FEDD_ERR_STATE, "no quadrature-point records");
+  int rc = reserve(c, h, h->n + nrec);
+  if (rc) return rc;
+  k_mlp_collect<<<(unsigned)((nrec + 255) / 256), 256, 0, c->stream>>>(
+      c->d_qp.as<double>(), nrec, h->model, h->E, h->nu, h->nrm, h->d_x.as<float>() + 9 * h->n,
+      h->d_t.as<float>() + 21 * h->n);
+  FEDD_HIP(c, hipGetLastError());
+  h->n += nrec;
+  return FEDD_OK;
 }
 
 }  // namespace fedd
@@ -552,7 +606,11 @@ int fedd_mlp_init(fedd_ctx* c, int neo_hooke_model, double E, double nu, uint64_
 int fedd_mlp_add_samples_random(fedd_ctx* c, int64_t n, double sigma, uint64_t seed) {
   if (!c || !c->mlp || n < 0 || !(sigma >= 0)) return FEDD_ERR_ARG;
   MlpHost* h = c->mlp;
+  (void)hipSetDevice(c->device);
   uint64_t s = seed * 0x9E3779B97F4A7C15ull + 7;
+  std::vector<float> xs, ts;
+  xs.reserve(n * 9);
+  ts.reserve(n * 21);
   int64_t added = 0;
   while (added < n) {
     double F[3][3];
@@ -561,46 +619,29 @@ int fedd_mlp_add_samples_random(fedd_ctx* c, int64_t n, double sigma, uint64_t s
     const double J = F[0][0] * (F[1][1] * F[2][2] - F[1][2] * F[2][1]) - F[0][1] * (F[1][0] * F[2][2] - F[1][2] * F[2][0]) +
                      F[0][2] * (F[1][0] * F[2][1] - F[1][1] * F[2][0]);
     if (!(J > 0.2)) continue;  // the laws are defined for J > 0; keep physically admissible samples
-    add_sample(h, F);
+    add_sample(h, F, xs, ts);
     ++added;
   }
+  int rc = reserve(c, h, h->n + n);
+  if (rc) return rc;
+  FEDD_HIP(c, hipMemcpyAsync(h->d_x.as<float>() + 9 * h->n, xs.data(), xs.size() * 4, hipMemcpyHostToDevice,
+                             c->stream));
+  FEDD_HIP(c, hipMemcpyAsync(h->d_t.as<float>() + 21 * h->n, ts.data(), ts.size() * 4, hipMemcpyHostToDevice,
+                             c->stream));
+  FEDD_HIP(c, hipStreamSynchronize(c->stream));
+  h->n += n;
   return FEDD_OK;
 }
 
 int fedd_mlp_add_samples_qp(fedd_ctx* c) {
   if (!c || !c->mlp) return FEDD_ERR_ARG;
-  const Mesh& m = c->mesh[0];
-  if (c->d_qp.bytes == 0 || m.nelem == 0) return fail(c, FEDD_ERR_STATE, "no quadrature-point records");
   (void)hipSetDevice(c->device);
-  const int64_t nrec = m.nelem * 5;
-  std::vector<double> rec(nrec * 52);
-  FEDD_HIP(c, hipMemcpyAsync(rec.data(), c->d_qp.p, rec.size() * 8, hipMemcpyDeviceToHost, c->stream));
-  FEDD_HIP(c, hipStreamSynchronize(c->stream));
-  const int nq = c->mlp->model == 0 ? 4 : 5;
-  for (int64_t e = 0; e < m.nelem; ++e)
-    for (int q = 0; q < nq; ++q) {
-      const double* r = &rec[(e * nq + q) * 52];
-      double F[3][3];
-      for (int k = 0; k < 9; ++k) F[k / 3][k % 3] = r[k];
-      add_sample(c->mlp, F);
-    }
-  return FEDD_OK;
+  return mlp_collect_qp(c);
 }
 
 int fedd_mlp_num_samples(fedd_ctx* c, int64_t* n) {
   if (!c || !c->mlp || !n) return FEDD_ERR_ARG;
-  *n = (int64_t)c->mlp->xs.size() / 9;
-  return FEDD_OK;
-}
-
-static int upload_dataset(Ctx* c) {
-  MlpHost* h = c->mlp;
-  const int64_t n = (int64_t)h->xs.size() / 9;
-  if (h->dev_n == n) return FEDD_OK;
-  int rc;
-  if ((rc = upload(c, h->d_x, h->xs.data(), h->xs.size() * 4))) return rc;
-  if ((rc = upload(c, h->d_t, h->ts.data(), h->ts.size() * 4))) return rc;
-  h->dev_n = n;
+  *n = c->mlp->n;
   return FEDD_OK;
 }
 
@@ -623,10 +664,9 @@ int fedd_mlp_train(fedd_ctx* c, int steps, int batch, float lr, uint64_t seed, f
   if (!c || !c->mlp || steps < 0 || batch < 16) return FEDD_ERR_ARG;
   MlpHost* h = c->mlp;
   (void)hipSetDevice(c->device);
-  const int64_t n = (int64_t)h->xs.size() / 9;
+  const int64_t n = h->n;
   if (n < batch) return fail(c, FEDD_ERR_STATE, "fewer training samples than one batch");
-  int rc = upload_dataset(c);
-  if (rc) return rc;
+  int rc;
   // one random permutation per epoch, consecutive batches from it
   std::vector<int64_t> perm(n);
   uint64_t s = seed * 0xD1B54A32D192ED03ull + 3;

@@ -420,7 +420,8 @@ int fedd_newton_elasticity(fedd_ctx* cc, const fedd_newton_params* P, fedd_newto
   if (c->graph_kind != FEDD_GRAPH_VECTOR) return fail(c, FEDD_ERR_STATE, "needs a vector (P2) graph");
   if (P->load_steps < 1 || P->max_newton < 1 || !(P->abs_tol > 0)) return fail(c, FEDD_ERR_ARG, "bad Newton parameters");
   const int model = P->neo_hooke_model == 1 ? 1 : 2;
-  if (P->tangent == 1 && !c->mlp_ready) return fail(c, FEDD_ERR_STATE, "surrogate tangent requested but no network");
+  if ((P->tangent == 1 || P->collect_samples) && !c->mlp_ready)
+    return fail(c, FEDD_ERR_STATE, "surrogate tangent or sample collection requested but no network");
   if (P->tangent == 1 && !mlp_matches(c, model, P->E, P->nu))
     return fail(c, FEDD_ERR_STATE, "the surrogate was built for another law or other material constants");
   (void)hipSetDevice(c->device);
@@ -448,6 +449,7 @@ int fedd_newton_elasticity(fedd_ctx* cc, const fedd_newton_params* P, fedd_newto
       FEDD_HIP(c, hipEventRecord(ea, c->stream));
       if ((rc = launch_elasticity(c, model, P->E, P->nu, P->precision, 1, surrogate ? 2 : 1))) return rc;
       FEDD_HIP(c, hipEventRecord(eb, c->stream));
+      if (P->collect_samples && (rc = mlp_collect_qp(c))) return rc;
       if ((rc = newton_residual(c, lam, &norm))) return rc;
       float ms = 0.f;
       FEDD_HIP(c, hipEventElapsedTime(&ms, ea, eb));

@@ -189,6 +189,8 @@ typedef struct {
   double stall_ratio;   /* surrogate: switch to the exact tangent for the rest of the load step
                            when ||R_k+1|| > stall_ratio ||R_k|| (the step is kept), and reject
                            the step first when ||R_k+1|| > ||R_k|| or is not finite */
+  int collect_samples;  /* nonzero: append the quadrature-point states of every Newton iterate
+                           to the surrogate's training set (needs fedd_mlp_init) */
 } fedd_newton_params;
 
 typedef struct {
