@@ -29,6 +29,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+COPY_BYTES = 1 << 30
 NU, RHO = 0.01, 1.0
 
 
@@ -53,6 +54,9 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--newton-M", type=int, default=24)
     ap.add_argument("--no-newton", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run the STREAM copy kernel (1 GiB read + 1 GiB write, 20 times): the "
+                         "attainable bandwidth and the calibration of rocprofv3 byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (the elements would be "
@@ -62,6 +66,13 @@ def main():
 
     from feddlib_amd import native as N
     from feddlib_amd import workloads as W
+
+    if args.copy_probe:
+        with N.Context(0) as ctx:
+            ms = ctx.stream_copy(COPY_BYTES, 20)
+        print(json.dumps({"probe": "stream_copy", "bytes_read": COPY_BYTES, "bytes_written": COPY_BYTES,
+                          "launches": 21, "ms_per_copy": ms, "GBs": 2 * COPY_BYTES / (ms * 1e-3) / 1e9}))
+        return
 
     M = args.M
     t0 = time.time()

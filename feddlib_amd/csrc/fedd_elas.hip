@@ -344,7 +344,7 @@ template <typename T>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FEDD_ELAS_WAVES))) void k_elas_rows(ElasArgs a) {
   extern __shared__ double lds[];
   __shared__ int2 tab[256];
-  const int64_t ch = blockIdx.x;
+  const int64_t ch = xcd_chunk(blockIdx.x, gridDim.x);
   const int n0 = a.chunk_node[ch], n1 = a.chunk_node[ch + 1];
   const int q0 = a.chunk_pair[ch], q1 = a.chunk_pair[ch + 1];
   const int64_t na0 = a.nptr[n0], na1 = a.nptr[n1];

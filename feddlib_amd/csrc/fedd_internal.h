@@ -158,4 +158,24 @@ int mlp_collect_qp(Ctx* c);
 
 }  // namespace fedd
 
+namespace fedd {
+// XCD-aware chunk order.  Workgroups are dispatched round-robin over the 8 XCDs
+// (block b runs on XCD b % 8) and each XCD has its own L2.  Neighbouring node
+// chunks share elements, so each XCD is given one contiguous range of chunks:
+// XCD x takes chunks [x*q + min(x,r), ...) in order, q = G / 8, r = G % 8.
+// The mapping is a bijection of [0, G) for any G.
+#ifndef FEDD_XCD_REMAP
+#define FEDD_XCD_REMAP 1
+#endif
+__device__ __forceinline__ int64_t xcd_chunk(int64_t b, int64_t G) {
+#if FEDD_XCD_REMAP
+  const int64_t q = G >> 3, r = G & 7, x = b & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
+#else
+  (void)G;
+  return b;
+#endif
+}
+}  // namespace fedd
+
 struct fedd_ctx : public fedd::Ctx {};

@@ -29,6 +29,9 @@
 #ifndef FEDD_ROW_WAVES
 #define FEDD_ROW_WAVES 2
 #endif
+#ifndef FEDD_ROW_BLOCKS_PER_CU
+#define FEDD_ROW_BLOCKS_PER_CU 0  // 0: one workgroup per chunk
+#endif
 
 namespace fedd {
 
@@ -311,10 +314,11 @@ struct VelRow {
             if (NT::TT[ax][k][c] != 0.0) Nacc[c] += ut[ax] * (T)NT::TT[ax][k][c];
       }
 #pragma unroll
-      for (int c = 0; c < 10; ++c) S[c] += (T)a.rho * Nacc[c];
+      for (int c = 0; c < 10; ++c) S[c] = (T)((double)(S[c] + (T)a.rho * Nacc[c]) * adet);
     }
     if (a.newton) {
       // grad u is P1: du_d1/dxhat_ax = sum_m lambda_m gh[m][ax], gh[m][ax] = sum_k u_k,d1 DV[ax][k][m]
+      const T rdet = (T)(a.rho * adet);
 #pragma unroll
       for (int d1 = 0; d1 < 3; ++d1) {
         T gh[4][3];
@@ -331,10 +335,10 @@ struct VelRow {
               if (ft::DV[ax][k][mm] != 0.0) gh[mm][ax] += U[k][d1] * (T)ft::DV[ax][k][mm];
 #pragma unroll
         for (int d2 = 0; d2 < 3; ++d2) {
-          T gw[4];  // rho * du_d1/dx_d2 at vertex m
+          T gw[4];  // rho |det| du_d1/dx_d2 at vertex m
 #pragma unroll
           for (int mm = 0; mm < 4; ++mm)
-            gw[mm] = (T)a.rho * (Bi[0][d2] * gh[mm][0] + Bi[1][d2] * gh[mm][1] + Bi[2][d2] * gh[mm][2]);
+            gw[mm] = rdet * (Bi[0][d2] * gh[mm][0] + Bi[1][d2] * gh[mm][1] + Bi[2][d2] * gh[mm][2]);
 #pragma unroll
           for (int c = 0; c < 10; ++c) {
             T w = 0;
@@ -342,14 +346,14 @@ struct VelRow {
             for (int mm = 0; mm < 4; ++mm)
               if (NT::L[mm][c] != 0.0) w += gw[mm] * (T)NT::L[mm][c];
             if (d1 == d2) w += S[c];
-            atomicAdd(rowA + d1 * 3 * deg + 3 * m[2 + c] + d2, (double)w * adet);
+            atomicAdd(rowA + d1 * 3 * deg + 3 * m[2 + c] + d2, (double)w);
           }
         }
       }
     } else {
 #pragma unroll
       for (int c = 0; c < 10; ++c) {
-        double v = (double)S[c] * adet;
+        const double v = (double)S[c];
         double* r = rowA + 3 * m[2 + c];
         atomicAdd(r + 0 * 3 * deg + 0, v);
         atomicAdd(r + 1 * 3 * deg + 1, v);
@@ -376,62 +380,90 @@ struct VelRow {
 // flight, each in a different phase: zeroing, gathering, writing).  Inside a
 // chunk, vertex-class pairs occupy whole waves before the edge-class pairs start
 // (nv is padded to a multiple of 64), so each wave runs one specialised row kernel.
+// Block barrier that orders LDS only.  __syncthreads() also waits for every
+// outstanding global store of the wave; here the nontemporal stores of one chunk
+// keep draining to HBM while the block gathers the next chunk.
+__device__ __forceinline__ void lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Chunks [c0, c1) of the block: contiguous, and contiguous per XCD (xcd_chunk).
+__device__ __forceinline__ void block_chunks(int64_t nchunks, int64_t& c0, int64_t& c1) {
+  const int64_t G = gridDim.x;
+  const int64_t L = xcd_chunk(blockIdx.x, G);
+  const int64_t q = nchunks / G, r = nchunks % G;
+  c0 = L * q + (L < r ? L : r);
+  c1 = c0 + q + (L < r ? 1 : 0);
+}
+
+// Each workgroup walks its chunks in order.  Per chunk: zero the LDS rows, add
+// every pair's canonical row with LDS atomics, then stream the finished rows out
+// with nontemporal stores in CRS order; the stores drain while the next chunk is
+// gathered.  Inside a chunk, vertex-class pairs occupy whole waves before the
+// edge-class pairs start (nv is padded to a multiple of 64), so each wave runs one
+// specialised row kernel.
 template <typename T>
 __global__ __launch_bounds__(FEDD_ROW_BLOCK) __attribute__((amdgpu_waves_per_eu(FEDD_ROW_WAVES))) void k_ns_vel_rows(NsArgs a) {
   extern __shared__ double lds[];
   __shared__ int4 tab[256];  // per chunk node: A-row offset, deg, B^T-row offset, pdeg
-  const int64_t ch = blockIdx.x;
-  const int n0 = a.chunk_node[ch], n1 = a.chunk_node[ch + 1];
-  const int q0 = a.chunk_pair[ch], q1 = a.chunk_pair[ch + 1], qs = a.chunk_split[ch];
-  const int64_t na0 = a.nptr[n0], na1 = a.nptr[n1];
-  const int aLen = (int)(9 * (na1 - na0));
-  int64_t nb0 = 0;
-  int btLen = 0;
-  if (a.with_bt) {
-    nb0 = a.pptr[n0];
-    btLen = (int)(3 * (a.pptr[n1] - nb0));
-  }
-  for (int t = threadIdx.x; t < n1 - n0; t += blockDim.x) {
-    const int64_t ai = a.nptr[n0 + t], ai1 = a.nptr[n0 + t + 1];
-    int4 v;
-    v.x = (int)(9 * (ai - na0));
-    v.y = (int)(ai1 - ai);
+  int64_t c0, c1;
+  block_chunks(a.nchunks, c0, c1);
+  for (int64_t ch = c0; ch < c1; ++ch) {
+    const int n0 = a.chunk_node[ch], n1 = a.chunk_node[ch + 1];
+    const int q0 = a.chunk_pair[ch], q1 = a.chunk_pair[ch + 1], qs = a.chunk_split[ch];
+    const int64_t na0 = a.nptr[n0], na1 = a.nptr[n1];
+    const int aLen = (int)(9 * (na1 - na0));
+    int64_t nb0 = 0;
+    int btLen = 0;
     if (a.with_bt) {
-      const int64_t bi = a.pptr[n0 + t], bi1 = a.pptr[n0 + t + 1];
-      v.z = aLen + (int)(3 * (bi - nb0));
-      v.w = (int)(bi1 - bi);
-    } else {
-      v.z = 0;
-      v.w = 0;
+      nb0 = a.pptr[n0];
+      btLen = (int)(3 * (a.pptr[n1] - nb0));
     }
-    tab[t] = v;
-  }
-  for (int k = threadIdx.x; k < aLen + btLen; k += blockDim.x) lds[k] = 0.0;
-  __syncthreads();
-  const int nv = qs - q0, ne = q1 - qs;
-  const int nvpad = (nv + 63) & ~63;
-  for (int t = threadIdx.x; t < nvpad + ne; t += blockDim.x) {
-    const bool vert = t < nvpad;  // uniform across the wave
-    if (vert && t >= nv) continue;
-    const int q = vert ? q0 + t : qs + (t - nvpad);
-    const int32_t e = a.pair_elem[q];
-    const uint4 mv = a.pair_meta[q];
-    const uint8_t* m = reinterpret_cast<const uint8_t*>(&mv);
-    const int4 nd = tab[m[1]];
-    double* rowA = lds + nd.x;
-    double* rowBT = a.with_bt ? lds + nd.z : nullptr;
-    if (vert)
-      VelRow<T, 0>::run(a, e, m, rowA, nd.y, rowBT, nd.w);
-    else
-      VelRow<T, 1>::run(a, e, m, rowA, nd.y, rowBT, nd.w);
-  }
-  __syncthreads();
-  double* outA = a.valsA + 9 * na0;
-  for (int k = threadIdx.x; k < aLen; k += blockDim.x) __builtin_nontemporal_store(lds[k], outA + k);
-  if (a.with_bt) {
-    double* outB = a.valsBT + 3 * nb0;
-    for (int k = threadIdx.x; k < btLen; k += blockDim.x)
-      __builtin_nontemporal_store(lds[aLen + k], outB + k);
+    for (int t = threadIdx.x; t < n1 - n0; t += blockDim.x) {
+      const int64_t ai = a.nptr[n0 + t], ai1 = a.nptr[n0 + t + 1];
+      int4 v;
+      v.x = (int)(9 * (ai - na0));
+      v.y = (int)(ai1 - ai);
+      if (a.with_bt) {
+        const int64_t bi = a.pptr[n0 + t], bi1 = a.pptr[n0 + t + 1];
+        v.z = aLen + (int)(3 * (bi - nb0));
+        v.w = (int)(bi1 - bi);
+      } else {
+        v.z = 0;
+        v.w = 0;
+      }
+      tab[t] = v;
+    }
+    for (int k = threadIdx.x; k < aLen + btLen; k += blockDim.x) lds[k] = 0.0;
+    lds_sync();
+    const int nv = qs - q0, ne = q1 - qs;
+    const int nvpad = (nv + 63) & ~63;
+    for (int t = threadIdx.x; t < nvpad + ne; t += blockDim.x) {
+      const bool vert = t < nvpad;  // uniform across the wave
+      if (vert && t >= nv) continue;
+      const int q = vert ? q0 + t : qs + (t - nvpad);
+      const int32_t e = a.pair_elem[q];
+      const uint4 mv = a.pair_meta[q];
+      const uint8_t* m = reinterpret_cast<const uint8_t*>(&mv);
+      const int4 nd = tab[m[1]];
+      double* rowA = lds + nd.x;
+      double* rowBT = a.with_bt ? lds + nd.z : nullptr;
+      if (vert)
+        VelRow<T, 0>::run(a, e, m, rowA, nd.y, rowBT, nd.w);
+      else
+        VelRow<T, 1>::run(a, e, m, rowA, nd.y, rowBT, nd.w);
+    }
+    lds_sync();
+    double* outA = a.valsA + 9 * na0;
+    for (int k = threadIdx.x; k < aLen; k += blockDim.x) __builtin_nontemporal_store(lds[k], outA + k);
+    if (a.with_bt) {
+      double* outB = a.valsBT + 3 * nb0;
+      for (int k = threadIdx.x; k < btLen; k += blockDim.x)
+        __builtin_nontemporal_store(lds[aLen + k], outB + k);
+    }
+    lds_sync();  // the rows are read out before the next chunk zeroes them
   }
 }
 
@@ -483,7 +515,7 @@ int launch_geometry(Ctx* c) {
 template <typename T>
 __global__ __launch_bounds__(256) void k_ns_pres_rows(NsArgs a) {
   extern __shared__ double lds[];
-  const int ch = blockIdx.x;
+  const int ch = (int)xcd_chunk(blockIdx.x, gridDim.x);
   const int n0 = a.chunk_node[ch], n1 = a.chunk_node[ch + 1];
   const int q0 = a.chunk_pair[ch], q1 = a.chunk_pair[ch + 1];
   const int64_t nb0 = a.bptr[n0];
@@ -585,7 +617,7 @@ __device__ __forceinline__ void res_row(const NsArgs& a, int32_t e, int li, doub
 template <typename T>
 __global__ __launch_bounds__(256) void k_ns_res_vel(NsArgs a) {
   extern __shared__ double lds[];
-  const int ch = blockIdx.x;
+  const int ch = (int)xcd_chunk(blockIdx.x, gridDim.x);
   const int n0 = a.chunk_node[ch], n1 = a.chunk_node[ch + 1];
   const int q0 = a.chunk_pair[ch], q1 = a.chunk_pair[ch + 1], qs = a.chunk_split[ch];
   const int len = 3 * (n1 - n0);
@@ -612,7 +644,7 @@ __global__ __launch_bounds__(256) void k_ns_res_vel(NsArgs a) {
 template <typename T>
 __global__ __launch_bounds__(256) void k_ns_res_pres(NsArgs a) {
   extern __shared__ double lds[];
-  const int ch = blockIdx.x;
+  const int ch = (int)xcd_chunk(blockIdx.x, gridDim.x);
   const int n0 = a.chunk_node[ch], n1 = a.chunk_node[ch + 1];
   const int q0 = a.chunk_pair[ch], q1 = a.chunk_pair[ch + 1];
   const int len = n1 - n0;
@@ -712,7 +744,7 @@ __device__ __forceinline__ void lap_row(const LapArgs& a, int32_t e, const uint8
 template <typename T>
 __global__ __launch_bounds__(256) void k_laplace_rows(LapArgs a) {
   extern __shared__ double lds[];
-  const int ch = blockIdx.x;
+  const int ch = (int)xcd_chunk(blockIdx.x, gridDim.x);
   const int n0 = a.chunk_node[ch], n1 = a.chunk_node[ch + 1];
   const int q0 = a.chunk_pair[ch], q1 = a.chunk_pair[ch + 1], qs = a.chunk_split[ch];
   const int64_t na0 = a.nptr[n0];
@@ -735,7 +767,7 @@ __global__ __launch_bounds__(256) void k_laplace_rows(LapArgs a) {
 
 __global__ __launch_bounds__(256) void k_rhs_rows(LapArgs a) {
   extern __shared__ double lds[];
-  const int ch = blockIdx.x;
+  const int ch = (int)xcd_chunk(blockIdx.x, gridDim.x);
   const int n0 = a.chunk_node[ch], n1 = a.chunk_node[ch + 1];
   const int q0 = a.chunk_pair[ch], q1 = a.chunk_pair[ch + 1];
   const int len = n1 - n0;
@@ -842,7 +874,11 @@ int launch_ns_assembly(Ctx* c, const fedd_ns_params* p, int mode, int reassemble
     NsArgs a = ns_args(c, &q, c->vplan);
     a.with_bt = reassemble ? 0 : 1;
     const size_t lds = (size_t)c->vplan.lds_doubles * 8;
-    const unsigned grid = (unsigned)c->vplan.nchunks;
+    // FEDD_ROW_BLOCKS_PER_CU > 0: that many workgroups per CU walk the chunks
+    // (measured slower on MI355X than one workgroup per chunk)
+    const unsigned grid = (unsigned)(FEDD_ROW_BLOCKS_PER_CU > 0 ? std::min<int64_t>(c->vplan.nchunks,
+                                                                                 (int64_t)c->num_cu * FEDD_ROW_BLOCKS_PER_CU)
+                                                                : c->vplan.nchunks);
     ktimer_begin(c);
     k_pad_u<<<nblk(n2, 256), 256, 0, c->stream>>>(n2, a.u, c->d_upad.as<double>());
     FEDD_HIP(c, hipGetLastError());
