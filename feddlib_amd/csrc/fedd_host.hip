@@ -509,8 +509,8 @@ int fedd_graph_build(fedd_ctx* c, int kind) {
   }
   // node-level pointers and adjacency on the device, expanded by a kernel
   if ((rc = upload(c, c->d_nptr, g.a.ptr.data(), g.a.ptr.size() * 8))) return rc;
-  DevBuf d_aidx, d_btidx, d_bidx;
-  if ((rc = upload(c, d_aidx, g.a.idx.data(), g.a.idx.size() * 4))) return rc;
+  DevBuf d_btidx, d_bidx;
+  if ((rc = upload(c, c->d_nidx, g.a.idx.data(), g.a.idx.size() * 4))) return rc;
   if (kind == FEDD_GRAPH_NS) {
     if ((rc = upload(c, c->d_pptr, g.bt.ptr.data(), g.bt.ptr.size() * 8))) return rc;
     if ((rc = upload(c, c->d_bptr, g.b.ptr.data(), g.b.ptr.size() * 8))) return rc;
@@ -526,7 +526,7 @@ int fedd_graph_build(fedd_ctx* c, int kind) {
     FEDD_HIP(c, hipMemsetAsync(b.d_vals.p, 0, std::max<int64_t>(b.nnz, 1) * 8, c->stream));
   }
   c->graph_kind = kind;
-  rc = expand_graph(c, kind, d_aidx.as<int32_t>(), d_btidx.as<int32_t>(), d_bidx.as<int32_t>());
+  rc = expand_graph(c, kind, c->d_nidx.as<int32_t>(), d_btidx.as<int32_t>(), d_bidx.as<int32_t>());
   if (rc) {
     c->graph_kind = GK_NONE;
     return rc;

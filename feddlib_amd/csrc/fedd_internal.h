@@ -86,6 +86,7 @@ struct Ctx {
   Block blk[4];
   // node-level structures on the device
   DevBuf d_nptr, d_pptr, d_bptr;  // int64 node-level row pointers (A, B^T, B)
+  DevBuf d_nidx;                  // int32 node-level column indices of A (node blocks)
   GatherPlan vplan, pplan;        // velocity (primary) and pressure row gathers
   DevBuf d_geo;                   // per element: grad lambda_0..3 (12) + |det| + pad, 16 doubles
   DevBuf d_upad;                  // velocity iterate padded to 4 doubles per node

@@ -17,6 +17,10 @@ namespace fedd {
 
 static inline unsigned nb(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
+#ifndef FEDD_SPMV_LANES
+#define FEDD_SPMV_LANES 32  // lanes per node in the node-block SpMV (16: +8%, 8: +42% time)
+#endif
+
 // ---------------------------------------------------------------- Dirichlet
 // Constrained rows and columns become identity rows/columns (symmetric
 // elimination keeps the tangent SPD; the increment of a constrained dof is 0).
@@ -79,52 +83,91 @@ __global__ __launch_bounds__(1024) void k_reduce(const double* __restrict__ part
   }
 }
 
-// y = A x with 16 lanes per row; partial sums of x.y per block.
-__global__ __launch_bounds__(256) void k_spmv_dot(int64_t nrows, const int64_t* __restrict__ rowptr,
-                                                  const int32_t* __restrict__ colind,
-                                                  const double* __restrict__ vals, const double* __restrict__ x,
-                                                  double* __restrict__ y, double* __restrict__ part) {
-  __shared__ double red[16];
-  const int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 4;
-  const int lane = threadIdx.x & 15;
-  double s = 0.0;
-  if (row < nrows)
-    for (int64_t k = rowptr[row] + lane; k < rowptr[row + 1]; k += 16) s += vals[k] * x[colind[k]];
+// Block sum of v, added with one FP64 atomic per workgroup (256 threads) to one of
+// 16 sub-slots out[0..15] (spreads same-address contention; readers sum all 16).
+__device__ __forceinline__ void block_atomic_sum(double v, double* out) {
+  __shared__ double red[4];
+  __syncthreads();  // red[] may still be read by thread 0 of a previous call
 #pragma unroll
-  for (int o = 8; o > 0; o >>= 1) s += __shfl_down(s, o, 16);
-  if (lane == 0) {
-    double d = 0.0;
-    if (row < nrows) {
-      y[row] = s;
-      d = s * x[row];
-    }
-    red[threadIdx.x >> 4] = d;
-  }
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) t += red[k];
-    part[blockIdx.x] = t;
-  }
+  if (threadIdx.x == 0) atomicAdd(out + (blockIdx.x & 15), red[0] + red[1] + red[2] + red[3]);
 }
 
-// x += alpha p; r -= alpha Ap; z = Binv r (3x3 node blocks); partials of r.z and r.r.
+__device__ __forceinline__ double sum16(const double* p) {
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) s += p[k];
+  return s;
+}
+
+// PCG scalars live in rotating device slots (16 sub-slots each) so that no kernel
+// clears a slot that another kernel of the same iteration reads: iteration t
+// accumulates p.Ap in PAP(t), r.z in RZ(t) and r.r in RR(t), slot index t mod 3;
+// rz of the previous iteration is RZ(t-1) (slot 2 for t = 0, from the first update).
+#define PCG_PAP(t) (32 + 16 * ((t) % 3))
+#define PCG_RZ(t) (80 + 16 * ((t) % 3))
+#define PCG_RR(t) (128 + 16 * ((t) % 3))
+#define PCG_SCAL 176
+
+// y = A x on the 3x3 node-block structure of the vector graph: the three rows of
+// node i are contiguous (row 3i+d starts at 9 nptr[i] + 3 deg_i d) and entry q of a
+// row multiplies x[3 nidx[nptr[i] + q/3] + q%3].  G lanes per node walk each row
+// with unit stride (coalesced value reads); x.y is added to *dot.
+template <int G>
+__global__ __launch_bounds__(256) void k_bspmv_dot(int64_t nnode, const int64_t* __restrict__ nptr,
+                                                   const int32_t* __restrict__ nidx,
+                                                   const double* __restrict__ vals, const double* __restrict__ x,
+                                                   double* __restrict__ y, double* __restrict__ dot) {
+  const int lane = threadIdx.x % G;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x / G;
+  const int64_t nit = (nnode + stride - 1) / stride;  // uniform trip count (shuffles below)
+  double dd = 0.0;
+  for (int64_t it = 0; it < nit; ++it) {
+    const int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G + it * stride;
+    double s[3] = {0.0, 0.0, 0.0};
+    if (i < nnode) {
+      const int64_t k0 = nptr[i];
+      const int L = 3 * (int)(nptr[i + 1] - k0);
+      const double* v = vals + 9 * k0;
+      const int32_t* ni = nidx + k0;
+      for (int q = lane; q < L; q += G) {
+        const int k = (q * 0xAAAB) >> 17;  // q / 3 for q < 2^15
+        const double xv = x[3 * (int64_t)ni[k] + (q - 3 * k)];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) s[d] += v[d * L + q] * xv;
+      }
+    }
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) s[d] += __shfl_xor(s[d], o, G);
+    if (i < nnode && lane == 0) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        y[3 * i + d] = s[d];
+        dd += s[d] * x[3 * i + d];
+      }
+    }
+  }
+  block_atomic_sum(dd, dot);
+}
+
+// x += alpha p; r -= alpha Ap; z = Binv r (3x3 node blocks); r.z and r.r added to
+// the slots of iteration t.  first: z = Binv r only, into the slots of t = -1.
 __global__ __launch_bounds__(256) void k_pcg_update(int64_t nnode, double* __restrict__ x, double* __restrict__ r,
                                                     const double* __restrict__ p, const double* __restrict__ Ap,
                                                     double* __restrict__ z, const double* __restrict__ binv,
-                                                    const double* __restrict__ scal, double* __restrict__ part_rz,
-                                                    double* __restrict__ part_rr, int first) {
-  __shared__ double red[2][256];
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+                                                    double* __restrict__ scal, int t, int first) {
   double rz = 0.0, rr = 0.0;
-  if (i < nnode) {
+  const double alpha = first ? 0.0 : sum16(scal + PCG_RZ(t + 2)) / sum16(scal + PCG_PAP(t));
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnode; i += (int64_t)gridDim.x * blockDim.x) {
     double rl[3];
     if (first) {
 #pragma unroll
       for (int d = 0; d < 3; ++d) rl[d] = r[3 * i + d];
     } else {
-      const double alpha = scal[0] / scal[1];
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         x[3 * i + d] += alpha * p[3 * i + d];
@@ -141,28 +184,22 @@ __global__ __launch_bounds__(256) void k_pcg_update(int64_t nnode, double* __res
       rr += rl[d] * rl[d];
     }
   }
-  red[0][threadIdx.x] = rz;
-  red[1][threadIdx.x] = rr;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      red[0][threadIdx.x] += red[0][threadIdx.x + s];
-      red[1][threadIdx.x] += red[1][threadIdx.x + s];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    part_rz[blockIdx.x] = red[0][0];
-    part_rr[blockIdx.x] = red[1][0];
-  }
+  const int ts = first ? 2 : t;
+  block_atomic_sum(rz, scal + PCG_RZ(ts));
+  block_atomic_sum(rr, scal + PCG_RR(ts));
 }
 
-// p = z + beta p  (first: p = z)
+// p = z + beta p, beta = RZ(t) / RZ(t-1)  (first: p = z).  Block 0 clears the
+// slots of iteration t+1 (last read in iteration t-2).
 __global__ __launch_bounds__(256) void k_pcg_p(int64_t n, double* __restrict__ p, const double* __restrict__ z,
-                                               const double* __restrict__ scal, int first) {
+                                               double* __restrict__ scal, int t, int first) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (!first && blockIdx.x == 0 && threadIdx.x < 48) {
+    const int k = threadIdx.x & 15, w = threadIdx.x >> 4;
+    scal[(w == 0 ? PCG_PAP(t + 1) : w == 1 ? PCG_RZ(t + 1) : PCG_RR(t + 1)) + k] = 0.0;
+  }
   if (i >= n) return;
-  p[i] = first ? z[i] : z[i] + scal[5] * p[i];
+  p[i] = first ? z[i] : z[i] + (sum16(scal + PCG_RZ(t)) / sum16(scal + PCG_RZ(t + 2))) * p[i];
 }
 
 // Inverses of the 3x3 diagonal node blocks (block Jacobi).
@@ -249,10 +286,7 @@ int pcg_solve(Ctx* c, double rtol, double atol, int maxit, int* iters, double* r
   for (int k = 0; k < 4; ++k)
     if ((rc = ensure(c, c->d_kw[k], n * 8))) return rc;
   if ((rc = ensure(c, c->d_kw[4], nnode * 9 * 8))) return rc;
-  const int64_t nblk_rows = nb(n * 16, 256);
-  const int64_t nblk_nodes = nb(nnode, 256);
-  if ((rc = ensure(c, c->d_part, (size_t)std::max(nblk_rows, 2 * nblk_nodes) * 8))) return rc;
-  if ((rc = ensure(c, c->d_scal, 16 * 8))) return rc;
+  if ((rc = ensure(c, c->d_scal, PCG_SCAL * 8))) return rc;
   if ((rc = ensure_vec(c, FEDD_VEC_X, n))) return rc;
   double* r = c->d_kw[0].as<double>();
   double* z = c->d_kw[1].as<double>();
@@ -261,40 +295,51 @@ int pcg_solve(Ctx* c, double rtol, double atol, int maxit, int* iters, double* r
   double* binv = c->d_kw[4].as<double>();
   double* x = c->vec[FEDD_VEC_X].as<double>();
   double* scal = c->d_scal.as<double>();
-  double* part = c->d_part.as<double>();
-  double* part2 = part + nblk_nodes;
   const int64_t* rp = A.d_rowptr.as<int64_t>();
   const int32_t* ci = A.d_colind.as<int32_t>();
   const double* va = A.d_vals.as<double>();
-  k_block_diag<<<nb(nnode, 256), 256, 0, c->stream>>>(nnode, rp, ci, va, binv);
+  const int64_t* nptr = c->d_nptr.as<int64_t>();
+  const int32_t* nidx = c->d_nidx.as<int32_t>();
+  // grid-stride kernels: a bounded number of workgroups bounds the atomics per dot product
+  const unsigned gmax = (unsigned)c->num_cu * 8;
+  const unsigned g_nodes = std::min(nb(nnode, 256), gmax);
+  const unsigned g_spmv = std::min(nb(nnode * FEDD_SPMV_LANES, 256), gmax), g_dofs = nb(n, 256);
+  k_block_diag<<<g_nodes, 256, 0, c->stream>>>(nnode, rp, ci, va, binv);
   FEDD_HIP(c, hipMemcpyAsync(r, c->vec[FEDD_VEC_B].p, n * 8, hipMemcpyDeviceToDevice, c->stream));
-  k_fill<<<nb(n, 256), 256, 0, c->stream>>>(n, x, 0.0);
-  k_pcg_update<<<nblk_nodes, 256, 0, c->stream>>>(nnode, x, r, p, Ap, z, binv, scal, part, part2, 1);
-  k_reduce<<<1, 1024, 0, c->stream>>>(part, nblk_nodes, scal, 0, 0);   // rz
-  k_reduce<<<1, 1024, 0, c->stream>>>(part2, nblk_nodes, scal, 3, 0);  // rr
-  k_pcg_p<<<nb(n, 256), 256, 0, c->stream>>>(n, p, z, scal, 1);
+  k_fill<<<g_dofs, 256, 0, c->stream>>>(n, x, 0.0);
+  k_fill<<<1, 256, 0, c->stream>>>(PCG_SCAL, scal, 0.0);
+  k_pcg_update<<<g_nodes, 256, 0, c->stream>>>(nnode, x, r, p, Ap, z, binv, scal, -1, 1);
+  k_pcg_p<<<g_dofs, 256, 0, c->stream>>>(n, p, z, scal, -1, 1);
   FEDD_HIP(c, hipGetLastError());
-  double h[4];
-  FEDD_HIP(c, hipMemcpyAsync(h, scal, 4 * 8, hipMemcpyDeviceToHost, c->stream));
-  FEDD_HIP(c, hipStreamSynchronize(c->stream));
-  const double bnorm = std::sqrt(h[3]);
+  double h[16];
+  auto read_rr = [&](int t) -> int {
+    FEDD_HIP(c, hipMemcpyAsync(h, scal + PCG_RR(t), 16 * 8, hipMemcpyDeviceToHost, c->stream));
+    FEDD_HIP(c, hipStreamSynchronize(c->stream));
+    return FEDD_OK;
+  };
+  auto sum = [&]() {
+    double v = 0.0;
+    for (int k = 0; k < 16; ++k) v += h[k];
+    return v;
+  };
+  if ((rc = read_rr(2))) return rc;
+  const double bnorm = std::sqrt(sum());
   double rnorm = bnorm;
   int it = 0;
   const double target = std::max(rtol * bnorm, atol);
   if (bnorm > 0.0 && rnorm > target) {
     while (it < maxit) {
-      k_spmv_dot<<<nblk_rows, 256, 0, c->stream>>>(n, rp, ci, va, p, Ap, part);
-      k_reduce<<<1, 1024, 0, c->stream>>>(part, nblk_rows, scal, 1, 0);  // pAp
-      k_pcg_update<<<nblk_nodes, 256, 0, c->stream>>>(nnode, x, r, p, Ap, z, binv, scal, part, part2, 0);
-      k_reduce<<<1, 1024, 0, c->stream>>>(part, nblk_nodes, scal, 0, 1);  // rz, beta
-      k_reduce<<<1, 1024, 0, c->stream>>>(part2, nblk_nodes, scal, 3, 0);  // rr
-      k_pcg_p<<<nb(n, 256), 256, 0, c->stream>>>(n, p, z, scal, 0);
+      const int t = it;
+      ktimer_begin(c);
+      k_bspmv_dot<FEDD_SPMV_LANES><<<g_spmv, 256, 0, c->stream>>>(nnode, nptr, nidx, va, p, Ap, scal + PCG_PAP(t));
+      ktimer_end(c, "bspmv");
+      k_pcg_update<<<g_nodes, 256, 0, c->stream>>>(nnode, x, r, p, Ap, z, binv, scal, t, 0);
+      k_pcg_p<<<g_dofs, 256, 0, c->stream>>>(n, p, z, scal, t, 0);
       ++it;
       if ((it & 7) == 0 || it == maxit) {
         FEDD_HIP(c, hipGetLastError());
-        FEDD_HIP(c, hipMemcpyAsync(h, scal, 4 * 8, hipMemcpyDeviceToHost, c->stream));
-        FEDD_HIP(c, hipStreamSynchronize(c->stream));
-        rnorm = std::sqrt(h[3]);
+        if ((rc = read_rr(t))) return rc;
+        rnorm = std::sqrt(sum());
         if (!(rnorm == rnorm)) break;  // NaN: let the caller reject the step
         if (rnorm <= target) break;
       }
@@ -311,7 +356,7 @@ int newton_residual(Ctx* c, double lam, double* norm) {
   const int64_t nblk = nb(n, 256);
   int rc;
   if ((rc = ensure(c, c->d_part, nblk * 8))) return rc;
-  if ((rc = ensure(c, c->d_scal, 16 * 8))) return rc;
+  if ((rc = ensure(c, c->d_scal, PCG_SCAL * 8))) return rc;
   if ((rc = ensure_vec(c, FEDD_VEC_B, n))) return rc;
   if ((rc = ensure(c, c->d_bcflag, n))) return rc;
   if (c->nbc == 0) FEDD_HIP(c, hipMemsetAsync(c->d_bcflag.p, 0, n, c->stream));
