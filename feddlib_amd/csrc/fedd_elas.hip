@@ -121,6 +121,7 @@ struct ElasArgs {
   int nq;
   int model;
   double Emod, nu;
+  const double* Evec;     // per-element Young's modulus (overrides Emod) or null
   int exact_tangent;      // 1: write CC from the law; 0: leave CC (surrogate wrote it)
   // plan
   const int64_t* nptr;
@@ -167,7 +168,7 @@ __global__ __launch_bounds__(256) void k_elas_qp(ElasArgs a) {
     }
   double* o = a.qp + 52 * t;
   double S6[6], CC[21];
-  exact_law(a.model, a.Emod, a.nu, F, S6, a.exact_tangent ? CC : nullptr);
+  exact_law(a.model, a.Evec ? a.Evec[e] : a.Emod, a.nu, F, S6, a.exact_tangent ? CC : nullptr);
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -379,7 +380,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FEDD_ELAS_W
     for (int k = threadIdx.x; k < rLen; k += blockDim.x) a.res[3 * (int64_t)n0 + k] = lds[aLen + k];
 }
 
-int launch_elasticity(Ctx* c, int model, double Emod, double nu, int precision, int with_res, int tangent) {
+int launch_elasticity(Ctx* c, int model, double Emod, double nu, int precision, int with_res, int tangent,
+                      const double* Evec) {
   const int exact_tangent = tangent == 2 ? 0 : 1;
   const Mesh& m = c->mesh[0];
   ElasArgs a{};
@@ -392,6 +394,7 @@ int launch_elasticity(Ctx* c, int model, double Emod, double nu, int precision, 
   a.model = model;
   a.Emod = Emod;
   a.nu = nu;
+  a.Evec = Evec;
   a.exact_tangent = exact_tangent;
   const size_t qbytes = (size_t)m.nelem * 5 * 52 * 8;
   if (c->d_qp.bytes < qbytes) {

@@ -750,7 +750,8 @@ int fedd_assembly_navier_stokes(fedd_ctx* c, const fedd_ns_params* p, int mode, 
   return FEDD_OK;
 }
 
-static int elasticity_common(fedd_ctx* c, int model, double E, double nu, int precision, int with_res) {
+static int elasticity_common(fedd_ctx* c, int model, double E, double nu, int precision, int with_res,
+                             const double* Evec = nullptr) {
   if (!c) return FEDD_ERR_ARG;
   if (c->graph_kind != FEDD_GRAPH_VECTOR) return fail(c, FEDD_ERR_STATE, "needs a vector (P2) graph");
   if (precision != FEDD_FP64 && precision != FEDD_FP32_LOCAL) return fail(c, FEDD_ERR_ARG, "bad precision");
@@ -765,7 +766,7 @@ static int elasticity_common(fedd_ctx* c, int model, double E, double nu, int pr
       c->vec_n[FEDD_VEC_RES_U] = n;
     }
   }
-  return launch_elasticity(c, model, E, nu, precision, with_res, 1);
+  return launch_elasticity(c, model, E, nu, precision, with_res, 1, Evec);
 }
 
 int fedd_assembly_linear_elasticity(fedd_ctx* c, double E, double nu, int precision) {
@@ -775,6 +776,18 @@ int fedd_assembly_linear_elasticity(fedd_ctx* c, double E, double nu, int precis
 int fedd_assembly_nonlinear_elasticity(fedd_ctx* c, int neo_hooke_model, double E, double nu, int precision) {
   // "Neo-Hooke Modell" == 1 -> NonLinearElasticity (skr2), otherwise NonLinearElasticity2 (skr3)
   return elasticity_common(c, neo_hooke_model == 1 ? 1 : 2, E, nu, precision, 1);
+}
+
+int fedd_assembly_nonlinear_elasticity_efield(fedd_ctx* c, int neo_hooke_model, const double* E_elem, int64_t nelem,
+                                              double nu, int precision) {
+  if (!c || !E_elem) return FEDD_ERR_ARG;
+  if (nelem != c->mesh[0].nelem) return fail(c, FEDD_ERR_ARG, "Number Elements not the same as the E field length");
+  for (int64_t e = 0; e < nelem; ++e)
+    if (!(E_elem[e] > 0)) return fail(c, FEDD_ERR_ARG, "E must be > 0 in every element");
+  (void)hipSetDevice(c->device);
+  int rc = upload(c, c->d_emod, E_elem, nelem * 8);
+  if (rc) return rc;
+  return elasticity_common(c, neo_hooke_model == 1 ? 1 : 2, E_elem[0], nu, precision, 1, c->d_emod.as<double>());
 }
 
 int fedd_newton_step(fedd_ctx* c, int* step) {

@@ -86,6 +86,8 @@ _SIGS = {
     "fedd_assembly_linear_elasticity": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_int]),
     "fedd_assembly_nonlinear_elasticity": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                                           ctypes.c_int]),
+    "fedd_assembly_nonlinear_elasticity_efield": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.c_int64,
+                                                                 ctypes.c_double, ctypes.c_int]),
     "fedd_dirichlet_set": (ctypes.c_int, [_vp, _ip, ctypes.c_int64]),
     "fedd_apply_dirichlet": (ctypes.c_int, [_vp]),
     "fedd_traction_load": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, _dp]),
@@ -270,6 +272,11 @@ class Context:
 
     def assembly_nonlinear_elasticity(self, neo_hooke_model, E, nu, precision=FP64):
         self._chk(self._L.fedd_assembly_nonlinear_elasticity(self.h, neo_hooke_model, E, nu, precision))
+
+    def assembly_nonlinear_elasticity_efield(self, neo_hooke_model, E_elem, nu, precision=FP64):
+        E_elem = np.ascontiguousarray(E_elem, np.float64)
+        self._chk(self._L.fedd_assembly_nonlinear_elasticity_efield(self.h, neo_hooke_model, _p(E_elem, _dp),
+                                                                    E_elem.size, nu, precision))
 
     def dirichlet_set(self, dofs):
         d = np.ascontiguousarray(dofs, np.int32)

@@ -164,6 +164,11 @@ int fedd_assembly_linear_elasticity(fedd_ctx* ctx, double E, double nu, int prec
  * neo_hooke_model = Parameter/"Neo-Hooke Modell": 1 -> skr2, otherwise skr3. */
 int fedd_assembly_nonlinear_elasticity(fedd_ctx* ctx, int neo_hooke_model, double E, double nu,
                                        int precision);
+/* The overload with a per-element Young's modulus field eModVec
+ * (FE::assemblyNonLinearElasticity(..., domain, eModVec, ...), FE_def.hpp:538-614,
+ * updateParameter("E", eModVecA[T]) per element): E_elem has one entry per element. */
+int fedd_assembly_nonlinear_elasticity_efield(fedd_ctx* ctx, int neo_hooke_model, const double* E_elem,
+                                              int64_t nelem, double nu, int precision);
 /* ------------------------------------------------------- nonlinear solve */
 /* Dirichlet dofs of the primary field (BCBuilder::addBC / setSystem,
  * BCBuilder_def.hpp:590-618): constrained rows and columns of the assembled tangent

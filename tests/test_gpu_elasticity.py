@@ -90,3 +90,45 @@ def test_zero_displacement_is_linear_elasticity(gpu_ctx):
         assert np.max(np.abs(r)) < 1e-12 * np.max(np.abs(vals[-1]))
     rp, _ = gpu_ctx.block_graph(N.BLOCK_A)
     assert parity_ratio(vals[0], vals[1], rp, rtol=1e-11) <= 1.0
+
+
+@pytest.mark.parametrize("model", [1, 2])
+def test_elasticity_efield(gpu_ctx, model):
+    """Per-element Young's modulus (FE_def.hpp:538-614): two E values on a checker of
+    elements; the oracle assembles each element group with its own E and the two
+    partial systems are summed."""
+    import scipy.sparse as sp
+    pts, el, u = _setup(gpu_ctx, 2)
+    nel = el.shape[0]
+    group = (np.arange(nel) % 3 == 0)
+    E = np.where(group, 2500.0, 700.0)
+    n = 3 * pts.shape[0]
+    mats, res = [], np.zeros(n)
+    for mask, Ev in ((group, 2500.0), (~group, 700.0)):
+        (rp, ci, v), r = O.elasticity(model, Ev, NU, pts, np.ascontiguousarray(el[mask]), u)
+        mats.append(sp.csr_matrix((v, ci, rp), shape=(n, n)))
+        res += r
+    ref = (mats[0] + mats[1]).tocsr()
+    gpu_ctx.assembly_nonlinear_elasticity_efield(1 if model == 1 else 2, E, NU, N.FP64)
+    rp, ci = gpu_ctx.block_graph(N.BLOCK_A)
+    v = gpu_ctx.block_values(N.BLOCK_A)
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    refv = np.asarray(ref[rows, ci]).ravel()
+    assert parity_ratio(v, refv, rp) <= 1.0
+    r = gpu_ctx.vector_download(N.VEC_RES_U, n)
+    assert vec_rel_err(r, res) <= 1e-12
+    # constant field == the constant-E entry point
+    gpu_ctx.assembly_nonlinear_elasticity_efield(1 if model == 1 else 2, np.full(nel, EMOD), NU, N.FP64)
+    v1 = gpu_ctx.block_values(N.BLOCK_A)
+    gpu_ctx.assembly_nonlinear_elasticity(1 if model == 1 else 2, EMOD, NU, N.FP64)
+    assert np.array_equal(v1, gpu_ctx.block_values(N.BLOCK_A))
+
+
+def test_efield_rejects_bad_field(gpu_ctx):
+    pts, el, u = _setup(gpu_ctx, 1)
+    with pytest.raises(N.FeddError):
+        gpu_ctx.assembly_nonlinear_elasticity_efield(1, np.full(el.shape[0] - 1, EMOD), NU)
+    bad = np.full(el.shape[0], EMOD)
+    bad[3] = -1.0
+    with pytest.raises(N.FeddError):
+        gpu_ctx.assembly_nonlinear_elasticity_efield(1, bad, NU)
