@@ -121,7 +121,9 @@ def test_elasticity_efield(gpu_ctx, model):
     gpu_ctx.assembly_nonlinear_elasticity_efield(1 if model == 1 else 2, np.full(nel, EMOD), NU, N.FP64)
     v1 = gpu_ctx.block_values(N.BLOCK_A)
     gpu_ctx.assembly_nonlinear_elasticity(1 if model == 1 else 2, EMOD, NU, N.FP64)
-    assert np.array_equal(v1, gpu_ctx.block_values(N.BLOCK_A))
+    v0 = gpu_ctx.block_values(N.BLOCK_A)
+    # equal up to the order of the LDS atomics
+    assert np.max(np.abs(v1 - v0)) <= 1e-13 * np.max(np.abs(v0))
 
 
 def test_efield_rejects_bad_field(gpu_ctx):
