@@ -33,6 +33,21 @@ COPY_BYTES = 1 << 30
 NU, RHO = 0.01, 1.0
 
 
+TRAFFIC_FILE = "profiles/ns_vel_rows_M40_hbm_counters.txt"
+
+
+def profiled_traffic():
+    """HBM bytes per launch of k_ns_vel_rows<double> at 40^3x6 tets, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel calibration)."""
+    path = os.path.join(ROOT, TRAFFIC_FILE)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -133,6 +148,8 @@ def main():
     vel_avg_ms = kv64[0] / max(kv64[1], 1)
     achieved = bytes_vel / (vel_avg_ms * 1e-3) / 1e9
 
+    traffic = profiled_traffic() if M == 40 else None
+
     cpu = None
     if args.cpu_sample > 0:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -166,7 +183,8 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "kernel": "k_ns_vel_rows<double>", "achieved": achieved,
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "traffic_source": TRAFFIC_FILE if traffic else None,
             "algorithmic_bytes_per_launch": bytes_vel, "avg_launch_ms": vel_avg_ms,
         },
         "step": {

@@ -114,3 +114,37 @@ def test_laplace_and_rhs(gpu_ctx, fe, M):
     bref = O.rhs(fe, pts, el, 1.0)
     assert vec_rel_err(b, bref) <= 1e-12
     assert abs(b.sum() - 1.0) < 1e-12  # int_Omega f = |Omega| = 1
+
+
+def _greedy_colours(el):
+    """First-fit colouring in ascending element order (restated for the check)."""
+    inc = {}
+    for e, T in enumerate(el):
+        for n in T:
+            inc.setdefault(int(n), []).append(e)
+    col = np.full(len(el), -1)
+    for e, T in enumerate(el):
+        used = {col[f] for n in T for f in inc[int(n)] if f < e}
+        k = 0
+        while k in used:
+            k += 1
+        col[e] = k
+    return col
+
+
+@pytest.mark.parametrize("fe", [1, 2])
+def test_colouring_conflict_free(gpu_ctx, fe):
+    pts, _, el = N.mesh_structured(3, fe)
+    gpu_ctx.mesh_upload(N.MESH_PRIMARY, pts, el)
+    nc = gpu_ctx.colour_build()
+    cptr, perm = gpu_ctx.colour_export(nc, el.shape[0])
+    assert cptr[0] == 0 and cptr[-1] == el.shape[0] and np.all(np.diff(cptr) > 0)
+    assert np.array_equal(np.sort(perm), np.arange(el.shape[0]))
+    col = np.empty(el.shape[0], np.int64)
+    for k in range(nc):
+        members = perm[cptr[k]:cptr[k + 1]]
+        assert np.all(np.diff(members) > 0)  # ascending element id inside a colour
+        nodes = el[members].ravel()
+        assert len(np.unique(nodes)) == len(nodes)  # no two elements of a colour share a node
+        col[members] = k
+    assert np.array_equal(col, _greedy_colours(el))  # bit-exact vs the restated greedy rule
