@@ -18,6 +18,7 @@ int fail(Ctx* c, int code, const std::string& msg) {
 }
 
 int alloc(Ctx* c, DevBuf& b, size_t bytes) {
+  if (c->capturing) return fail(c, FEDD_ERR_STATE, "device allocation while a capture is open: run the call once before capturing");
   b.release();
   if (bytes == 0) return FEDD_OK;
   FEDD_HIP(c, hipMalloc(&b.p, bytes));
@@ -26,8 +27,10 @@ int alloc(Ctx* c, DevBuf& b, size_t bytes) {
 }
 
 int upload(Ctx* c, DevBuf& b, const void* host, size_t bytes) {
-  int rc = alloc(c, b, bytes);
-  if (rc) return rc;
+  if (b.bytes != bytes) {  // same size: keep the buffer (captured graphs keep its address)
+    int rc = alloc(c, b, bytes);
+    if (rc) return rc;
+  }
   if (bytes) FEDD_HIP(c, hipMemcpyAsync(b.p, host, bytes, hipMemcpyHostToDevice, c->stream));
   return FEDD_OK;
 }
@@ -58,14 +61,14 @@ void ktimer_resolve(Ctx* c) {
 }
 
 void ktimer_begin(Ctx* c) {
-  if (!c->timing) return;
+  if (!c->timing || c->capturing) return;
   if (c->kpending.size() >= 4096) ktimer_resolve(c);
   c->kopen = kevent(c);
   (void)hipEventRecord(c->kopen, c->stream);
 }
 
 void ktimer_end(Ctx* c, const char* name) {
-  if (!c->timing || !c->kopen) return;
+  if (!c->timing || c->capturing || !c->kopen) return;
   hipEvent_t b = kevent(c);
   (void)hipEventRecord(b, c->stream);
   c->kpending.push_back({name, c->kopen, b});
@@ -399,7 +402,14 @@ int fedd_ctx_create(int device, fedd_ctx** out) {
 void fedd_ctx_destroy(fedd_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
+  if (c->capturing) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(c->stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+  }
   (void)hipStreamSynchronize(c->stream);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
   mlp_free(c);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -678,6 +688,7 @@ static int64_t vec_len(fedd_ctx* c, int which) {
 
 int fedd_vector_upload(fedd_ctx* c, int which, const double* host, int64_t n) {
   if (!c || which < 0 || which >= FEDD_NUM_VECTORS || !host) return FEDD_ERR_ARG;
+  if (c->capturing) return fail(c, FEDD_ERR_STATE, "host transfers cannot be captured");
   int64_t want = vec_len(c, which);
   if (n != want) return fail(c, FEDD_ERR_ARG, "vector length does not match the mesh");
   (void)hipSetDevice(c->device);
@@ -685,6 +696,49 @@ int fedd_vector_upload(fedd_ctx* c, int which, const double* host, int64_t n) {
   if (rc) return rc;
   c->vec_n[which] = n;
   FEDD_HIP(c, hipStreamSynchronize(c->stream));
+  return FEDD_OK;
+}
+
+// ------------------------------------------------------------ graph capture
+int fedd_capture_begin(fedd_ctx* c) {
+  if (!c) return FEDD_ERR_ARG;
+  if (c->capturing) return fail(c, FEDD_ERR_STATE, "a capture is already open");
+  (void)hipSetDevice(c->device);
+  FEDD_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+  c->capturing = true;
+  return FEDD_OK;
+}
+
+int fedd_capture_end(fedd_ctx* c, int64_t* nodes) {
+  if (!c) return FEDD_ERR_ARG;
+  if (!c->capturing) return fail(c, FEDD_ERR_STATE, "no capture is open");
+  (void)hipSetDevice(c->device);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &g);
+  c->capturing = false;
+  if (e != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    return fail(c, FEDD_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  }
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = g;
+  FEDD_HIP(c, hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+  if (nodes) {
+    size_t cnt = 0;
+    FEDD_HIP(c, hipGraphGetNodes(g, nullptr, &cnt));
+    *nodes = (int64_t)cnt;
+  }
+  return FEDD_OK;
+}
+
+int fedd_graph_launch(fedd_ctx* c) {
+  if (!c) return FEDD_ERR_ARG;
+  if (!c->gexec) return fail(c, FEDD_ERR_STATE, "no captured graph");
+  if (c->capturing) return fail(c, FEDD_ERR_STATE, "a capture is open");
+  (void)hipSetDevice(c->device);
+  FEDD_HIP(c, hipGraphLaunch(c->gexec, c->stream));
   return FEDD_OK;
 }
 

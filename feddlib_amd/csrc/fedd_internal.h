@@ -109,6 +109,10 @@ struct Ctx {
   struct MlpHost* mlp = nullptr;
   bool mlp_ready = false;
   int newton_step = 0;
+  // stream capture (fedd_capture_begin / end) and the instantiated graph
+  bool capturing = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
   // timing
   hipEvent_t ev[16] = {};
   bool timing = false;

@@ -902,18 +902,14 @@ int launch_ns_assembly(Ctx* c, const fedd_ns_params* p, int mode, int reassemble
       ktimer_end(c, "ns_pres_rows");
     }
   } else {
+    // The Newton residual stays FP64 in both precision modes (FP32-local applies to
+    // the element matrices only).
     NsArgs a = ns_args(c, p, c->vplan);
     ktimer_begin(c);
-    if (f32)
-      k_ns_res_vel<float><<<(unsigned)c->vplan.nchunks, 256, 3 * 256 * 8, c->stream>>>(a);
-    else
-      k_ns_res_vel<double><<<(unsigned)c->vplan.nchunks, 256, 3 * 256 * 8, c->stream>>>(a);
+    k_ns_res_vel<double><<<(unsigned)c->vplan.nchunks, 256, 3 * 256 * 8, c->stream>>>(a);
     FEDD_HIP(c, hipGetLastError());
     NsArgs b = ns_args(c, p, c->pplan);
-    if (f32)
-      k_ns_res_pres<float><<<(unsigned)c->pplan.nchunks, 256, 256 * 8, c->stream>>>(b);
-    else
-      k_ns_res_pres<double><<<(unsigned)c->pplan.nchunks, 256, 256 * 8, c->stream>>>(b);
+    k_ns_res_pres<double><<<(unsigned)c->pplan.nchunks, 256, 256 * 8, c->stream>>>(b);
     FEDD_HIP(c, hipGetLastError());
     ktimer_end(c, "ns_residual");
   }

@@ -104,6 +104,9 @@ _SIGS = {
     "fedd_mlp_forward": (ctypes.c_int, [_vp, _fp, ctypes.c_int64, _fp]),
     "fedd_mlp_gradient": (ctypes.c_int, [_vp, _fp, _fp, ctypes.c_int64, _fp, _fp]),
     "fedd_mlp_norm": (ctypes.c_int, [_vp, _fp, _fp, _dp]),
+    "fedd_capture_begin": (ctypes.c_int, [_vp]),
+    "fedd_capture_end": (ctypes.c_int, [_vp, _lp]),
+    "fedd_graph_launch": (ctypes.c_int, [_vp]),
     "fedd_stream_copy": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]),
     "fedd_event_record": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fedd_event_elapsed": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int,
@@ -360,6 +363,18 @@ class Context:
         so, si = ctypes.c_float(), ctypes.c_double()
         self._chk(self._L.fedd_mlp_norm(self.h, _p(cc0, _fp), ctypes.byref(so), ctypes.byref(si)))
         return cc0, so.value, si.value
+
+    # graph capture
+    def capture_begin(self):
+        self._chk(self._L.fedd_capture_begin(self.h))
+
+    def capture_end(self):
+        n = ctypes.c_int64()
+        self._chk(self._L.fedd_capture_end(self.h, ctypes.byref(n)))
+        return n.value
+
+    def graph_launch(self):
+        self._chk(self._L.fedd_graph_launch(self.h))
 
     # timing
     def synchronize(self):

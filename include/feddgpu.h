@@ -72,8 +72,8 @@ enum fedd_vector {
 
 enum fedd_precision {
   FEDD_FP64 = 0,       /* everything in FP64 */
-  FEDD_FP32_LOCAL = 1  /* element-local products in FP32, geometry (Binv, |det|) and the
-                          global accumulation in FP64 */
+  FEDD_FP32_LOCAL = 1  /* element-matrix products in FP32; geometry (Binv, |det|), the global
+                          accumulation and every residual (NS Rhs mode, elasticity) in FP64 */
 };
 
 enum fedd_ns_mode { FEDD_NS_JACOBIAN = 0, FEDD_NS_FIXEDPOINT = 1, FEDD_NS_RHS = 2 };
@@ -252,6 +252,20 @@ int fedd_mlp_norm(fedd_ctx* ctx, float* cc0, float* s_out, double* s_in);
 /* AssembleFE::advanceNewtonStep / getNewtonStep (AssembleFE_decl.hpp:146) for the
  * NS elements: the counter the reference keeps per element. */
 int fedd_newton_step(fedd_ctx* ctx, int* step);
+
+/* ---------------------------------------------------------- graph capture
+ * No counterpart in the reference.  Records the work that the calls between begin
+ * and end enqueue on the context stream into one HIP graph; fedd_graph_launch
+ * replays it (stream-ordered, asynchronous).  Inputs are read from the context's
+ * device buffers at replay time, so update them with fedd_vector_upload between
+ * replays (same lengths keep the buffers in place).  Calls that allocate, transfer
+ * host data or synchronise fail with FEDD_ERR_STATE (or FEDD_ERR_HIP) while a capture
+ * is open: run an assembly once before capturing it.  The NS Newton-step counter
+ * advances only for the captured call, not per replay.  No memset nodes are
+ * recorded by the assembly calls. */
+int fedd_capture_begin(fedd_ctx* ctx);
+int fedd_capture_end(fedd_ctx* ctx, int64_t* nodes);  /* nodes: graph node count (may be NULL) */
+int fedd_graph_launch(fedd_ctx* ctx);
 
 /* ----------------------------------------------------------------- timing */
 /* STREAM-style copy of `bytes` (read + write) repeated `iters` times on the context

@@ -52,3 +52,35 @@ def rel_err_rowfloor(got, ref, rowptr, floor=1e-3):
 def vec_rel_err(got, ref):
     s = np.max(np.abs(ref))
     return float(np.max(np.abs(got - ref)) / (s if s > 0 else 1.0))
+
+
+def incident_elements(elems, nodes):
+    """Elements that contain any of `nodes` (sorted ids)."""
+    mask = np.isin(elems, nodes).any(axis=1)
+    return np.nonzero(mask)[0]
+
+
+def rows_of(rowptr, colind, vals, rows):
+    """(row, col) -> value dict-free extraction: concatenated (cols, vals) of `rows`."""
+    idx = np.concatenate([np.arange(rowptr[r], rowptr[r + 1]) for r in rows])
+    return colind[idx], vals[idx], idx
+
+
+def subset_rows_parity(got, ref, rows, rtol=1e-12, atol_row=1e-14):
+    """Parity of selected rows.  got/ref = (rowptr, colind, vals) of two CRS matrices whose
+    rows `rows` must hold the same columns (ref assembled from every element touching
+    those rows, so they are complete); returns the max parity ratio over those rows."""
+    gr, gc, gv = got
+    rr, rc, rv = ref
+    worst = 0.0
+    for r in rows:
+        a = slice(gr[r], gr[r + 1])
+        b = slice(rr[r], rr[r + 1])
+        assert np.array_equal(gc[a], rc[b]), "row %d: columns differ" % r
+        x, y = gv[a], rv[b]
+        rm = np.max(np.abs(y)) if y.size else 0.0
+        tol = rtol * np.maximum(np.abs(y), 1e-3 * rm) + atol_row * rm
+        tol[tol == 0] = np.finfo(float).tiny
+        if y.size:
+            worst = max(worst, float(np.max(np.abs(x - y) / tol)))
+    return worst
