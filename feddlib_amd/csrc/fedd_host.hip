@@ -173,6 +173,10 @@ struct PlanSpec {
   int lds_budget_doubles;
 };
 
+#ifndef FEDD_PAIR_ORDER
+#define FEDD_PAIR_ORDER 0
+#endif
+
 int build_plan(Ctx* c, const Incidence& inc, int64_t nnodes, const Mesh& rows_mesh,
                const PlanSpec& s, GatherPlan& plan) {
   std::vector<int32_t> chunk_node(1, 0), chunk_pair(1, 0), chunk_split;
@@ -211,7 +215,9 @@ int build_plan(Ctx* c, const Incidence& inc, int64_t nnodes, const Mesh& rows_me
         int cls = (rows_mesh.npe == 10 && li >= 4) ? 1 : 0;
         // class first; inside a class the k-th pairs of all nodes follow each
         // other, so neighbouring lanes add into different rows
-        int64_t key = ((int64_t)cls << 60) | ((int64_t)rank[cls]++ << 20) | (i - n0);
+        // (FEDD_PAIR_ORDER 1: by element inside a class, lanes share element data)
+        int64_t key = FEDD_PAIR_ORDER == 1 ? (((int64_t)cls << 60) | ((int64_t)inc.elem[q] << 20) | (i - n0))
+                                           : (((int64_t)cls << 60) | ((int64_t)rank[cls]++ << 20) | (i - n0));
         order.push_back({key, i, q});
       }
     }

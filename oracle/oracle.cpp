@@ -10,6 +10,7 @@
 #include <cstring>
 #include <limits>
 #include <utility>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -970,6 +971,67 @@ double orc_time_ns_jacobian(const double* pts2, int64_t np2, const int32_t* el2,
   orc_crs_destroy(BT);
   orc_crs_destroy(B);
   orc_crs_destroy(C);
+  return std::chrono::duration<double>(t1 - t0).count();
+}
+
+// The reference's unit of parallelism is an MPI rank assembling its own elements
+// into its own rows, followed by the export-Add of globalAssemble/fillComplete.
+// Here `nthreads` element partitions are assembled concurrently (each with its own
+// row-wise insertion and fillComplete), then the partial matrices are merged row by
+// row in parallel.  Returns the wall time of both phases.
+static Crs* merge_parts(const std::vector<Crs*>& parts, int nthreads) {
+  const int64_t n = (int64_t)parts[0]->rows.size();
+  Crs* M = new Crs(n);
+  std::vector<std::thread> th;
+  for (int t = 0; t < nthreads; ++t)
+    th.emplace_back([&, t]() {
+      const int64_t r0 = n * t / nthreads, r1 = n * (t + 1) / nthreads;
+      for (int64_t r = r0; r < r1; ++r) {
+        auto& out = M->rows[r];
+        for (const Crs* P : parts) out.insert(out.end(), P->rows[r].begin(), P->rows[r].end());
+        std::stable_sort(out.begin(), out.end(),
+                         [](const std::pair<int32_t, double>& a, const std::pair<int32_t, double>& b) {
+                           return a.first < b.first;
+                         });
+        size_t w = 0;
+        for (size_t k = 0; k < out.size(); ++k) {
+          if (w > 0 && out[w - 1].first == out[k].first)
+            out[w - 1].second += out[k].second;
+          else
+            out[w++] = out[k];
+        }
+        out.resize(w);
+      }
+    });
+  for (auto& x : th) x.join();
+  M->filled = true;
+  return M;
+}
+
+double orc_time_ns_jacobian_mt(const double* pts2, int64_t np2, const int32_t* el2, const int32_t* el1,
+                               int64_t nelem, int64_t np1, const double* u, const double* p, double nu,
+                               double rho, int nthreads) {
+  if (nthreads < 1) nthreads = 1;
+  static const double one4[4] = {1, 1, 1, 1};
+  auto t0 = std::chrono::steady_clock::now();
+  std::vector<void*> parts(4 * nthreads, nullptr);
+  std::vector<std::thread> th;
+  for (int t = 0; t < nthreads; ++t)
+    th.emplace_back([&, t]() {
+      const int64_t e0 = nelem * t / nthreads, e1 = nelem * (t + 1) / nthreads;
+      orc_assembly_ns(pts2, np2, el2 + 10 * e0, el1 + 4 * e0, e1 - e0, np1, u, p, nu, rho, 1, 0, 0, one4,
+                      &parts[4 * t], &parts[4 * t + 1], &parts[4 * t + 2], &parts[4 * t + 3], nullptr, nullptr);
+    });
+  for (auto& x : th) x.join();
+  std::vector<Crs*> merged;
+  for (int b = 0; b < 4; ++b) {
+    std::vector<Crs*> blk;
+    for (int t = 0; t < nthreads; ++t) blk.push_back(static_cast<Crs*>(parts[4 * t + b]));
+    merged.push_back(nthreads > 1 ? merge_parts(blk, nthreads) : nullptr);
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  for (auto* q : parts) orc_crs_destroy(q);
+  for (auto* m : merged) delete m;
   return std::chrono::duration<double>(t1 - t0).count();
 }
 

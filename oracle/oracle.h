@@ -91,6 +91,11 @@ void orc_assembly_ns(const double* pts2, int64_t np2, const int32_t* el2, const 
 double orc_time_ns_jacobian(const double* pts2, int64_t np2, const int32_t* el2, const int32_t* el1,
                             int64_t nelem, int64_t np1, const double* u, const double* p,
                             double nu, double rho);
+/* As orc_time_ns_jacobian on all elements, with `nthreads` element partitions assembled
+ * concurrently and merged row by row (the reference's ranks + export-Add). */
+double orc_time_ns_jacobian_mt(const double* pts2, int64_t np2, const int32_t* el2, const int32_t* el1,
+                               int64_t nelem, int64_t np1, const double* u, const double* p, double nu,
+                               double rho, int nthreads);
 
 /* ---- Hyperelastic P2 tets (AssembleFE_{LinElas,NonLinElas,NonLinElas2}_def.hpp) ---- */
 /* model: 0 = "LinearElasticity" (skr1 law, 4-pt rule), 1 = "NonLinearElasticity" (skr2,

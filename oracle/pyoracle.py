@@ -55,6 +55,9 @@ def lib():
             "orc_time_ns_jacobian": (ctypes.c_double, [_dp, ctypes.c_int64, _ip, _ip, ctypes.c_int64,
                                                        ctypes.c_int64, _dp, _dp, ctypes.c_double,
                                                        ctypes.c_double]),
+            "orc_time_ns_jacobian_mt": (ctypes.c_double, [_dp, ctypes.c_int64, _ip, _ip, ctypes.c_int64,
+                                                          ctypes.c_int64, _dp, _dp, ctypes.c_double,
+                                                          ctypes.c_double, ctypes.c_int]),
             "orc_nh_skr": (None, [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _dp]),
             "orc_assembly_elasticity": (_vp, [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp,
                                               ctypes.c_int64, _ip, ctypes.c_int64, _dp, _dp]),
@@ -161,6 +164,13 @@ def ns_assembly(p2, e2, e1, np1, u, p, nu, rho, newton, mode, reassemble, coeff=
         out["res_u"] = ru
         out["res_p"] = rp
     return out
+
+
+def time_ns_jacobian_mt(p2, e2, e1, np1, u, p, nu, rho, nthreads):
+    """Wall time of the whole element loop on `nthreads` host threads (partitions + merge)."""
+    uu = np.ascontiguousarray(u, np.float64).ravel()
+    return lib().orc_time_ns_jacobian_mt(P(p2), p2.shape[0], P(e2, _ip), P(e1, _ip), e2.shape[0], np1, P(uu),
+                                         P(np.ascontiguousarray(p, np.float64)), nu, rho, nthreads)
 
 
 def time_ns_jacobian(p2, e2, e1, np1, u, p, nu, rho, nelem):
