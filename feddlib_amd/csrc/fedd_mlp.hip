@@ -19,6 +19,13 @@
 #include "fedd_internal.h"
 #include "fedd_law.h"
 
+#ifndef FEDD_MLP_BPC
+#define FEDD_MLP_BPC 2  // inference workgroups per CU
+#endif
+#ifndef FEDD_MLP_TPW
+#define FEDD_MLP_TPW 2  // 16-sample tiles per wave in a training step
+#endif
+
 namespace fedd {
 
 typedef short s4 __attribute__((ext_vector_type(4)));
@@ -476,7 +483,8 @@ int launch_mlp_tangent(Ctx* c, int nq) {
   MlpHost* h = c->mlp;
   if (!h || !c->mlp_ready) return fail(c, FEDD_ERR_STATE, "no surrogate network loaded");
   const int64_t nrec = c->mesh[0].nelem * nq;
-  const unsigned grid = (unsigned)std::min<int64_t>((nrec + 63) / 64, (int64_t)c->num_cu * 8);
+  // one workgroup per CU stages the weights once and walks its tiles
+  const unsigned grid = (unsigned)std::min<int64_t>((nrec + 63) / 64, (int64_t)c->num_cu * FEDD_MLP_BPC);
   ktimer_begin(c);
   k_mlp_tangent<<<grid, 256, 0, c->stream>>>(c->d_qp.as<double>(), nrec, h->wb(), h->p(), h->nrm);
   FEDD_HIP(c, hipGetLastError());
@@ -647,7 +655,7 @@ int fedd_mlp_num_samples(fedd_ctx* c, int64_t* n) {
 
 static int train_step(Ctx* c, const int64_t* d_idx, int64_t nb, float lr, int64_t t) {
   MlpHost* h = c->mlp;
-  const int tiles_per_wave = 4;
+  const int tiles_per_wave = FEDD_MLP_TPW;
   const int64_t ntile = (nb + 15) / 16;
   const unsigned grid = (unsigned)((ntile + 4 * tiles_per_wave - 1) / (4 * tiles_per_wave));
   k_mlp_train<<<grid, 256, 0, c->stream>>>(h->d_x.as<float>(), h->d_t.as<float>(), d_idx, nb, h->wb(), h->p(),
@@ -747,7 +755,7 @@ int fedd_mlp_gradient(fedd_ctx* c, const float* x, const float* t, int64_t n, fl
   FEDD_HIP(c, hipMemcpyAsync(didx, idx.data(), n * 8, hipMemcpyHostToDevice, c->stream));
   FEDD_HIP(c, hipMemsetAsync(h->g(), 0, P_TOTAL * 4, c->stream));
   FEDD_HIP(c, hipMemsetAsync(h->loss.p, 0, 4, c->stream));
-  const int tiles_per_wave = 4;
+  const int tiles_per_wave = FEDD_MLP_TPW;
   const int64_t ntile = (n + 15) / 16;
   const unsigned grid = (unsigned)((ntile + 4 * tiles_per_wave - 1) / (4 * tiles_per_wave));
   k_mlp_train<<<grid, 256, 0, c->stream>>>(dx, dt, didx, n, h->wb(), h->p(), h->g(), h->loss.as<float>(),
