@@ -417,6 +417,7 @@ void fedd_ctx_destroy(fedd_ctx* c) {
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
   mlp_free(c);
+  mg_free(c);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   ktimer_resolve(c);
@@ -435,6 +436,7 @@ int fedd_synchronize(fedd_ctx* c) {
 
 int fedd_mesh_upload(fedd_ctx* c, int slot, const double* pts, int64_t npts,
                      const int32_t* elems, int64_t nelem, int npe) {
+  if (c) mg_free(c);
   if (!c) return FEDD_ERR_ARG;
   if (slot < 0 || slot > 1) return fail(c, FEDD_ERR_ARG, "mesh slot must be 0 or 1");
   if (!pts || !elems || npts <= 0 || nelem <= 0)
@@ -473,6 +475,7 @@ int fedd_mesh_upload(fedd_ctx* c, int slot, const double* pts, int64_t npts,
 }
 
 int fedd_graph_build(fedd_ctx* c, int kind) {
+  if (c) mg_free(c);
   if (!c) return FEDD_ERR_ARG;
   if (kind < FEDD_GRAPH_SCALAR || kind > FEDD_GRAPH_NS) return fail(c, FEDD_ERR_ARG, "unknown graph kind");
   Mesh& m = c->mesh[0];

@@ -18,6 +18,20 @@ struct DevBuf {
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      bytes = o.bytes;
+      o.p = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
   ~DevBuf() { release(); }
   void release() {
     if (p) (void)hipFree(p);
@@ -107,6 +121,10 @@ struct Ctx {
   DevBuf d_scal;        // device scalars of the Krylov loop
   // learned tangent surrogate (FP32 master weights + BF16 copies)
   struct MlpHost* mlp = nullptr;
+  // geometric multigrid hierarchy (structured lattice meshes), rebuilt on demand
+  struct MgHost* mg = nullptr;
+  int precond = 0;                      // 0: block Jacobi, 1: geometric multigrid
+  std::vector<uint8_t> bcflag_host;     // Dirichlet flags per dof (host copy)
   bool mlp_ready = false;
   int newton_step = 0;
   // stream capture (fedd_capture_begin / end) and the instantiated graph
@@ -163,6 +181,12 @@ void mlp_free(Ctx* c);
 bool mlp_matches(const Ctx* c, int model, double Emod, double nu);
 // Appends the quadrature-point states in d_qp to the surrogate's training set.
 int mlp_collect_qp(Ctx* c);
+// Geometric multigrid (fedd_mg.hip): hierarchy for the current mesh / Dirichlet set,
+// coarse operators for the current A, and one V-cycle x = M^-1 b on the finest level.
+int mg_setup(Ctx* c);
+int mg_refresh(Ctx* c);
+int mg_vcycle(Ctx* c, const double* b, double* x);
+void mg_free(Ctx* c);
 
 }  // namespace fedd
 

@@ -1,0 +1,637 @@
+// Geometric multigrid preconditioner for the P2 vector (elasticity) problem on the
+// structured unit-cube lattice of MeshStructured::buildMesh3D.
+//
+// Levels: the P2 space on mesh M (lattice of P0 = 2M+1 points per axis), then P1 on
+// meshes M, M/2, M/4, ... (lattices P_(k+1) = (P_k + 1) / 2) while P_k - 1 is even and
+// P_k >= 5.  Every prolongation is the same lattice rule: a fine node whose lattice
+// coordinates are odd on the axis set S is the midpoint of the Kuhn edge e_S between
+// the coarse nodes (x - e_S)/2 and (x + e_S)/2 (weights 1/2), an all-even node copies
+// the coarse node x/2.  P1 on a mesh is embedded in P2 on the same mesh, and the
+// Kuhn (Freudenthal) meshes are nested, so every level is a conforming subspace.
+// Coarse operators are Galerkin products P^T A P on the 3x3 node blocks; Dirichlet
+// dofs are masked out of P (rows and columns) and get identity rows on every level.
+// Smoother: damped 3x3 block Jacobi (nu sweeps before and after); coarsest level:
+// dense inverse (Gauss-Jordan on the device).  The V-cycle is symmetric, so it is a
+// valid CG preconditioner.
+//
+// Reference role: the FROSch / MueLu preconditioners behind Belos
+// (Preconditioner_def.hpp:206-291, LinearSolver_def.hpp:23-137).
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "fedd_internal.h"
+
+#ifndef FEDD_MG_NU
+#define FEDD_MG_NU 2  // smoothing sweeps before and after the coarse correction
+#endif
+#ifndef FEDD_MG_OMEGA
+#define FEDD_MG_OMEGA 0.6  // block-Jacobi damping
+#endif
+
+namespace fedd {
+
+static inline unsigned nbk(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+struct MgLevel {
+  int64_t P = 0, nn = 0, nblk = 0, maxdeg = 0;
+  DevBuf nptr, nidx, vals;  // node-block operator (levels >= 1)
+  const int64_t* d_nptr = nullptr;
+  const int32_t* d_nidx = nullptr;
+  double* d_vals = nullptr;
+  DevBuf binv, free, x, x2, b, r;
+  std::vector<int64_t> h_ptr;
+  std::vector<int32_t> h_idx;
+};
+
+struct MgHost {
+  std::vector<MgLevel> lv;
+  DevBuf dense, inv;  // coarsest level: Gauss-Jordan ping-pong buffers (n x n); inverse in inv
+  int64_t ncoarse = 0;
+  // replayed HIP graphs of the V-cycle (for one (b, x) pair) and of the refresh
+  hipGraphExec_t vexec = nullptr, rexec = nullptr;
+  const double* vb = nullptr;
+  double* vx = nullptr;
+  ~MgHost() {
+    if (vexec) (void)hipGraphExecDestroy(vexec);
+    if (rexec) (void)hipGraphExecDestroy(rexec);
+  }
+};
+
+void mg_free(Ctx* c) {
+  delete c->mg;
+  c->mg = nullptr;
+}
+
+// ------------------------------------------------------------------ kernels
+__device__ __forceinline__ void lattice(int64_t i, int64_t P, int64_t x[3]) {
+  x[0] = i % P;
+  x[1] = (i / P) % P;
+  x[2] = i / (P * P);
+}
+
+// Node i of lattice P: parents in lattice Pc = (P + 1) / 2 and their weights.
+__device__ __forceinline__ int parents(int64_t i, int64_t P, int64_t Pc, int64_t par[2], double w[2]) {
+  int64_t x[3];
+  lattice(i, P, x);
+  const int64_t o0 = x[0] & 1, o1 = x[1] & 1, o2 = x[2] & 1;
+  if (!(o0 | o1 | o2)) {
+    par[0] = (x[0] >> 1) + Pc * ((x[1] >> 1) + Pc * (x[2] >> 1));
+    w[0] = 1.0;
+    return 1;
+  }
+  par[0] = ((x[0] - o0) >> 1) + Pc * (((x[1] - o1) >> 1) + Pc * ((x[2] - o2) >> 1));
+  par[1] = ((x[0] + o0) >> 1) + Pc * (((x[1] + o1) >> 1) + Pc * ((x[2] + o2) >> 1));
+  w[0] = w[1] = 0.5;
+  return 2;
+}
+
+// Children of coarse node I (lattice Pc) in the fine lattice Pf = 2 Pc - 1.
+__device__ __forceinline__ int children(int64_t I, int64_t Pc, int64_t Pf, int64_t ch[15], double w[15]) {
+  int64_t X[3];
+  lattice(I, Pc, X);
+  int n = 0;
+  ch[n] = 2 * X[0] + Pf * (2 * X[1] + Pf * 2 * X[2]);
+  w[n++] = 1.0;
+  for (int s = 1; s < 8; ++s)
+    for (int sg = -1; sg <= 1; sg += 2) {
+      const int64_t f0 = 2 * X[0] + sg * (s & 1), f1 = 2 * X[1] + sg * ((s >> 1) & 1),
+                    f2 = 2 * X[2] + sg * ((s >> 2) & 1);
+      if (f0 < 0 || f1 < 0 || f2 < 0 || f0 >= Pf || f1 >= Pf || f2 >= Pf) continue;
+      ch[n] = f0 + Pf * (f1 + Pf * f2);
+      w[n++] = 0.5;
+    }
+  return n;
+}
+
+__device__ __forceinline__ int find_slot(const int32_t* __restrict__ idx, int deg, int32_t J) {
+  int lo = 0, hi = deg;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (idx[mid] < J) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < deg && idx[lo] == J) ? lo : -1;
+}
+
+// Damped block-Jacobi sweep xout = xin + omega Binv (b - A xin), G lanes per node
+// (first: xin = 0, no product).  RESID: r = b - A x instead.
+template <int G, int MODE>  // MODE 0: sweep, 1: first sweep, 2: residual
+__global__ __launch_bounds__(256) void k_mg_sweep(int64_t nn, const int64_t* __restrict__ nptr,
+                                                  const int32_t* __restrict__ nidx,
+                                                  const double* __restrict__ vals, const double* __restrict__ binv,
+                                                  const double* __restrict__ b, const double* __restrict__ xin,
+                                                  double* __restrict__ out, double omega) {
+  const int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G;
+  const int lane = threadIdx.x % G;
+  double s[3] = {0.0, 0.0, 0.0};
+  if (i < nn && MODE != 1) {
+    const int64_t k0 = nptr[i];
+    const int L = 3 * (int)(nptr[i + 1] - k0);
+    const double* v = vals + 9 * k0;
+    const int32_t* ni = nidx + k0;
+    for (int q = lane; q < L; q += G) {
+      const int k = (q * 0xAAAB) >> 17;  // q / 3 for q < 2^15
+      const double xv = xin[3 * (int64_t)ni[k] + (q - 3 * k)];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) s[d] += v[d * L + q] * xv;
+    }
+  }
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) s[d] += __shfl_xor(s[d], o, G);
+  if (i >= nn || lane != 0) return;
+  double rv[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) rv[d] = b[3 * i + d] - s[d];
+  if (MODE == 2) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) out[3 * i + d] = rv[d];
+    return;
+  }
+  const double* B = binv + 9 * i;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double z = B[3 * d] * rv[0] + B[3 * d + 1] * rv[1] + B[3 * d + 2] * rv[2];
+    out[3 * i + d] = (MODE == 1 ? 0.0 : xin[3 * i + d]) + omega * z;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_mg_restrict(int64_t Pf, int64_t Pc, const double* __restrict__ rf,
+                                                     const uint8_t* __restrict__ freef,
+                                                     const uint8_t* __restrict__ freec, double* __restrict__ bc) {
+  const int64_t I = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (I >= Pc * Pc * Pc) return;
+  int64_t ch[15];
+  double w[15];
+  const int n = children(I, Pc, Pf, ch, w);
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < n; ++k)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) acc[d] += freef[3 * ch[k] + d] ? w[k] * rf[3 * ch[k] + d] : 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) bc[3 * I + d] = freec[3 * I + d] ? acc[d] : 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_mg_prolong(int64_t Pf, int64_t Pc, const double* __restrict__ xc,
+                                                    const uint8_t* __restrict__ freef,
+                                                    const uint8_t* __restrict__ freec, double* __restrict__ xf) {
+  const int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (f >= Pf * Pf * Pf) return;
+  int64_t par[2];
+  double w[2];
+  const int n = parents(f, Pf, Pc, par, w);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (!freef[3 * f + d]) continue;
+    double v = 0.0;
+    for (int k = 0; k < n; ++k) v += freec[3 * par[k] + d] ? w[k] * xc[3 * par[k] + d] : 0.0;
+    xf[3 * f + d] += v;
+  }
+}
+
+// Galerkin product on node blocks: one workgroup (64 threads) per coarse node row,
+// accumulated in LDS, then written in the node-block layout; constrained coarse dofs
+// get identity rows (their rows and columns are masked to zero).
+__global__ __launch_bounds__(64) void k_mg_galerkin(int64_t Pf, int64_t Pc, const int64_t* __restrict__ fptr,
+                                                    const int32_t* __restrict__ fidx,
+                                                    const double* __restrict__ fval,
+                                                    const uint8_t* __restrict__ ffree,
+                                                    const int64_t* __restrict__ cptr,
+                                                    const int32_t* __restrict__ cidx, double* __restrict__ cval,
+                                                    const uint8_t* __restrict__ cfree) {
+  extern __shared__ double acc[];
+  const int64_t nc = Pc * Pc * Pc;
+  for (int64_t I = blockIdx.x; I < nc; I += gridDim.x) {
+    const int64_t c0 = cptr[I];
+    const int degc = (int)(cptr[I + 1] - c0);
+    const int32_t* crow = cidx + c0;
+    for (int k = threadIdx.x; k < 9 * degc; k += blockDim.x) acc[k] = 0.0;
+    __syncthreads();
+    int64_t ch[15];
+    double wc[15];
+    const int nch = children(I, Pc, Pf, ch, wc);
+    uint8_t cf[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) cf[d] = cfree[3 * I + d];
+    for (int a = 0; a < nch; ++a) {
+      const int64_t f = ch[a];
+      const int64_t f0 = fptr[f];
+      const int degf = (int)(fptr[f + 1] - f0);
+      uint8_t fr[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) fr[d] = ffree[3 * f + d] & cf[d];
+      const double* fv = fval + 9 * f0;
+      for (int k = threadIdx.x; k < degf; k += blockDim.x) {
+        const int32_t j = fidx[f0 + k];
+        int64_t par[2];
+        double wj[2];
+        const int np = parents(j, Pf, Pc, par, wj);
+        double blk[3][3];
+#pragma unroll
+        for (int d1 = 0; d1 < 3; ++d1)
+#pragma unroll
+          for (int d2 = 0; d2 < 3; ++d2)
+            blk[d1][d2] = (fr[d1] && ffree[3 * (int64_t)j + d2]) ? wc[a] * fv[3 * degf * d1 + 3 * k + d2] : 0.0;
+        for (int p = 0; p < np; ++p) {
+          const int slot = find_slot(crow, degc, (int32_t)par[p]);
+          if (slot < 0) continue;  // cannot happen: the coarse graph holds every coupling
+#pragma unroll
+          for (int d1 = 0; d1 < 3; ++d1)
+#pragma unroll
+            for (int d2 = 0; d2 < 3; ++d2) {
+              const double v = cfree[3 * par[p] + d2] ? wj[p] * blk[d1][d2] : 0.0;
+              if (v != 0.0) atomicAdd(acc + 3 * degc * d1 + 3 * slot + d2, v);
+            }
+        }
+      }
+      __syncthreads();
+    }
+    const int self = find_slot(crow, degc, (int32_t)I);
+    for (int k = threadIdx.x; k < 9 * degc; k += blockDim.x) {
+      const int d1 = k / (3 * degc), q = k - 3 * degc * d1, slot = q / 3, d2 = q - 3 * slot;
+      double v = acc[k];
+      if (!cf[d1] && slot == self && d2 == d1) v = 1.0;
+      cval[9 * c0 + k] = v;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_mg_block_diag(int64_t nn, const int64_t* __restrict__ nptr,
+                                                       const int32_t* __restrict__ nidx,
+                                                       const double* __restrict__ vals, double* __restrict__ binv) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  const int64_t k0 = nptr[i];
+  const int deg = (int)(nptr[i + 1] - k0);
+  const int self = find_slot(nidx + k0, deg, (int32_t)i);
+  double a[3][3];
+#pragma unroll
+  for (int d1 = 0; d1 < 3; ++d1)
+#pragma unroll
+    for (int d2 = 0; d2 < 3; ++d2) a[d1][d2] = self >= 0 ? vals[9 * k0 + 3 * deg * d1 + 3 * self + d2] : (d1 == d2);
+  const double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) -
+                     a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+                     a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+  const double id = det != 0.0 ? 1.0 / det : 0.0;
+  double* o = binv + 9 * i;
+  o[0] = (a[1][1] * a[2][2] - a[1][2] * a[2][1]) * id;
+  o[1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) * id;
+  o[2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) * id;
+  o[3] = (a[1][2] * a[2][0] - a[1][0] * a[2][2]) * id;
+  o[4] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) * id;
+  o[5] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) * id;
+  o[6] = (a[1][0] * a[2][1] - a[1][1] * a[2][0]) * id;
+  o[7] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) * id;
+  o[8] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) * id;
+}
+
+// Dense copy of a small node-block matrix (n = 3 nn), row-major.
+__global__ __launch_bounds__(256) void k_mg_densify(int64_t nn, const int64_t* __restrict__ nptr,
+                                                    const int32_t* __restrict__ nidx,
+                                                    const double* __restrict__ vals, double* __restrict__ D) {
+  const int64_t I = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (I >= nn) return;
+  const int64_t n = 3 * nn, k0 = nptr[I];
+  const int deg = (int)(nptr[I + 1] - k0);
+  for (int d1 = 0; d1 < 3; ++d1)
+    for (int s = 0; s < deg; ++s) {
+      const int64_t J = nidx[k0 + s];
+      for (int d2 = 0; d2 < 3; ++d2) D[(3 * I + d1) * n + 3 * J + d2] = vals[9 * k0 + 3 * deg * d1 + 3 * s + d2];
+    }
+}
+
+// One Gauss-Jordan pivot step of the inverse of an SPD n x n matrix, from the old
+// matrix A into B (pivot k, no pivoting): after n steps the result is A^-1.
+__global__ __launch_bounds__(256) void k_mg_gj_step(int n, int k, const double* __restrict__ A,
+                                                    double* __restrict__ B) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= (int64_t)n * n) return;
+  const int i = (int)(e / n), j = (int)(e - (int64_t)i * n);
+  const double ip = 1.0 / A[(int64_t)k * n + k];
+  double v;
+  if (i == k && j == k)
+    v = ip;
+  else if (i == k)
+    v = A[e] * ip;
+  else if (j == k)
+    v = -A[e] * ip;
+  else
+    v = A[e] - A[(int64_t)i * n + k] * A[(int64_t)k * n + j] * ip;
+  B[e] = v;
+}
+
+__global__ __launch_bounds__(256) void k_mg_fill_zero(int64_t n, double* __restrict__ y) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) y[i] = 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_mg_dense_mv(int64_t n, const double* __restrict__ A,
+                                                     const double* __restrict__ x, double* __restrict__ y) {
+  const int64_t i = blockIdx.x * (int64_t)(blockDim.x / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  double s = 0.0;
+  if (i < n)
+    for (int64_t j = lane; j < n; j += 64) s += A[i * n + j] * x[j];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (i < n && lane == 0) y[i] = s;
+}
+
+// ------------------------------------------------------------------ host side
+// True when the primary mesh is the P2 structured lattice of fedd_mesh_structured
+// (node id = r + s P + t P^2, vertices on even, edge nodes on midpoint lattice
+// positions, every edge a Kuhn edge e_S).
+static bool lattice_mesh(const Mesh& m, int64_t& P) {
+  if (m.npe != 10 || m.npts < 27) return false;
+  P = (int64_t)std::llround(std::cbrt((double)m.npts));
+  if (P * P * P != m.npts || (P & 1) == 0) return false;
+  const int ea[6] = {0, 1, 0, 0, 1, 2}, eb[6] = {1, 2, 2, 3, 3, 3};
+  auto xyz = [&](int64_t i, int64_t x[3]) {
+    x[0] = i % P;
+    x[1] = (i / P) % P;
+    x[2] = i / (P * P);
+  };
+  for (int64_t e = 0; e < m.nelem; ++e) {
+    const int32_t* nd = &m.elems[e * 10];
+    int64_t v[4][3];
+    for (int k = 0; k < 4; ++k) {
+      xyz(nd[k], v[k]);
+      if ((v[k][0] | v[k][1] | v[k][2]) & 1) return false;
+    }
+    for (int t = 0; t < 6; ++t) {
+      int64_t x[3];
+      xyz(nd[4 + t], x);
+      for (int d = 0; d < 3; ++d) {
+        const int64_t dv = v[eb[t]][d] - v[ea[t]][d];
+        if (2 * x[d] != v[ea[t]][d] + v[eb[t]][d] || (dv != 0 && dv != 2 && dv != -2)) return false;
+      }
+    }
+  }
+  return true;
+}
+
+int mg_setup(Ctx* c) {
+  if (c->mg) return FEDD_OK;
+  const Mesh& m = c->mesh[0];
+  int64_t P0 = 0;
+  if (!lattice_mesh(m, P0))
+    return fail(c, FEDD_ERR_STATE, "multigrid needs the structured P2 lattice mesh of fedd_mesh_structured");
+  MgHost* g = new MgHost();
+  c->mg = g;
+  // level 0: the context's operator
+  {
+    MgLevel L0;
+    L0.P = P0;
+    L0.nn = P0 * P0 * P0;
+    L0.h_ptr.resize(L0.nn + 1);
+    FEDD_HIP(c, hipMemcpy(L0.h_ptr.data(), c->d_nptr.p, (L0.nn + 1) * 8, hipMemcpyDeviceToHost));
+    L0.nblk = L0.h_ptr[L0.nn];
+    L0.h_idx.resize(L0.nblk);
+    FEDD_HIP(c, hipMemcpy(L0.h_idx.data(), c->d_nidx.p, L0.nblk * 4, hipMemcpyDeviceToHost));
+    g->lv.push_back(std::move(L0));
+  }
+  while (g->lv.back().P >= 5 && ((g->lv.back().P - 1) & 1) == 0) {
+    const MgLevel& F = g->lv.back();
+    MgLevel C;
+    C.P = (F.P + 1) / 2;
+    C.nn = C.P * C.P * C.P;
+    // coarse adjacency: parents(i) x parents(j) over every fine coupling (i, j)
+    std::vector<std::vector<int32_t>> rows(C.nn);
+    auto par = [&](int64_t i, int64_t out[2]) -> int {
+      const int64_t x0 = i % F.P, x1 = (i / F.P) % F.P, x2 = i / (F.P * F.P);
+      const int64_t o0 = x0 & 1, o1 = x1 & 1, o2 = x2 & 1;
+      if (!(o0 | o1 | o2)) {
+        out[0] = (x0 >> 1) + C.P * ((x1 >> 1) + C.P * (x2 >> 1));
+        return 1;
+      }
+      out[0] = ((x0 - o0) >> 1) + C.P * (((x1 - o1) >> 1) + C.P * ((x2 - o2) >> 1));
+      out[1] = ((x0 + o0) >> 1) + C.P * (((x1 + o1) >> 1) + C.P * ((x2 + o2) >> 1));
+      return 2;
+    };
+    for (int64_t i = 0; i < F.nn; ++i) {
+      int64_t pi[2];
+      const int ni = par(i, pi);
+      for (int64_t q = F.h_ptr[i]; q < F.h_ptr[i + 1]; ++q) {
+        int64_t pj[2];
+        const int nj = par(F.h_idx[q], pj);
+        for (int a = 0; a < ni; ++a)
+          for (int b = 0; b < nj; ++b) rows[pi[a]].push_back((int32_t)pj[b]);
+      }
+    }
+    C.h_ptr.assign(C.nn + 1, 0);
+    for (int64_t I = 0; I < C.nn; ++I) {
+      auto& r = rows[I];
+      std::sort(r.begin(), r.end());
+      r.erase(std::unique(r.begin(), r.end()), r.end());
+      C.h_ptr[I + 1] = C.h_ptr[I] + (int64_t)r.size();
+      C.maxdeg = std::max<int64_t>(C.maxdeg, (int64_t)r.size());
+    }
+    C.nblk = C.h_ptr[C.nn];
+    C.h_idx.reserve(C.nblk);
+    for (auto& r : rows) C.h_idx.insert(C.h_idx.end(), r.begin(), r.end());
+    g->lv.push_back(std::move(C));
+  }
+  // device arrays and Dirichlet masks
+  std::vector<uint8_t> fr(3 * g->lv[0].nn, 1);
+  for (size_t k = 0; k < c->bcflag_host.size() && k < fr.size(); ++k) fr[k] = c->bcflag_host[k] ? 0 : 1;
+  for (size_t l = 0; l < g->lv.size(); ++l) {
+    MgLevel& L = g->lv[l];
+    int rc;
+    if (l == 0) {
+      L.d_nptr = c->d_nptr.as<int64_t>();
+      L.d_nidx = c->d_nidx.as<int32_t>();
+      L.d_vals = c->blk[0].d_vals.as<double>();
+    } else {
+      if ((rc = upload(c, L.nptr, L.h_ptr.data(), L.h_ptr.size() * 8))) return rc;
+      if ((rc = upload(c, L.nidx, L.h_idx.data(), L.h_idx.size() * 4))) return rc;
+      if ((rc = alloc(c, L.vals, L.nblk * 9 * 8))) return rc;
+      L.d_nptr = L.nptr.as<int64_t>();
+      L.d_nidx = L.nidx.as<int32_t>();
+      L.d_vals = L.vals.as<double>();
+      // coarse dof free iff the fine dof at the same lattice point is free
+      const MgLevel& F = g->lv[l - 1];
+      std::vector<uint8_t> cf(3 * L.nn);
+      for (int64_t I = 0; I < L.nn; ++I) {
+        const int64_t x0 = I % L.P, x1 = (I / L.P) % L.P, x2 = I / (L.P * L.P);
+        const int64_t f = 2 * x0 + F.P * (2 * x1 + F.P * 2 * x2);
+        for (int d = 0; d < 3; ++d) cf[3 * I + d] = fr[3 * f + d];
+      }
+      fr.swap(cf);
+    }
+    if ((rc = upload(c, L.free, fr.data(), fr.size()))) return rc;
+    for (DevBuf* b : {&L.x, &L.x2, &L.b, &L.r})
+      if ((rc = alloc(c, *b, L.nn * 3 * 8))) return rc;
+    if ((rc = alloc(c, L.binv, L.nn * 9 * 8))) return rc;
+    if (l > 0) {  // host copies only needed to build the next level
+      g->lv[l - 1].h_ptr.clear();
+      g->lv[l - 1].h_ptr.shrink_to_fit();
+      g->lv[l - 1].h_idx.clear();
+      g->lv[l - 1].h_idx.shrink_to_fit();
+    }
+  }
+  g->ncoarse = 3 * g->lv.back().nn;
+  int rc;
+  if ((rc = alloc(c, g->inv, g->ncoarse * g->ncoarse * 8))) return rc;
+  if ((rc = alloc(c, g->dense, g->ncoarse * g->ncoarse * 8))) return rc;
+  return FEDD_OK;
+}
+
+// Coarse operators, smoother blocks and the coarsest inverse for the current A.
+static int refresh_eager(Ctx* c) {
+  MgHost* g = c->mg;
+  if (!g) return fail(c, FEDD_ERR_STATE, "multigrid not set up");
+  for (size_t l = 0; l < g->lv.size(); ++l) {
+    MgLevel& L = g->lv[l];
+    if (l > 0) {
+      const MgLevel& F = g->lv[l - 1];
+      const unsigned grid = (unsigned)std::min<int64_t>(L.nn, (int64_t)c->num_cu * 16);
+      ktimer_begin(c);
+      k_mg_galerkin<<<grid, 64, (size_t)9 * L.maxdeg * 8, c->stream>>>(
+          F.P, L.P, F.d_nptr, F.d_nidx, F.d_vals, F.free.as<uint8_t>(), L.d_nptr, L.d_nidx, L.d_vals,
+          L.free.as<uint8_t>());
+      FEDD_HIP(c, hipGetLastError());
+      ktimer_end(c, l == 1 ? "mg_galerkin_1" : "mg_galerkin_coarse");
+    }
+    k_mg_block_diag<<<nbk(L.nn, 256), 256, 0, c->stream>>>(L.nn, L.d_nptr, L.d_nidx, L.d_vals,
+                                                           L.binv.as<double>());
+    FEDD_HIP(c, hipGetLastError());
+  }
+  const MgLevel& Cl = g->lv.back();
+  const int64_t n = g->ncoarse;
+  // ping-pong so that the last of the n steps lands in inv
+  double* bufs[2] = {g->inv.as<double>(), g->dense.as<double>()};
+  double* M0 = bufs[n % 2];
+  ktimer_begin(c);
+  k_mg_fill_zero<<<nbk(n * n, 256), 256, 0, c->stream>>>(n * n, M0);
+  k_mg_densify<<<nbk(Cl.nn, 256), 256, 0, c->stream>>>(Cl.nn, Cl.d_nptr, Cl.d_nidx, Cl.d_vals, M0);
+  for (int k = 0; k < (int)n; ++k)
+    k_mg_gj_step<<<nbk(n * n, 256), 256, 0, c->stream>>>((int)n, k, bufs[(n - k) % 2], bufs[(n - k + 1) % 2]);
+  FEDD_HIP(c, hipGetLastError());
+  ktimer_end(c, "mg_coarsest_inverse");
+  return FEDD_OK;
+}
+
+static void vcycle(Ctx* c, size_t l, const double* b, double* x) {
+  MgHost* g = c->mg;
+  MgLevel& L = g->lv[l];
+  if (l + 1 == g->lv.size()) {
+    k_mg_dense_mv<<<nbk(g->ncoarse, 4), 256, 0, c->stream>>>(g->ncoarse, g->inv.as<double>(), b, x);
+    return;
+  }
+  const double w = FEDD_MG_OMEGA;
+  const unsigned gs = l == 0 ? nbk(L.nn * 32, 256) : nbk(L.nn * 16, 256);
+  auto sweep = [&](int mode, const double* xin, double* out) {
+    if (l == 0) {
+      if (mode == 0)
+        k_mg_sweep<32, 0><<<gs, 256, 0, c->stream>>>(L.nn, L.d_nptr, L.d_nidx, L.d_vals, L.binv.as<double>(), b,
+                                                      xin, out, w);
+      else if (mode == 1)
+        k_mg_sweep<32, 1><<<gs, 256, 0, c->stream>>>(L.nn, L.d_nptr, L.d_nidx, L.d_vals, L.binv.as<double>(), b,
+                                                      xin, out, w);
+      else
+        k_mg_sweep<32, 2><<<gs, 256, 0, c->stream>>>(L.nn, L.d_nptr, L.d_nidx, L.d_vals, L.binv.as<double>(), b,
+                                                      xin, out, w);
+    } else {
+      if (mode == 0)
+        k_mg_sweep<16, 0><<<gs, 256, 0, c->stream>>>(L.nn, L.d_nptr, L.d_nidx, L.d_vals, L.binv.as<double>(), b,
+                                                      xin, out, w);
+      else if (mode == 1)
+        k_mg_sweep<16, 1><<<gs, 256, 0, c->stream>>>(L.nn, L.d_nptr, L.d_nidx, L.d_vals, L.binv.as<double>(), b,
+                                                      xin, out, w);
+      else
+        k_mg_sweep<16, 2><<<gs, 256, 0, c->stream>>>(L.nn, L.d_nptr, L.d_nidx, L.d_vals, L.binv.as<double>(), b,
+                                                      xin, out, w);
+    }
+  };
+  // The sweeps alternate between x and the level's spare buffer.  Starting the
+  // pre-smoothing in the spare buffer makes the nu post-sweeps end in x for any nu.
+  double* xa = x;
+  double* xb = L.x2.as<double>();
+  sweep(1, nullptr, xb);  // from x = 0
+  double* cur = xb;
+  for (int s = 1; s < FEDD_MG_NU; ++s) {
+    double* nxt = cur == xa ? xb : xa;
+    sweep(0, cur, nxt);
+    cur = nxt;
+  }
+  // coarse correction
+  sweep(2, cur, L.r.as<double>());
+  MgLevel& C = g->lv[l + 1];
+  k_mg_restrict<<<nbk(C.nn, 256), 256, 0, c->stream>>>(L.P, C.P, L.r.as<double>(), L.free.as<uint8_t>(),
+                                                       C.free.as<uint8_t>(), C.b.as<double>());
+  vcycle(c, l + 1, C.b.as<double>(), C.x.as<double>());
+  k_mg_prolong<<<nbk(L.nn, 256), 256, 0, c->stream>>>(L.P, C.P, C.x.as<double>(), L.free.as<uint8_t>(),
+                                                      C.free.as<uint8_t>(), cur);
+  for (int s = 0; s < FEDD_MG_NU; ++s) {
+    double* nxt = cur == xa ? xb : xa;
+    sweep(0, cur, nxt);
+    cur = nxt;
+  }
+}
+
+// Records what `body` enqueues on the context stream into *exec (the buffers it
+// touches keep their addresses while the hierarchy lives).
+template <class F>
+static int capture(Ctx* c, hipGraphExec_t* exec, F body) {
+  FEDD_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+  c->capturing = true;
+  const int rc = body();
+  c->capturing = false;
+  hipGraph_t gr = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &gr);
+  if (rc) {
+    if (gr) (void)hipGraphDestroy(gr);
+    return rc;
+  }
+  if (e != hipSuccess || !gr) return fail(c, FEDD_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  const hipError_t ei = hipGraphInstantiate(exec, gr, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(gr);
+  if (ei != hipSuccess) return fail(c, FEDD_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+  return FEDD_OK;
+}
+
+// Eager launches when kernel timing is on (per-kernel events) or a caller's own
+// capture is open; otherwise one replayed graph per call.
+int mg_refresh(Ctx* c) {
+  MgHost* g = c->mg;
+  if (!g) return fail(c, FEDD_ERR_STATE, "multigrid not set up");
+  if (c->capturing || c->timing) return refresh_eager(c);
+  if (!g->rexec) {
+    int rc = capture(c, &g->rexec, [&]() -> int { return refresh_eager(c); });
+    if (rc) return rc;
+  }
+  FEDD_HIP(c, hipGraphLaunch(g->rexec, c->stream));
+  return FEDD_OK;
+}
+
+// x = V-cycle(b) on the finest level (3 dofs per node).
+int mg_vcycle(Ctx* c, const double* b, double* x) {
+  MgHost* g = c->mg;
+  if (!g) return fail(c, FEDD_ERR_STATE, "multigrid not set up");
+  if (c->capturing || c->timing) {
+    ktimer_begin(c);
+    vcycle(c, 0, b, x);
+    ktimer_end(c, "mg_vcycle");
+    FEDD_HIP(c, hipGetLastError());
+    return FEDD_OK;
+  }
+  if (!g->vexec || g->vb != b || g->vx != x) {
+    if (g->vexec) (void)hipGraphExecDestroy(g->vexec);
+    g->vexec = nullptr;
+    int rc = capture(c, &g->vexec, [&]() -> int {
+      vcycle(c, 0, b, x);
+      FEDD_HIP(c, hipGetLastError());
+      return FEDD_OK;
+    });
+    if (rc) return rc;
+    g->vb = b;
+    g->vx = x;
+  }
+  FEDD_HIP(c, hipGraphLaunch(g->vexec, c->stream));
+  return FEDD_OK;
+}
+
+}  // namespace fedd

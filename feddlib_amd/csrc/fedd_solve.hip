@@ -202,6 +202,34 @@ __global__ __launch_bounds__(256) void k_pcg_p(int64_t n, double* __restrict__ p
   p[i] = first ? z[i] : z[i] + (sum16(scal + PCG_RZ(t)) / sum16(scal + PCG_RZ(t + 2))) * p[i];
 }
 
+// x += alpha p; r -= alpha Ap; r.r added to RR(t) (multigrid-preconditioned CG).
+__global__ __launch_bounds__(256) void k_pcg_xr(int64_t n, double* __restrict__ x, double* __restrict__ r,
+                                                const double* __restrict__ p, const double* __restrict__ Ap,
+                                                double* __restrict__ scal, int t) {
+  const double alpha = sum16(scal + PCG_RZ(t + 2)) / sum16(scal + PCG_PAP(t));
+  double rr = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    x[i] += alpha * p[i];
+    const double v = r[i] - alpha * Ap[i];
+    r[i] = v;
+    rr += v * v;
+  }
+  block_atomic_sum(rr, scal + PCG_RR(t));
+}
+
+// r.z added to RZ(ts); with_rr: r.r added to RR(ts) as well.
+__global__ __launch_bounds__(256) void k_pcg_dot(int64_t n, const double* __restrict__ r,
+                                                 const double* __restrict__ z, double* __restrict__ scal, int ts,
+                                                 int with_rr) {
+  double rz = 0.0, rr = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    rz += r[i] * z[i];
+    rr += r[i] * r[i];
+  }
+  block_atomic_sum(rz, scal + PCG_RZ(ts));
+  if (with_rr) block_atomic_sum(rr, scal + PCG_RR(ts));
+}
+
 // Inverses of the 3x3 diagonal node blocks (block Jacobi).
 __global__ __launch_bounds__(256) void k_block_diag(int64_t nnode, const int64_t* __restrict__ rowptr,
                                                     const int32_t* __restrict__ colind,
@@ -351,6 +379,77 @@ int pcg_solve(Ctx* c, double rtol, double atol, int maxit, int* iters, double* r
   return FEDD_OK;
 }
 
+// CG preconditioned with one geometric-multigrid V-cycle per iteration (structured
+// lattice meshes); same scalar slots and convergence test as pcg_solve.
+int pcg_mg_solve(Ctx* c, double rtol, double atol, int maxit, int* iters, double* relres, bool refresh) {
+  const Block& A = c->blk[0];
+  const int64_t n = A.nrows, nnode = n / 3;
+  int rc;
+  if ((rc = mg_setup(c))) return rc;
+  if (refresh && (rc = mg_refresh(c))) return rc;
+  for (int k = 0; k < 4; ++k)
+    if ((rc = ensure(c, c->d_kw[k], n * 8))) return rc;
+  if ((rc = ensure(c, c->d_scal, PCG_SCAL * 8))) return rc;
+  if ((rc = ensure_vec(c, FEDD_VEC_X, n))) return rc;
+  double* r = c->d_kw[0].as<double>();
+  double* z = c->d_kw[1].as<double>();
+  double* p = c->d_kw[2].as<double>();
+  double* Ap = c->d_kw[3].as<double>();
+  double* x = c->vec[FEDD_VEC_X].as<double>();
+  double* scal = c->d_scal.as<double>();
+  const double* va = A.d_vals.as<double>();
+  const int64_t* nptr = c->d_nptr.as<int64_t>();
+  const int32_t* nidx = c->d_nidx.as<int32_t>();
+  const unsigned gmax = (unsigned)c->num_cu * 8;
+  const unsigned g_spmv = std::min(nb(nnode * FEDD_SPMV_LANES, 256), gmax);
+  const unsigned g_dofs = nb(n, 256), g_red = std::min(nb(n, 256), gmax);
+  FEDD_HIP(c, hipMemcpyAsync(r, c->vec[FEDD_VEC_B].p, n * 8, hipMemcpyDeviceToDevice, c->stream));
+  k_fill<<<g_dofs, 256, 0, c->stream>>>(n, x, 0.0);
+  k_fill<<<1, 256, 0, c->stream>>>(PCG_SCAL, scal, 0.0);
+  if ((rc = mg_vcycle(c, r, z))) return rc;
+  k_pcg_dot<<<g_red, 256, 0, c->stream>>>(n, r, z, scal, 2, 1);
+  k_pcg_p<<<g_dofs, 256, 0, c->stream>>>(n, p, z, scal, -1, 1);
+  FEDD_HIP(c, hipGetLastError());
+  double h[16];
+  auto read_rr = [&](int t) -> int {
+    FEDD_HIP(c, hipMemcpyAsync(h, scal + PCG_RR(t), 16 * 8, hipMemcpyDeviceToHost, c->stream));
+    FEDD_HIP(c, hipStreamSynchronize(c->stream));
+    return FEDD_OK;
+  };
+  auto sum = [&]() {
+    double v = 0.0;
+    for (int k = 0; k < 16; ++k) v += h[k];
+    return v;
+  };
+  if ((rc = read_rr(2))) return rc;
+  const double bnorm = std::sqrt(sum());
+  double rnorm = bnorm;
+  int it = 0;
+  const double target = std::max(rtol * bnorm, atol);
+  if (bnorm > 0.0 && rnorm > target) {
+    while (it < maxit) {
+      const int t = it;
+      k_bspmv_dot<FEDD_SPMV_LANES><<<g_spmv, 256, 0, c->stream>>>(nnode, nptr, nidx, va, p, Ap, scal + PCG_PAP(t));
+      k_pcg_xr<<<g_red, 256, 0, c->stream>>>(n, x, r, p, Ap, scal, t);
+      if ((rc = mg_vcycle(c, r, z))) return rc;
+      k_pcg_dot<<<g_red, 256, 0, c->stream>>>(n, r, z, scal, t, 0);
+      k_pcg_p<<<g_dofs, 256, 0, c->stream>>>(n, p, z, scal, t, 0);
+      ++it;
+      if ((it & 1) == 0 || it == maxit) {
+        FEDD_HIP(c, hipGetLastError());
+        if ((rc = read_rr(t))) return rc;
+        rnorm = std::sqrt(sum());
+        if (!(rnorm == rnorm)) break;  // NaN: let the caller reject the step
+        if (rnorm <= target) break;
+      }
+    }
+  }
+  FEDD_HIP(c, hipGetLastError());
+  if (iters) *iters = it;
+  if (relres) *relres = bnorm > 0 ? rnorm / bnorm : 0.0;
+  return FEDD_OK;
+}
+
 int newton_residual(Ctx* c, double lam, double* norm) {
   const int64_t n = 3 * c->mesh[0].npts;
   const int64_t nblk = nb(n, 256);
@@ -429,6 +528,8 @@ int fedd_dirichlet_set(fedd_ctx* cc, const int32_t* dofs, int64_t n) {
   int rc = upload(c, c->d_bcflag, flag.data(), nd);
   if (rc) return rc;
   c->nbc = n;
+  c->bcflag_host = flag;
+  mg_free(c);  // the multigrid masks follow the Dirichlet set
   FEDD_HIP(c, hipStreamSynchronize(c->stream));
   return FEDD_OK;
 }
@@ -447,7 +548,15 @@ int fedd_linear_solve(fedd_ctx* cc, double rtol, int maxit, int* iters, double* 
   if (c->graph_kind != FEDD_GRAPH_VECTOR) return fail(c, FEDD_ERR_STATE, "needs a vector graph");
   if (c->vec_n[FEDD_VEC_B] != c->blk[0].nrows) return fail(c, FEDD_ERR_STATE, "right-hand side not set");
   (void)hipSetDevice(c->device);
+  if (c->precond == 1) return pcg_mg_solve(c, rtol, 0.0, maxit, iters, relres, true);
   return pcg_solve(c, rtol, 0.0, maxit, iters, relres);
+}
+
+int fedd_set_preconditioner(fedd_ctx* cc, int kind) {
+  Ctx* c = cc;
+  if (!c || kind < 0 || kind > 1) return FEDD_ERR_ARG;
+  c->precond = kind;
+  return FEDD_OK;
 }
 
 int fedd_apply_dirichlet(fedd_ctx* cc) {
@@ -463,7 +572,9 @@ int fedd_newton_elasticity(fedd_ctx* cc, const fedd_newton_params* P, fedd_newto
   Ctx* c = cc;
   if (!c || !P || !st) return FEDD_ERR_ARG;
   if (c->graph_kind != FEDD_GRAPH_VECTOR) return fail(c, FEDD_ERR_STATE, "needs a vector (P2) graph");
-  if (P->load_steps < 1 || P->max_newton < 1 || !(P->abs_tol > 0)) return fail(c, FEDD_ERR_ARG, "bad Newton parameters");
+  if (P->load_steps < 1 || P->max_newton < 1 || !(P->abs_tol > 0) || P->preconditioner < 0 ||
+      P->preconditioner > 1)
+    return fail(c, FEDD_ERR_ARG, "bad Newton parameters");
   const int model = P->neo_hooke_model == 1 ? 1 : 2;
   if ((P->tangent == 1 || P->collect_samples) && !c->mlp_ready)
     return fail(c, FEDD_ERR_STATE, "surrogate tangent or sample collection requested but no network");
@@ -538,7 +649,11 @@ int fedd_newton_elasticity(fedd_ctx* cc, const fedd_newton_params* P, fedd_newto
       int iters = 0;
       double relres = 0.0;
       FEDD_HIP(c, hipEventRecord(ea, c->stream));
-      if ((rc = pcg_solve(c, eta, 0.0, P->lin_maxit, &iters, &relres))) return rc;
+      if (P->preconditioner == 1)
+        rc = pcg_mg_solve(c, eta, 0.0, P->lin_maxit, &iters, &relres, true);
+      else
+        rc = pcg_solve(c, eta, 0.0, P->lin_maxit, &iters, &relres);
+      if (rc) return rc;
       FEDD_HIP(c, hipEventRecord(eb, c->stream));
       FEDD_HIP(c, hipEventSynchronize(eb));
       FEDD_HIP(c, hipEventElapsedTime(&ms, ea, eb));

@@ -36,7 +36,8 @@ class NewtonParams(ctypes.Structure):
     _fields_ = [("neo_hooke_model", ctypes.c_int), ("E", ctypes.c_double), ("nu", ctypes.c_double),
                 ("load_steps", ctypes.c_int), ("abs_tol", ctypes.c_double), ("max_newton", ctypes.c_int),
                 ("lin_rtol", ctypes.c_double), ("lin_maxit", ctypes.c_int), ("tangent", ctypes.c_int),
-                ("precision", ctypes.c_int), ("stall_ratio", ctypes.c_double), ("collect_samples", ctypes.c_int)]
+                ("precision", ctypes.c_int), ("stall_ratio", ctypes.c_double), ("collect_samples", ctypes.c_int),
+                ("preconditioner", ctypes.c_int)]
 
 
 class NewtonStats(ctypes.Structure):
@@ -91,6 +92,7 @@ _SIGS = {
     "fedd_dirichlet_set": (ctypes.c_int, [_vp, _ip, ctypes.c_int64]),
     "fedd_apply_dirichlet": (ctypes.c_int, [_vp]),
     "fedd_traction_load": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, _dp]),
+    "fedd_set_preconditioner": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fedd_linear_solve": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _ip, _dp]),
     "fedd_newton_elasticity": (ctypes.c_int, [_vp, ctypes.POINTER(NewtonParams), ctypes.POINTER(NewtonStats),
                                               _dp, ctypes.c_int]),
@@ -292,6 +294,9 @@ class Context:
         t = np.ascontiguousarray(t, np.float64)
         self._chk(self._L.fedd_traction_load(self.h, axis, value, _p(t, _dp)))
 
+    def set_preconditioner(self, kind):
+        self._chk(self._L.fedd_set_preconditioner(self.h, kind))
+
     def linear_solve(self, rtol, maxit):
         it, rr = ctypes.c_int(), ctypes.c_double()
         self._chk(self._L.fedd_linear_solve(self.h, rtol, maxit, ctypes.byref(it), ctypes.byref(rr)))
@@ -299,9 +304,9 @@ class Context:
 
     def newton_elasticity(self, model=1, E=1000.0, nu=0.3, load_steps=5, abs_tol=1e-8, max_newton=30,
                           lin_rtol=1e-10, lin_maxit=20000, tangent=0, precision=FP64, stall_ratio=0.5,
-                          history_cap=512, collect_samples=False):
+                          history_cap=512, collect_samples=False, preconditioner=0):
         P = NewtonParams(model, E, nu, load_steps, abs_tol, max_newton, lin_rtol, lin_maxit, tangent, precision,
-                         stall_ratio, 1 if collect_samples else 0)
+                         stall_ratio, 1 if collect_samples else 0, preconditioner)
         st = NewtonStats()
         hist = np.zeros(history_cap)
         self._chk(self._L.fedd_newton_elasticity(self.h, ctypes.byref(P), ctypes.byref(st), _p(hist, _dp),

@@ -178,8 +178,14 @@ int fedd_apply_dirichlet(fedd_ctx* ctx);
 /* Uniform traction t on the face {x_axis == value} of the P2 mesh into VEC_FEXT
  * (surface term of the load-stepped source, NonLinElasAssFE_def.hpp:262-325). */
 int fedd_traction_load(fedd_ctx* ctx, int axis, double value, const double* traction);
-/* Block-Jacobi preconditioned CG on block A: solves A x = VEC_B into VEC_X. */
+/* Preconditioned CG on block A: solves A x = VEC_B into VEC_X (Belos CG behind
+ * LinearSolver_def.hpp:23-137), preconditioner chosen by fedd_set_preconditioner. */
 int fedd_linear_solve(fedd_ctx* ctx, double rtol, int maxit, int* iters, double* relres);
+/* 0: 3x3 block Jacobi (any mesh); 1: geometric multigrid V-cycle on the structured
+ * lattice (P2 -> P1 -> coarser P1, Galerkin coarse operators, damped block-Jacobi
+ * smoothing, dense coarsest solve) standing in for the reference's FROSch/MueLu
+ * preconditioners (Preconditioner_def.hpp:206-291). */
+int fedd_set_preconditioner(fedd_ctx* ctx, int kind);
 
 typedef struct {
   int neo_hooke_model;  /* Parameter/"Neo-Hooke Modell": 1 -> skr2 law, otherwise skr3 */
@@ -196,6 +202,8 @@ typedef struct {
                            the step first when ||R_k+1|| > ||R_k|| or is not finite */
   int collect_samples;  /* nonzero: append the quadrature-point states of every Newton iterate
                            to the surrogate's training set (needs fedd_mlp_init) */
+  int preconditioner;   /* 0: 3x3 block Jacobi; 1: geometric multigrid V-cycle (structured
+                           lattice meshes of fedd_mesh_structured only) */
 } fedd_newton_params;
 
 typedef struct {
