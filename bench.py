@@ -69,6 +69,8 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--newton-M", type=int, default=24)
     ap.add_argument("--no-newton", action="store_true")
+    ap.add_argument("--newton-exact-only", action="store_true",
+                    help="run only the exact-tangent solves of the Newton leg (profiling runs)")
     ap.add_argument("--large-M", type=int, default=64,
                     help="cells per edge of the graph-replayed FP32-local Jacobian + residual leg (0: skip)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -211,7 +213,7 @@ def main():
         "cpu_baseline": cpu,
     }
     if not args.no_newton:
-        out["newton"] = newton_leg(ctx, N, args.newton_M)
+        out["newton"] = newton_leg(ctx, N, args.newton_M, exact_only=args.newton_exact_only)
     if args.large_M > 0:
         out["ns_large"] = large_leg(ctx, N, W, args.large_M)
     ctx.close()
@@ -294,7 +296,7 @@ def _nh_solve(ctx, N, prob, tangent, collect=False, precond=1):
     return st, ctx.vector_download(N.VEC_U, 3 * prob[0].shape[0])
 
 
-def newton_leg(ctx, N, M):
+def newton_leg(ctx, N, M, exact_only=False):
     """Time-to-tolerance of the load-stepped Newton solve, exact vs learned tangent."""
     prob = _nh_problem(N, M)
     nelem, nnode = prob[1].shape[0], prob[0].shape[0]
@@ -312,6 +314,9 @@ def newton_leg(ctx, N, M):
     st_bj, u_bj = _nh_solve(ctx, N, prob, 0, precond=0)
     print("# newton exact, block-Jacobi CG %.3fs (%d Krylov)" % (st_bj["seconds"], st_bj["krylov_iters"]),
           file=sys.stderr, flush=True)
+
+    if exact_only:
+        return {"exact": {k: st_e[k] for k in keys}, "exact_block_jacobi_cg": {k: st_bj[k] for k in keys}}
 
     # surrogate: 10^6 samples F = I + N(0, 0.1) plus the iterates of the same
     # problem solved on an 8^3 x 6 mesh, then Adam on the device
