@@ -2,6 +2,7 @@
 // construction, row-gather schedules, colouring and the C ABI entry points.
 #include <algorithm>
 #include <array>
+#include <cstdio>
 #include <cstring>
 #include <limits>
 #include <numeric>
@@ -634,6 +635,119 @@ int fedd_block_device_values(fedd_ctx* c, int block, double** d) {
   if (c->graph_kind == GK_NONE) return fail(c, FEDD_ERR_STATE, "no graph");
   *d = c->blk[block].d_vals.as<double>();
   return FEDD_OK;
+}
+
+// ------------------------------------------------------ export formats
+namespace {
+struct HostCrs {
+  int64_t nrows = 0, ncols = 0;
+  std::vector<int64_t> rp;
+  std::vector<int32_t> ci;
+  std::vector<double> v;
+};
+
+int host_block(fedd::Ctx* c, int block, HostCrs& h) {
+  const fedd::Block& b = c->blk[block];
+  h.nrows = b.nrows;
+  h.ncols = b.ncols;
+  h.rp.resize(b.nrows + 1);
+  h.ci.resize(b.nnz);
+  h.v.resize(b.nnz);
+  FEDD_HIP(c, hipMemcpyAsync(h.rp.data(), b.d_rowptr.p, (b.nrows + 1) * 8, hipMemcpyDeviceToHost, c->stream));
+  if (b.nnz) {
+    FEDD_HIP(c, hipMemcpyAsync(h.ci.data(), b.d_colind.p, b.nnz * 4, hipMemcpyDeviceToHost, c->stream));
+    FEDD_HIP(c, hipMemcpyAsync(h.v.data(), b.d_vals.p, b.nnz * 8, hipMemcpyDeviceToHost, c->stream));
+  }
+  FEDD_HIP(c, hipStreamSynchronize(c->stream));
+  return FEDD_OK;
+}
+
+// The 2x2 NS block matrix as one CRS matrix (BlockMatrix::merge): block (i, j) at row
+// offset 3 N2 i and column offset 3 N2 j; on one rank the merged rows are the block
+// rows side by side, already sorted.
+int merged_ns(fedd::Ctx* c, HostCrs& M) {
+  HostCrs b[4];
+  for (int k = 0; k < 4; ++k) {
+    int rc = host_block(c, k, b[k]);
+    if (rc) return rc;
+  }
+  const int64_t nv = b[0].nrows, np = b[2].nrows;
+  M.nrows = M.ncols = nv + np;
+  M.rp.assign(M.nrows + 1, 0);
+  M.ci.clear();
+  M.v.clear();
+  M.ci.reserve(b[0].ci.size() + b[1].ci.size() + b[2].ci.size() + b[3].ci.size());
+  M.v.reserve(M.ci.capacity());
+  auto append = [&](const HostCrs& B, int64_t r, int64_t coff) {
+    for (int64_t k = B.rp[r]; k < B.rp[r + 1]; ++k) {
+      M.ci.push_back((int32_t)(B.ci[k] + coff));
+      M.v.push_back(B.v[k]);
+    }
+  };
+  for (int64_t r = 0; r < nv; ++r) {
+    append(b[0], r, 0);
+    append(b[1], r, nv);
+    M.rp[r + 1] = (int64_t)M.ci.size();
+  }
+  for (int64_t r = 0; r < np; ++r) {
+    append(b[2], r, 0);
+    append(b[3], r, nv);
+    M.rp[nv + r + 1] = (int64_t)M.ci.size();
+  }
+  return FEDD_OK;
+}
+
+// MatrixMarket coordinate file (the layout of Tpetra::MatrixMarket::Writer::writeSparseFile
+// behind Matrix::writeMM, Matrix_def.hpp:264-277): 1-based indices, explicit zeros kept,
+// values with 17 significant digits (exact round trip).
+int write_mm(fedd::Ctx* c, const HostCrs& M, const char* path, const char* name) {
+  FILE* f = std::fopen(path, "w");
+  if (!f) return fedd::fail(c, FEDD_ERR_ARG, std::string("cannot open ") + path);
+  std::fprintf(f, "%%%%MatrixMarket matrix coordinate real general\n%% %s\n", name);
+  std::fprintf(f, "%lld %lld %lld\n", (long long)M.nrows, (long long)M.ncols, (long long)M.ci.size());
+  for (int64_t r = 0; r < M.nrows; ++r)
+    for (int64_t k = M.rp[r]; k < M.rp[r + 1]; ++k)
+      std::fprintf(f, "%lld %d %.17e\n", (long long)(r + 1), M.ci[k] + 1, M.v[k]);
+  const bool ok = std::fclose(f) == 0;
+  return ok ? FEDD_OK : fedd::fail(c, FEDD_ERR_ARG, std::string("write failed: ") + path);
+}
+}  // namespace
+
+int fedd_ns_merged_info(fedd_ctx* c, int64_t* nrows, int64_t* nnz) {
+  if (!c) return FEDD_ERR_ARG;
+  if (c->graph_kind != FEDD_GRAPH_NS) return fail(c, FEDD_ERR_STATE, "needs a Navier-Stokes graph");
+  if (nrows) *nrows = c->blk[0].nrows + c->blk[2].nrows;
+  if (nnz) *nnz = c->blk[0].nnz + c->blk[1].nnz + c->blk[2].nnz + c->blk[3].nnz;
+  return FEDD_OK;
+}
+
+int fedd_ns_merged_export(fedd_ctx* c, int64_t* rowptr, int32_t* colind, double* values) {
+  if (!c) return FEDD_ERR_ARG;
+  if (c->graph_kind != FEDD_GRAPH_NS) return fail(c, FEDD_ERR_STATE, "needs a Navier-Stokes graph");
+  (void)hipSetDevice(c->device);
+  HostCrs M;
+  int rc = merged_ns(c, M);
+  if (rc) return rc;
+  if (rowptr) std::copy(M.rp.begin(), M.rp.end(), rowptr);
+  if (colind) std::copy(M.ci.begin(), M.ci.end(), colind);
+  if (values) std::copy(M.v.begin(), M.v.end(), values);
+  return FEDD_OK;
+}
+
+int fedd_write_matrix_market(fedd_ctx* c, int block, const char* path) {
+  if (!c || !path || block < -1 || block > 3) return FEDD_ERR_ARG;
+  if (c->graph_kind == GK_NONE) return fail(c, FEDD_ERR_STATE, "no graph");
+  (void)hipSetDevice(c->device);
+  HostCrs M;
+  int rc;
+  if (block < 0) {
+    if (c->graph_kind != FEDD_GRAPH_NS) return fail(c, FEDD_ERR_STATE, "the merged export needs a Navier-Stokes graph");
+    rc = merged_ns(c, M);
+  } else {
+    rc = host_block(c, block, M);
+  }
+  if (rc) return rc;
+  return write_mm(c, M, path, "matrix");
 }
 
 int fedd_colour_build(fedd_ctx* c, int* ncolours) {

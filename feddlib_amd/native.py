@@ -74,6 +74,9 @@ _SIGS = {
     "fedd_block_graph": (ctypes.c_int, [_vp, ctypes.c_int, _lp, _ip]),
     "fedd_block_values": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "fedd_block_device_values": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_dp)]),
+    "fedd_ns_merged_info": (ctypes.c_int, [_vp, _lp, _lp]),
+    "fedd_ns_merged_export": (ctypes.c_int, [_vp, _lp, _ip, _dp]),
+    "fedd_write_matrix_market": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_char_p]),
     "fedd_colour_build": (ctypes.c_int, [_vp, _ip]),
     "fedd_colour_export": (ctypes.c_int, [_vp, _lp, _ip]),
     "fedd_vector_upload": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.c_int64]),
@@ -237,6 +240,18 @@ class Context:
         v = np.zeros(max(nnz, 1))
         self._chk(self._L.fedd_block_values(self.h, block, _p(v, _dp)))
         return v[:nnz]
+
+    def ns_merged(self):
+        nr, nz = ctypes.c_int64(), ctypes.c_int64()
+        self._chk(self._L.fedd_ns_merged_info(self.h, ctypes.byref(nr), ctypes.byref(nz)))
+        rp = np.zeros(nr.value + 1, np.int64)
+        ci = np.zeros(nz.value, np.int32)
+        v = np.zeros(nz.value)
+        self._chk(self._L.fedd_ns_merged_export(self.h, _p(rp, _lp), _p(ci, _ip), _p(v, _dp)))
+        return rp, ci, v
+
+    def write_matrix_market(self, block, path):
+        self._chk(self._L.fedd_write_matrix_market(self.h, block, str(path).encode()))
 
     def colour_build(self):
         n = ctypes.c_int()

@@ -126,6 +126,16 @@ int fedd_block_values(fedd_ctx* ctx, int block, double* values);
 /* Device pointer of the value array (owned by the context, valid until the graph is rebuilt). */
 int fedd_block_device_values(fedd_ctx* ctx, int block, double** d_values);
 
+/* --------------------------------------------------------- export formats */
+/* BlockMatrix::merge (BlockMatrix_def.hpp:119-163) of the NS blocks into one square CRS
+ * matrix: block (i, j) at row/column offsets (0 | 3 N2); rows sorted, explicit zeros kept. */
+int fedd_ns_merged_info(fedd_ctx* ctx, int64_t* nrows, int64_t* nnz);
+int fedd_ns_merged_export(fedd_ctx* ctx, int64_t* rowptr, int32_t* colind, double* values);
+/* Matrix::writeMM (Matrix_def.hpp:264-277): MatrixMarket coordinate real general file of
+ * one block (0..3) or, with block = -1, of the merged NS matrix; 1-based indices, values
+ * with 17 significant digits. */
+int fedd_write_matrix_market(fedd_ctx* ctx, int block, const char* path);
+
 /* ------------------------------------------------------------- colouring */
 /* Element colouring of the primary mesh: elements of one colour share no node.
  * No counterpart in the reference (it assembles serially per rank); used by the

@@ -148,3 +148,30 @@ def test_colouring_conflict_free(gpu_ctx, fe):
         assert len(np.unique(nodes)) == len(nodes)  # no two elements of a colour share a node
         col[members] = k
     assert np.array_equal(col, _greedy_colours(el))  # bit-exact vs the restated greedy rule
+
+
+def test_merged_matrix_and_matrix_market(gpu_ctx, tmp_path):
+    """BlockMatrix::merge and Matrix::writeMM: the merged NS matrix equals the 2x2 block
+    matrix of the oracle, and the MatrixMarket file reads back exactly."""
+    import scipy.io
+    import scipy.sparse as sp
+    p2, e2, p1, e1, u, p = _setup_ns(gpu_ctx, 2)
+    gpu_ctx.assembly_navier_stokes(NU, RHO, True, N.NS_JACOBIAN, reassemble=False)
+    ref = O.ns_assembly(p2, e2, e1, p1.shape[0], u, p, NU, RHO, True, 0, False)
+    nv, npr = 3 * p2.shape[0], p1.shape[0]
+    blk = {k: sp.csr_matrix((ref[k][2], ref[k][1], ref[k][0]), shape=s)
+           for k, s in (("A", (nv, nv)), ("BT", (nv, npr)), ("B", (npr, nv)))}
+    R = sp.bmat([[blk["A"], blk["BT"]], [blk["B"], None]], format="csr")
+    rp, ci, v = gpu_ctx.ns_merged()
+    M = sp.csr_matrix((v, ci, rp), shape=(nv + npr, nv + npr))
+    assert M.nnz == blk["A"].nnz + blk["BT"].nnz + blk["B"].nnz  # explicit zeros kept
+    assert np.all(np.diff(rp) >= 0) and all(np.all(np.diff(ci[rp[r]:rp[r + 1]]) > 0) for r in range(nv + npr))
+    assert abs(M - R).max() <= 1e-12 * abs(R).max()
+    path = tmp_path / "ns.mtx"
+    gpu_ctx.write_matrix_market(-1, path)
+    back = scipy.io.mmread(str(path)).tocsr()
+    assert back.shape == M.shape and (back != M).nnz == 0  # exact round trip
+    gpu_ctx.write_matrix_market(N.BLOCK_BT, tmp_path / "bt.mtx")
+    bt = scipy.io.mmread(str(tmp_path / "bt.mtx")).tocsr()
+    brp, bci = gpu_ctx.block_graph(N.BLOCK_BT)
+    assert (bt != sp.csr_matrix((gpu_ctx.block_values(N.BLOCK_BT), bci, brp), shape=(nv, npr))).nnz == 0
