@@ -78,28 +78,6 @@ void ktimer_end(Ctx* c, const char* name) {
 
 namespace {
 
-unsigned nthreads() {
-  unsigned n = std::thread::hardware_concurrency();
-  return std::max(1u, std::min(n, 16u));
-}
-
-template <class F>
-void parallel_for(int64_t n, F f) {
-  unsigned T = nthreads();
-  if (n < 4096 || T == 1) {
-    f(0, n, 0);
-    return;
-  }
-  std::vector<std::thread> th;
-  int64_t step = (n + T - 1) / T;
-  for (unsigned t = 0; t < T; ++t) {
-    int64_t a = t * step, b = std::min(n, a + step);
-    if (a >= b) break;
-    th.emplace_back(f, a, b, (int)t);
-  }
-  for (auto& x : th) x.join();
-}
-
 void build_incidence(const Mesh& m, Incidence& inc) {
   inc.ptr.assign(m.npts + 1, 0);
   for (int64_t e = 0; e < m.nelem; ++e)
@@ -421,6 +399,9 @@ void fedd_ctx_destroy(fedd_ctx* c) {
   mg_free(c);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& e : c->ev_rr)
+    if (e) (void)hipEventDestroy(e);
+  if (c->h_rr) (void)hipHostFree(c->h_rr);
   ktimer_resolve(c);
   for (auto e : c->kpool) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(c->stream);

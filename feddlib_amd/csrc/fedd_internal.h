@@ -2,9 +2,11 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/feddgpu.h"
@@ -119,6 +121,8 @@ struct Ctx {
   DevBuf d_kw[5];       // r, z, p, Ap and the block-Jacobi inverse (9 per node)
   DevBuf d_part;        // per-block partial sums
   DevBuf d_scal;        // device scalars of the Krylov loop
+  double* h_rr = nullptr;               // pinned: r.r partial sums read back by the CG loop (4 x 16)
+  hipEvent_t ev_rr[4] = {};             // completion of each h_rr slot's copy
   // learned tangent surrogate (FP32 master weights + BF16 copies)
   struct MlpHost* mlp = nullptr;
   // geometric multigrid hierarchy (structured lattice meshes), rebuilt on demand
@@ -187,6 +191,30 @@ int mg_setup(Ctx* c);
 int mg_refresh(Ctx* c);
 int mg_vcycle(Ctx* c, const double* b, double* x);
 void mg_free(Ctx* c);
+
+// Host worker threads (at most 16).
+inline unsigned nthreads() {
+  unsigned n = std::thread::hardware_concurrency();
+  return std::max(1u, std::min(n, 16u));
+}
+
+// f(begin, end, thread) over contiguous ranges of [0, n); serial below 4096 items.
+template <class F>
+void parallel_for(int64_t n, F f) {
+  unsigned T = nthreads();
+  if (n < 4096 || T == 1) {
+    f(0, n, 0);
+    return;
+  }
+  std::vector<std::thread> th;
+  int64_t step = (n + T - 1) / T;
+  for (unsigned t = 0; t < T; ++t) {
+    int64_t a = t * step, b = std::min(n, a + step);
+    if (a >= b) break;
+    th.emplace_back(f, a, b, (int)t);
+  }
+  for (auto& x : th) x.join();
+}
 
 }  // namespace fedd
 

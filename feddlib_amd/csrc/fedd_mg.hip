@@ -303,24 +303,153 @@ __global__ __launch_bounds__(256) void k_mg_densify(int64_t nn, const int64_t* _
     }
 }
 
-// One Gauss-Jordan pivot step of the inverse of an SPD n x n matrix, from the old
-// matrix A into B (pivot k, no pivoting): after n steps the result is A^-1.
+// One block Gauss-Jordan step of the inverse of an SPD n x n matrix (n = 3 nn), pivot
+// block K = rows/columns 3k..3k+2, from the old matrix A into B; after nn steps B = A^-1.
+//   P = A_KK^-1;  B_KK = P;  B_Kj = P A_Kj;  B_iK = -A_iK P;  B_ij = A_ij - A_iK P A_Kj.
 __global__ __launch_bounds__(256) void k_mg_gj_step(int n, int k, const double* __restrict__ A,
                                                     double* __restrict__ B) {
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (e >= (int64_t)n * n) return;
   const int i = (int)(e / n), j = (int)(e - (int64_t)i * n);
-  const double ip = 1.0 / A[(int64_t)k * n + k];
+  const int K0 = 3 * k;
+  double a[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) a[r][q] = A[(int64_t)(K0 + r) * n + K0 + q];
+  const double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) -
+                     a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+                     a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+  const double id = 1.0 / det;
+  double P[3][3];
+  P[0][0] = (a[1][1] * a[2][2] - a[1][2] * a[2][1]) * id;
+  P[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) * id;
+  P[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) * id;
+  P[1][0] = (a[1][2] * a[2][0] - a[1][0] * a[2][2]) * id;
+  P[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) * id;
+  P[1][2] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) * id;
+  P[2][0] = (a[1][0] * a[2][1] - a[1][1] * a[2][0]) * id;
+  P[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) * id;
+  P[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) * id;
+  const bool iK = i >= K0 && i < K0 + 3, jK = j >= K0 && j < K0 + 3;
   double v;
-  if (i == k && j == k)
-    v = ip;
-  else if (i == k)
-    v = A[e] * ip;
-  else if (j == k)
-    v = -A[e] * ip;
-  else
-    v = A[e] - A[(int64_t)i * n + k] * A[(int64_t)k * n + j] * ip;
+  if (iK && jK) {
+    v = P[i - K0][j - K0];
+  } else if (iK) {  // P A_Kj
+    v = P[i - K0][0] * A[(int64_t)K0 * n + j] + P[i - K0][1] * A[(int64_t)(K0 + 1) * n + j] +
+        P[i - K0][2] * A[(int64_t)(K0 + 2) * n + j];
+  } else if (jK) {  // -A_iK P
+    const double* ai = A + (int64_t)i * n + K0;
+    v = -(ai[0] * P[0][j - K0] + ai[1] * P[1][j - K0] + ai[2] * P[2][j - K0]);
+  } else {
+    const double* ai = A + (int64_t)i * n + K0;
+    double t[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) t[q] = ai[0] * P[0][q] + ai[1] * P[1][q] + ai[2] * P[2][q];
+    v = A[e] - (t[0] * A[(int64_t)K0 * n + j] + t[1] * A[(int64_t)(K0 + 1) * n + j] +
+                t[2] * A[(int64_t)(K0 + 2) * n + j]);
+  }
   B[e] = v;
+}
+
+// Inverse of a small SPD matrix (n <= FEDD_MG_LDS_N, n = 3 nn) in one workgroup: the
+// symmetric sweep operator with 3x3 pivot blocks on the packed lower triangle held in
+// LDS, (i, j) with i >= j at i (i + 1) / 2 + j.  Sweeping pivot block K with S = A_KK^-1:
+//   A_KK <- -S;  A_iK <- A_iK S;  A_Kj <- S A_Kj;  A_ij <- A_ij - A_iK S A_Kj.
+// After every block has been swept the matrix holds -A^-1 (Goodnight, "A tutorial on
+// the SWEEP operator", The American Statistician 33, 1979); it is written out negated
+// as the full row-major n x n matrix.
+#define FEDD_MG_LDS_N 192
+__global__ __launch_bounds__(1024) void k_mg_sweep_inverse(int n, const double* __restrict__ A,
+                                                           double* __restrict__ inv) {
+  constexpr int NP = FEDD_MG_LDS_N * (FEDD_MG_LDS_N + 1) / 2;
+  constexpr int PER = (NP + 1023) / 1024;
+  __shared__ double T[NP];
+  __shared__ double colK[3 * FEDD_MG_LDS_N];
+  __shared__ double S[9];
+  const int tid = threadIdx.x;
+  const int np = n * (n + 1) / 2;
+  uint32_t ij[PER];
+#pragma unroll
+  for (int m = 0; m < PER; ++m) {
+    const int e = tid + 1024 * m;
+    ij[m] = 0;
+    if (e < np) {
+      int i = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+      while (i * (i + 1) / 2 > e) --i;
+      while ((i + 1) * (i + 2) / 2 <= e) ++i;
+      const int j = e - i * (i + 1) / 2;
+      ij[m] = ((uint32_t)i << 16) | (uint32_t)j;
+      T[e] = A[(int64_t)i * n + j];
+    }
+  }
+  auto get = [&](int i, int j) -> double { return i >= j ? T[i * (i + 1) / 2 + j] : T[j * (j + 1) / 2 + i]; };
+  for (int k = 0; k < n / 3; ++k) {
+    const int K0 = 3 * k;
+    __syncthreads();
+    if (tid < n) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) colK[3 * tid + q] = get(tid, K0 + q);
+    }
+    if (tid == 0) {
+      double a[3][3];
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) a[r][q] = get(K0 + r, K0 + q);
+      const double c00 = a[1][1] * a[2][2] - a[1][2] * a[2][1];
+      const double c01 = a[1][2] * a[2][0] - a[1][0] * a[2][2];
+      const double c02 = a[1][0] * a[2][1] - a[1][1] * a[2][0];
+      const double id = 1.0 / (a[0][0] * c00 + a[0][1] * c01 + a[0][2] * c02);
+      S[0] = c00 * id;
+      S[1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) * id;
+      S[2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) * id;
+      S[3] = c01 * id;
+      S[4] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) * id;
+      S[5] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) * id;
+      S[6] = c02 * id;
+      S[7] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) * id;
+      S[8] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) * id;
+    }
+    __syncthreads();
+    double s[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) s[r] = S[r];
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+      const int e = tid + 1024 * m;
+      if (e >= np) continue;
+      const int i = (int)(ij[m] >> 16), j = (int)(ij[m] & 0xffffu);
+      const bool iK = (unsigned)(i - K0) < 3u, jK = (unsigned)(j - K0) < 3u;
+      const double* ci = colK + 3 * i;
+      const double* cj = colK + 3 * j;
+      double v;
+      if (iK && jK) {
+        v = -S[3 * (i - K0) + (j - K0)];
+      } else if (jK) {  // (A_iK S)_q
+        const int q = j - K0;
+        v = ci[0] * S[q] + ci[1] * S[3 + q] + ci[2] * S[6 + q];
+      } else if (iK) {  // (S A_Kj)_r
+        const int r = i - K0;
+        v = S[3 * r] * cj[0] + S[3 * r + 1] * cj[1] + S[3 * r + 2] * cj[2];
+      } else {
+        const double t0 = ci[0] * s[0] + ci[1] * s[3] + ci[2] * s[6];
+        const double t1 = ci[0] * s[1] + ci[1] * s[4] + ci[2] * s[7];
+        const double t2 = ci[0] * s[2] + ci[1] * s[5] + ci[2] * s[8];
+        v = T[e] - (t0 * cj[0] + t1 * cj[1] + t2 * cj[2]);
+      }
+      T[e] = v;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < PER; ++m) {
+    const int e = tid + 1024 * m;
+    if (e >= np) continue;
+    const int i = (int)(ij[m] >> 16), j = (int)(ij[m] & 0xffffu);
+    inv[(int64_t)i * n + j] = -T[e];
+    inv[(int64_t)j * n + i] = -T[e];
+  }
 }
 
 __global__ __launch_bounds__(256) void k_mg_fill_zero(int64_t n, double* __restrict__ y) {
@@ -398,8 +527,9 @@ int mg_setup(Ctx* c) {
     MgLevel C;
     C.P = (F.P + 1) / 2;
     C.nn = C.P * C.P * C.P;
-    // coarse adjacency: parents(i) x parents(j) over every fine coupling (i, j)
-    std::vector<std::vector<int32_t>> rows(C.nn);
+    // Coarse adjacency: J is a neighbour of I iff some child of I (a fine node with I
+    // among its parents) couples to a fine node with J among its parents.  Rows are
+    // built per coarse node, split over host threads, deduplicated with a stamp array.
     auto par = [&](int64_t i, int64_t out[2]) -> int {
       const int64_t x0 = i % F.P, x1 = (i / F.P) % F.P, x2 = i / (F.P * F.P);
       const int64_t o0 = x0 & 1, o1 = x1 & 1, o2 = x2 & 1;
@@ -411,27 +541,55 @@ int mg_setup(Ctx* c) {
       out[1] = ((x0 + o0) >> 1) + C.P * (((x1 + o1) >> 1) + C.P * ((x2 + o2) >> 1));
       return 2;
     };
-    for (int64_t i = 0; i < F.nn; ++i) {
-      int64_t pi[2];
-      const int ni = par(i, pi);
-      for (int64_t q = F.h_ptr[i]; q < F.h_ptr[i + 1]; ++q) {
-        int64_t pj[2];
-        const int nj = par(F.h_idx[q], pj);
-        for (int a = 0; a < ni; ++a)
-          for (int b = 0; b < nj; ++b) rows[pi[a]].push_back((int32_t)pj[b]);
+    const unsigned T = nthreads();
+    std::vector<std::vector<int32_t>> part_idx(T);
+    std::vector<std::vector<int64_t>> part_len(T);
+    std::vector<int64_t> part_begin(T, C.nn);
+    parallel_for(C.nn, [&](int64_t a, int64_t b, int t) {
+      std::vector<int32_t> stamp(C.nn, -1);
+      std::vector<int32_t> row;
+      auto& idx = part_idx[t];
+      auto& len = part_len[t];
+      part_begin[t] = a;
+      for (int64_t I = a; I < b; ++I) {
+        row.clear();
+        const int64_t X0 = I % C.P, X1 = (I / C.P) % C.P, X2 = I / (C.P * C.P);
+        for (int64_t d2 = -1; d2 <= 1; ++d2)
+          for (int64_t d1 = -1; d1 <= 1; ++d1)
+            for (int64_t d0 = -1; d0 <= 1; ++d0) {
+              const int64_t f0 = 2 * X0 + d0, f1 = 2 * X1 + d1, f2 = 2 * X2 + d2;
+              if (f0 < 0 || f1 < 0 || f2 < 0 || f0 >= F.P || f1 >= F.P || f2 >= F.P) continue;
+              const int64_t f = f0 + F.P * (f1 + F.P * f2);
+              int64_t pf[2];
+              const int nf = par(f, pf);
+              if (pf[0] != I && (nf < 2 || pf[1] != I)) continue;
+              for (int64_t q = F.h_ptr[f]; q < F.h_ptr[f + 1]; ++q) {
+                int64_t pj[2];
+                const int nj = par(F.h_idx[q], pj);
+                for (int k = 0; k < nj; ++k)
+                  if (stamp[pj[k]] != (int32_t)I) {
+                    stamp[pj[k]] = (int32_t)I;
+                    row.push_back((int32_t)pj[k]);
+                  }
+              }
+            }
+        std::sort(row.begin(), row.end());
+        idx.insert(idx.end(), row.begin(), row.end());
+        len.push_back((int64_t)row.size());
       }
-    }
+    });
     C.h_ptr.assign(C.nn + 1, 0);
-    for (int64_t I = 0; I < C.nn; ++I) {
-      auto& r = rows[I];
-      std::sort(r.begin(), r.end());
-      r.erase(std::unique(r.begin(), r.end()), r.end());
-      C.h_ptr[I + 1] = C.h_ptr[I] + (int64_t)r.size();
-      C.maxdeg = std::max<int64_t>(C.maxdeg, (int64_t)r.size());
+    C.h_idx.clear();
+    for (unsigned t = 0; t < T; ++t) {
+      int64_t I = part_begin[t];
+      for (int64_t l : part_len[t]) {
+        C.h_ptr[I + 1] = C.h_ptr[I] + l;
+        C.maxdeg = std::max(C.maxdeg, l);
+        ++I;
+      }
+      C.h_idx.insert(C.h_idx.end(), part_idx[t].begin(), part_idx[t].end());
     }
     C.nblk = C.h_ptr[C.nn];
-    C.h_idx.reserve(C.nblk);
-    for (auto& r : rows) C.h_idx.insert(C.h_idx.end(), r.begin(), r.end());
     g->lv.push_back(std::move(C));
   }
   // device arrays and Dirichlet masks
@@ -501,14 +659,22 @@ static int refresh_eager(Ctx* c) {
   }
   const MgLevel& Cl = g->lv.back();
   const int64_t n = g->ncoarse;
-  // ping-pong so that the last of the n steps lands in inv
-  double* bufs[2] = {g->inv.as<double>(), g->dense.as<double>()};
-  double* M0 = bufs[n % 2];
   ktimer_begin(c);
-  k_mg_fill_zero<<<nbk(n * n, 256), 256, 0, c->stream>>>(n * n, M0);
-  k_mg_densify<<<nbk(Cl.nn, 256), 256, 0, c->stream>>>(Cl.nn, Cl.d_nptr, Cl.d_nidx, Cl.d_vals, M0);
-  for (int k = 0; k < (int)n; ++k)
-    k_mg_gj_step<<<nbk(n * n, 256), 256, 0, c->stream>>>((int)n, k, bufs[(n - k) % 2], bufs[(n - k + 1) % 2]);
+  if (n <= FEDD_MG_LDS_N) {  // one workgroup, in LDS
+    double* M0 = g->dense.as<double>();
+    k_mg_fill_zero<<<nbk(n * n, 256), 256, 0, c->stream>>>(n * n, M0);
+    k_mg_densify<<<nbk(Cl.nn, 256), 256, 0, c->stream>>>(Cl.nn, Cl.d_nptr, Cl.d_nidx, Cl.d_vals, M0);
+    k_mg_sweep_inverse<<<1, 1024, 0, c->stream>>>((int)n, M0, g->inv.as<double>());
+  } else {  // one launch per pivot block; ping-pong so that the last step lands in inv
+    const int64_t nsteps = n / 3;
+    double* bufs[2] = {g->inv.as<double>(), g->dense.as<double>()};
+    double* M0 = bufs[nsteps % 2];
+    k_mg_fill_zero<<<nbk(n * n, 256), 256, 0, c->stream>>>(n * n, M0);
+    k_mg_densify<<<nbk(Cl.nn, 256), 256, 0, c->stream>>>(Cl.nn, Cl.d_nptr, Cl.d_nidx, Cl.d_vals, M0);
+    for (int k = 0; k < (int)nsteps; ++k)
+      k_mg_gj_step<<<nbk(n * n, 256), 256, 0, c->stream>>>((int)n, k, bufs[(nsteps - k) % 2],
+                                                           bufs[(nsteps - k + 1) % 2]);
+  }
   FEDD_HIP(c, hipGetLastError());
   ktimer_end(c, "mg_coarsest_inverse");
   return FEDD_OK;

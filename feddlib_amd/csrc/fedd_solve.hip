@@ -199,14 +199,20 @@ __global__ __launch_bounds__(256) void k_pcg_p(int64_t n, double* __restrict__ p
     scal[(w == 0 ? PCG_PAP(t + 1) : w == 1 ? PCG_RZ(t + 1) : PCG_RR(t + 1)) + k] = 0.0;
   }
   if (i >= n) return;
-  p[i] = first ? z[i] : z[i] + (sum16(scal + PCG_RZ(t)) / sum16(scal + PCG_RZ(t + 2))) * p[i];
+  if (first) {
+    p[i] = z[i];
+    return;
+  }
+  const double rz_old = sum16(scal + PCG_RZ(t + 2));
+  p[i] = z[i] + (rz_old != 0.0 ? sum16(scal + PCG_RZ(t)) / rz_old : 0.0) * p[i];
 }
 
 // x += alpha p; r -= alpha Ap; r.r added to RR(t) (multigrid-preconditioned CG).
 __global__ __launch_bounds__(256) void k_pcg_xr(int64_t n, double* __restrict__ x, double* __restrict__ r,
                                                 const double* __restrict__ p, const double* __restrict__ Ap,
                                                 double* __restrict__ scal, int t) {
-  const double alpha = sum16(scal + PCG_RZ(t + 2)) / sum16(scal + PCG_PAP(t));
+  const double pap = sum16(scal + PCG_PAP(t));
+  const double alpha = pap != 0.0 ? sum16(scal + PCG_RZ(t + 2)) / pap : 0.0;
   double rr = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     x[i] += alpha * p[i];
@@ -381,7 +387,11 @@ int pcg_solve(Ctx* c, double rtol, double atol, int maxit, int* iters, double* r
 
 // CG preconditioned with one geometric-multigrid V-cycle per iteration (structured
 // lattice meshes); same scalar slots and convergence test as pcg_solve.
-int pcg_mg_solve(Ctx* c, double rtol, double atol, int maxit, int* iters, double* relres, bool refresh) {
+// bnorm_known > 0: ||b|| is known to the caller (Newton: ||R||), which saves a
+// synchronisation.  The convergence test reads r.r of iteration t-1 from pinned host
+// memory while iteration t is already queued (at most one iteration beyond the target).
+int pcg_mg_solve(Ctx* c, double rtol, double atol, int maxit, int* iters, double* relres, bool refresh,
+                 double bnorm_known) {
   const Block& A = c->blk[0];
   const int64_t n = A.nrows, nnode = n / 3;
   int rc;
@@ -410,19 +420,28 @@ int pcg_mg_solve(Ctx* c, double rtol, double atol, int maxit, int* iters, double
   k_pcg_dot<<<g_red, 256, 0, c->stream>>>(n, r, z, scal, 2, 1);
   k_pcg_p<<<g_dofs, 256, 0, c->stream>>>(n, p, z, scal, -1, 1);
   FEDD_HIP(c, hipGetLastError());
-  double h[16];
-  auto read_rr = [&](int t) -> int {
-    FEDD_HIP(c, hipMemcpyAsync(h, scal + PCG_RR(t), 16 * 8, hipMemcpyDeviceToHost, c->stream));
-    FEDD_HIP(c, hipStreamSynchronize(c->stream));
+  if (!c->h_rr) {
+    FEDD_HIP(c, hipHostMalloc((void**)&c->h_rr, 4 * 16 * 8, hipHostMallocDefault));
+    for (auto& e : c->ev_rr) FEDD_HIP(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  auto post_rr = [&](int t, int slot) -> int {
+    FEDD_HIP(c, hipMemcpyAsync(c->h_rr + 16 * slot, scal + PCG_RR(t), 16 * 8, hipMemcpyDeviceToHost, c->stream));
+    FEDD_HIP(c, hipEventRecord(c->ev_rr[slot], c->stream));
     return FEDD_OK;
   };
-  auto sum = [&]() {
-    double v = 0.0;
-    for (int k = 0; k < 16; ++k) v += h[k];
-    return v;
+  auto wait_rr = [&](int slot, double* v) -> int {
+    FEDD_HIP(c, hipEventSynchronize(c->ev_rr[slot]));
+    double s = 0.0;
+    for (int k = 0; k < 16; ++k) s += c->h_rr[16 * slot + k];
+    *v = s;
+    return FEDD_OK;
   };
-  if ((rc = read_rr(2))) return rc;
-  const double bnorm = std::sqrt(sum());
+  double bnorm = bnorm_known;
+  if (!(bnorm > 0.0)) {
+    double v;
+    if ((rc = post_rr(2, 3)) || (rc = wait_rr(3, &v))) return rc;
+    bnorm = std::sqrt(v);
+  }
   double rnorm = bnorm;
   int it = 0;
   const double target = std::max(rtol * bnorm, atol);
@@ -434,14 +453,23 @@ int pcg_mg_solve(Ctx* c, double rtol, double atol, int maxit, int* iters, double
       if ((rc = mg_vcycle(c, r, z))) return rc;
       k_pcg_dot<<<g_red, 256, 0, c->stream>>>(n, r, z, scal, t, 0);
       k_pcg_p<<<g_dofs, 256, 0, c->stream>>>(n, p, z, scal, t, 0);
+      FEDD_HIP(c, hipGetLastError());
+      if ((rc = post_rr(t, t & 1))) return rc;
       ++it;
-      if ((it & 1) == 0 || it == maxit) {
-        FEDD_HIP(c, hipGetLastError());
-        if ((rc = read_rr(t))) return rc;
-        rnorm = std::sqrt(sum());
-        if (!(rnorm == rnorm)) break;  // NaN: let the caller reject the step
-        if (rnorm <= target) break;
+      if (t == 0 && it < maxit) continue;
+      // test the previous iteration while this one runs (at maxit: this one)
+      const int tc = it == maxit ? t : t - 1;
+      double v;
+      if ((rc = wait_rr(tc & 1, &v))) return rc;
+      rnorm = std::sqrt(v);
+      if (!(rnorm == rnorm)) break;  // NaN: let the caller reject the step
+      if (rnorm <= target && tc != t) {
+        // x already holds iteration t: report its residual (CG's is not monotone)
+        if ((rc = wait_rr(t & 1, &v))) return rc;
+        rnorm = std::sqrt(v);
+        if (!(rnorm == rnorm)) break;
       }
+      if (rnorm <= target) break;
     }
   }
   FEDD_HIP(c, hipGetLastError());
@@ -548,7 +576,7 @@ int fedd_linear_solve(fedd_ctx* cc, double rtol, int maxit, int* iters, double* 
   if (c->graph_kind != FEDD_GRAPH_VECTOR) return fail(c, FEDD_ERR_STATE, "needs a vector graph");
   if (c->vec_n[FEDD_VEC_B] != c->blk[0].nrows) return fail(c, FEDD_ERR_STATE, "right-hand side not set");
   (void)hipSetDevice(c->device);
-  if (c->precond == 1) return pcg_mg_solve(c, rtol, 0.0, maxit, iters, relres, true);
+  if (c->precond == 1) return pcg_mg_solve(c, rtol, 0.0, maxit, iters, relres, true, 0.0);
   return pcg_solve(c, rtol, 0.0, maxit, iters, relres);
 }
 
@@ -587,9 +615,21 @@ int fedd_newton_elasticity(fedd_ctx* cc, const fedd_newton_params* P, fedd_newto
   if ((rc = ensure_vec(c, FEDD_VEC_RES_U, n))) return rc;
   std::memset(st, 0, sizeof(*st));
   auto t0 = std::chrono::steady_clock::now();
-  hipEvent_t ea, eb;
+  hipEvent_t ea, eb, sa, sb;
   FEDD_HIP(c, hipEventCreate(&ea));
   FEDD_HIP(c, hipEventCreate(&eb));
+  FEDD_HIP(c, hipEventCreate(&sa));
+  FEDD_HIP(c, hipEventCreate(&sb));
+  bool solve_pending = false;
+  // the solve's span is read after the next synchronisation (no extra stall)
+  auto settle_solve = [&]() -> int {
+    if (!solve_pending) return FEDD_OK;
+    float ms = 0.f;
+    FEDD_HIP(c, hipEventElapsedTime(&ms, sa, sb));
+    st->solve_ms += ms;
+    solve_pending = false;
+    return FEDD_OK;
+  };
   DevBuf u_prev;
   if ((rc = alloc(c, u_prev, n * 8))) return rc;
   double* u = c->vec[FEDD_VEC_U].as<double>();
@@ -607,6 +647,7 @@ int fedd_newton_elasticity(fedd_ctx* cc, const fedd_newton_params* P, fedd_newto
       FEDD_HIP(c, hipEventRecord(eb, c->stream));
       if (P->collect_samples && (rc = mlp_collect_qp(c))) return rc;
       if ((rc = newton_residual(c, lam, &norm))) return rc;
+      if ((rc = settle_solve())) return rc;
       float ms = 0.f;
       FEDD_HIP(c, hipEventElapsedTime(&ms, ea, eb));
       st->assembly_ms += ms;
@@ -648,16 +689,14 @@ int fedd_newton_elasticity(fedd_ctx* cc, const fedd_newton_params* P, fedd_newto
       eta = std::max(eta, 0.1 * P->abs_tol / std::max(norm, 1e-300));
       int iters = 0;
       double relres = 0.0;
-      FEDD_HIP(c, hipEventRecord(ea, c->stream));
+      FEDD_HIP(c, hipEventRecord(sa, c->stream));
       if (P->preconditioner == 1)
-        rc = pcg_mg_solve(c, eta, 0.0, P->lin_maxit, &iters, &relres, true);
+        rc = pcg_mg_solve(c, eta, 0.0, P->lin_maxit, &iters, &relres, true, norm);  // b = -R, ||b|| = norm
       else
         rc = pcg_solve(c, eta, 0.0, P->lin_maxit, &iters, &relres);
       if (rc) return rc;
-      FEDD_HIP(c, hipEventRecord(eb, c->stream));
-      FEDD_HIP(c, hipEventSynchronize(eb));
-      FEDD_HIP(c, hipEventElapsedTime(&ms, ea, eb));
-      st->solve_ms += ms;
+      FEDD_HIP(c, hipEventRecord(sb, c->stream));
+      solve_pending = true;  // resolved after the next synchronisation
       st->krylov_iters += iters;
       FEDD_HIP(c, hipMemcpyAsync(u_prev.p, u, n * 8, hipMemcpyDeviceToDevice, c->stream));
       k_axpy<<<nb(n, 256), 256, 0, c->stream>>>(n, 1.0, c->vec[FEDD_VEC_X].as<double>(), u);
@@ -669,8 +708,11 @@ int fedd_newton_elasticity(fedd_ctx* cc, const fedd_newton_params* P, fedd_newto
   }
   FEDD_HIP(c, hipStreamSynchronize(c->stream));
   auto t1 = std::chrono::steady_clock::now();
+  if ((rc = settle_solve())) return rc;
   (void)hipEventDestroy(ea);
   (void)hipEventDestroy(eb);
+  (void)hipEventDestroy(sa);
+  (void)hipEventDestroy(sb);
   st->seconds = std::chrono::duration<double>(t1 - t0).count();
   st->final_residual = final_norm;
   st->converged = all_converged ? 1 : 0;

@@ -26,7 +26,9 @@ def _problem(M):
     return pts, el, dofs
 
 
-@pytest.mark.parametrize("M", [2, 3, 6])
+# coarsest levels: M=2 -> 27 nodes, M=3 and M=6 -> 64 nodes (single-workgroup LDS
+# inverse), M=5 -> 216 nodes (one Gauss-Jordan launch per pivot block)
+@pytest.mark.parametrize("M", [2, 3, 5, 6])
 def test_mg_linear_solve(gpu_ctx, M):
     pts, el, dofs = _problem(M)
     n = 3 * pts.shape[0]
