@@ -408,7 +408,10 @@ void fedd_ctx_destroy(fedd_ctx* c) {
   delete c;
 }
 
-const char* fedd_last_error(const fedd_ctx* c) { return c ? c->err.c_str() : "null context"; }
+const char* fedd_last_error(const fedd_ctx* c) {
+  if (c) return c->err.c_str();
+  return fedd::free_error().empty() ? "null context" : fedd::free_error().c_str();
+}
 
 int fedd_synchronize(fedd_ctx* c) {
   if (!c) return FEDD_ERR_ARG;

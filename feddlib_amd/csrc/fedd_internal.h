@@ -150,6 +150,8 @@ struct Ctx {
 
 // Error helpers --------------------------------------------------------------
 int fail(Ctx* c, int code, const std::string& msg);
+// Message of the last failed context-free call on this thread (fedd_last_error(NULL)).
+std::string& free_error();
 #define FEDD_HIP(c, call)                                                        \
   do {                                                                           \
     hipError_t _e = (call);                                                      \

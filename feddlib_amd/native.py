@@ -77,6 +77,8 @@ _SIGS = {
     "fedd_ns_merged_info": (ctypes.c_int, [_vp, _lp, _lp]),
     "fedd_ns_merged_export": (ctypes.c_int, [_vp, _lp, _ip, _dp]),
     "fedd_write_matrix_market": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_char_p]),
+    "fedd_hdf5_read_vector": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _dp, ctypes.c_int64, _lp,
+                                             ctypes.POINTER(ctypes.c_int)]),
     "fedd_colour_build": (ctypes.c_int, [_vp, _ip]),
     "fedd_colour_export": (ctypes.c_int, [_vp, _lp, _ip]),
     "fedd_vector_upload": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.c_int64]),
@@ -161,6 +163,21 @@ def mesh_structured(M, fe):
     if rc:
         raise FeddError(rc, "fedd_mesh_structured")
     return pts, bc, el
+
+
+def hdf5_read_vector(path, name):
+    """HDF5Import::readVariablesHDF5 (HDF5Import_def.hpp:52-70): the EpetraExt multivector
+    `name` of an HDF5 file, column 0, in global-id order (float64 array)."""
+    L = lib()
+    n, k = ctypes.c_int64(0), ctypes.c_int(0)
+    rc = L.fedd_hdf5_read_vector(str(path).encode(), name.encode(), None, 0, ctypes.byref(n), ctypes.byref(k))
+    if rc:
+        raise FeddError(rc, L.fedd_last_error(None).decode())
+    out = np.zeros(n.value)
+    rc = L.fedd_hdf5_read_vector(str(path).encode(), name.encode(), _p(out, _dp), out.size, None, None)
+    if rc:
+        raise FeddError(rc, L.fedd_last_error(None).decode())
+    return out
 
 
 def correct_orientation(pts, elems):

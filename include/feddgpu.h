@@ -136,6 +136,17 @@ int fedd_ns_merged_export(fedd_ctx* ctx, int64_t* rowptr, int32_t* colind, doubl
  * with 17 significant digits. */
 int fedd_write_matrix_market(fedd_ctx* ctx, int block, const char* path);
 
+/* EpetraExt-HDF5 vector reader: HDF5Import::readVariablesHDF5 (HDF5Import_def.hpp:52-70).
+ * Reads the multivector group `name` (datasets Values, GlobalLength, NumVectors) of a
+ * file written by HDF5Export / EpetraExt::HDF5.  On one rank the reference's read map
+ * lists every global id in order, so out[k] is the value of global id k (column 0,
+ * the column the reference copies into its vector).  out may be NULL to query
+ * global_length / num_vectors; cap is the capacity of out.  Context-free: a missing
+ * variable ("Requested varName ... not contained", HDF5Import_def.hpp:53) or an
+ * unreadable file returns FEDD_ERR_ARG with the message in fedd_last_error(NULL). */
+int fedd_hdf5_read_vector(const char* path, const char* name, double* out, int64_t cap,
+                          int64_t* global_length, int* num_vectors);
+
 /* ------------------------------------------------------------- colouring */
 /* Element colouring of the primary mesh: elements of one colour share no node.
  * No counterpart in the reference (it assembles serially per rank); used by the
