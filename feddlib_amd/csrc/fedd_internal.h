@@ -191,6 +191,8 @@ int mlp_collect_qp(Ctx* c);
 // coarse operators for the current A, and one V-cycle x = M^-1 b on the finest level.
 int mg_setup(Ctx* c);
 int mg_refresh(Ctx* c);
+// Finest level only (smoother blocks and FP32 copy); coarse levels keep their operators.
+int mg_refresh_fine(Ctx* c);
 int mg_vcycle(Ctx* c, const double* b, double* x);
 void mg_free(Ctx* c);
 

@@ -17,6 +17,7 @@
 // Reference role: the FROSch / MueLu preconditioners behind Belos
 // (Preconditioner_def.hpp:206-291, LinearSolver_def.hpp:23-137).
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <vector>
 
@@ -27,6 +28,12 @@
 #endif
 #ifndef FEDD_MG_OMEGA
 #define FEDD_MG_OMEGA 0.6  // block-Jacobi damping
+#endif
+// The smoother and residual of the finest level read an FP32 copy of the operator
+// (refreshed with it): half the bytes of the HBM-bound sweeps.  The preconditioner stays
+// a fixed symmetric linear operator, so CG's FP64 iteration is unaffected.
+#ifndef FEDD_MG_FP32_FINE
+#define FEDD_MG_FP32_FINE 1
 #endif
 
 namespace fedd {
@@ -40,6 +47,7 @@ struct MgLevel {
   const int32_t* d_nidx = nullptr;
   double* d_vals = nullptr;
   DevBuf binv, free, x, x2, b, r;
+  DevBuf vals32;  // finest level: FP32 copy of the operator used by the smoother
   std::vector<int64_t> h_ptr;
   std::vector<int32_t> h_idx;
 };
@@ -116,10 +124,12 @@ __device__ __forceinline__ int find_slot(const int32_t* __restrict__ idx, int de
 
 // Damped block-Jacobi sweep xout = xin + omega Binv (b - A xin), G lanes per node
 // (first: xin = 0, no product).  RESID: r = b - A x instead.
-template <int G, int MODE>  // MODE 0: sweep, 1: first sweep, 2: residual
+// VT: stored type of the operator values (float for the FP32 copy of the finest level);
+// products and sums are FP64.
+template <int G, int MODE, class VT>  // MODE 0: sweep, 1: first sweep, 2: residual
 __global__ __launch_bounds__(256) void k_mg_sweep(int64_t nn, const int64_t* __restrict__ nptr,
                                                   const int32_t* __restrict__ nidx,
-                                                  const double* __restrict__ vals, const double* __restrict__ binv,
+                                                  const VT* __restrict__ vals, const double* __restrict__ binv,
                                                   const double* __restrict__ b, const double* __restrict__ xin,
                                                   double* __restrict__ out, double omega) {
   const int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G;
@@ -128,13 +138,13 @@ __global__ __launch_bounds__(256) void k_mg_sweep(int64_t nn, const int64_t* __r
   if (i < nn && MODE != 1) {
     const int64_t k0 = nptr[i];
     const int L = 3 * (int)(nptr[i + 1] - k0);
-    const double* v = vals + 9 * k0;
+    const VT* v = vals + 9 * k0;
     const int32_t* ni = nidx + k0;
     for (int q = lane; q < L; q += G) {
       const int k = (q * 0xAAAB) >> 17;  // q / 3 for q < 2^15
       const double xv = xin[3 * (int64_t)ni[k] + (q - 3 * k)];
 #pragma unroll
-      for (int d = 0; d < 3; ++d) s[d] += v[d * L + q] * xv;
+      for (int d = 0; d < 3; ++d) s[d] += (double)v[d * L + q] * xv;
     }
   }
 #pragma unroll
@@ -156,6 +166,11 @@ __global__ __launch_bounds__(256) void k_mg_sweep(int64_t nn, const int64_t* __r
     const double z = B[3 * d] * rv[0] + B[3 * d + 1] * rv[1] + B[3 * d + 2] * rv[2];
     out[3 * i + d] = (MODE == 1 ? 0.0 : xin[3 * i + d]) + omega * z;
   }
+}
+
+__global__ __launch_bounds__(256) void k_mg_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
 }
 
 __global__ __launch_bounds__(256) void k_mg_restrict(int64_t Pf, int64_t Pc, const double* __restrict__ rf,
@@ -470,36 +485,49 @@ __global__ __launch_bounds__(256) void k_mg_dense_mv(int64_t n, const double* __
 }
 
 // ------------------------------------------------------------------ host side
-// True when the primary mesh is the P2 structured lattice of fedd_mesh_structured
-// (node id = r + s P + t P^2, vertices on even, edge nodes on midpoint lattice
-// positions, every edge a Kuhn edge e_S).
+// True when the primary mesh is the P2 structured lattice of fedd_mesh_structured:
+// node id = r + s P + t P^2, 6 M^3 elements (P = 2 M + 1), vertices on even lattice
+// points, edge nodes at the midpoints, and every edge a Kuhn edge +-e_S (nonzero
+// components of one sign), as the prolongation and the coarse stencils assume.
 static bool lattice_mesh(const Mesh& m, int64_t& P) {
   if (m.npe != 10 || m.npts < 27) return false;
   P = (int64_t)std::llround(std::cbrt((double)m.npts));
   if (P * P * P != m.npts || (P & 1) == 0) return false;
-  const int ea[6] = {0, 1, 0, 0, 1, 2}, eb[6] = {1, 2, 2, 3, 3, 3};
-  auto xyz = [&](int64_t i, int64_t x[3]) {
-    x[0] = i % P;
-    x[1] = (i / P) % P;
-    x[2] = i / (P * P);
-  };
-  for (int64_t e = 0; e < m.nelem; ++e) {
-    const int32_t* nd = &m.elems[e * 10];
-    int64_t v[4][3];
-    for (int k = 0; k < 4; ++k) {
-      xyz(nd[k], v[k]);
-      if ((v[k][0] | v[k][1] | v[k][2]) & 1) return false;
-    }
-    for (int t = 0; t < 6; ++t) {
-      int64_t x[3];
-      xyz(nd[4 + t], x);
-      for (int d = 0; d < 3; ++d) {
-        const int64_t dv = v[eb[t]][d] - v[ea[t]][d];
-        if (2 * x[d] != v[ea[t]][d] + v[eb[t]][d] || (dv != 0 && dv != 2 && dv != -2)) return false;
+  const int64_t Mc = (P - 1) / 2;
+  if (m.nelem != 6 * Mc * Mc * Mc) return false;
+  std::atomic<bool> ok(true);
+  parallel_for(m.nelem, [&](int64_t a, int64_t b, int) {
+    const int ea[6] = {0, 1, 0, 0, 1, 2}, eb[6] = {1, 2, 2, 3, 3, 3};
+    auto xyz = [&](int64_t i, int64_t x[3]) {
+      x[0] = i % P;
+      x[1] = (i / P) % P;
+      x[2] = i / (P * P);
+    };
+    for (int64_t e = a; e < b && ok.load(std::memory_order_relaxed); ++e) {
+      const int32_t* nd = &m.elems[e * 10];
+      int64_t v[4][3];
+      for (int k = 0; k < 4; ++k) {
+        if (nd[k] < 0 || nd[k] >= m.npts) return (void)ok.store(false);
+        xyz(nd[k], v[k]);
+        if ((v[k][0] | v[k][1] | v[k][2]) & 1) return (void)ok.store(false);
+      }
+      for (int t = 0; t < 6; ++t) {
+        if (nd[4 + t] < 0 || nd[4 + t] >= m.npts) return (void)ok.store(false);
+        int64_t x[3];
+        xyz(nd[4 + t], x);
+        int pos = 0, neg = 0;
+        for (int d = 0; d < 3; ++d) {
+          const int64_t dv = v[eb[t]][d] - v[ea[t]][d];
+          if (2 * x[d] != v[ea[t]][d] + v[eb[t]][d] || (dv != 0 && dv != 2 && dv != -2))
+            return (void)ok.store(false);
+          pos += dv > 0;
+          neg += dv < 0;
+        }
+        if (pos && neg) return (void)ok.store(false);
       }
     }
-  }
-  return true;
+  });
+  return ok.load();
 }
 
 int mg_setup(Ctx* c) {
@@ -510,84 +538,36 @@ int mg_setup(Ctx* c) {
     return fail(c, FEDD_ERR_STATE, "multigrid needs the structured P2 lattice mesh of fedd_mesh_structured");
   MgHost* g = new MgHost();
   c->mg = g;
-  // level 0: the context's operator
+  // level 0: the context's operator (9 values per node block)
   {
     MgLevel L0;
     L0.P = P0;
     L0.nn = P0 * P0 * P0;
-    L0.h_ptr.resize(L0.nn + 1);
-    FEDD_HIP(c, hipMemcpy(L0.h_ptr.data(), c->d_nptr.p, (L0.nn + 1) * 8, hipMemcpyDeviceToHost));
-    L0.nblk = L0.h_ptr[L0.nn];
-    L0.h_idx.resize(L0.nblk);
-    FEDD_HIP(c, hipMemcpy(L0.h_idx.data(), c->d_nidx.p, L0.nblk * 4, hipMemcpyDeviceToHost));
+    L0.nblk = c->blk[0].nnz / 9;
     g->lv.push_back(std::move(L0));
   }
+  // Coarse levels: P1 lattices.  The Galerkin product of the midpoint prolongation on the
+  // Kuhn lattice couples a coarse node exactly with itself and its 14 Kuhn neighbours
+  // I +- e_S (the edges of the P1 Kuhn mesh), from the P2 level as from every P1 level.
+  static const int kuhn[15][3] = {{0, 0, 0},  {1, 0, 0},  {-1, 0, 0},  {0, 1, 0},   {0, -1, 0},
+                                  {0, 0, 1},  {0, 0, -1}, {1, 1, 0},   {-1, -1, 0}, {1, 0, 1},
+                                  {-1, 0, -1}, {0, 1, 1}, {0, -1, -1}, {1, 1, 1},   {-1, -1, -1}};
   while (g->lv.back().P >= 5 && ((g->lv.back().P - 1) & 1) == 0) {
-    const MgLevel& F = g->lv.back();
     MgLevel C;
-    C.P = (F.P + 1) / 2;
+    C.P = (g->lv.back().P + 1) / 2;
     C.nn = C.P * C.P * C.P;
-    // Coarse adjacency: J is a neighbour of I iff some child of I (a fine node with I
-    // among its parents) couples to a fine node with J among its parents.  Rows are
-    // built per coarse node, split over host threads, deduplicated with a stamp array.
-    auto par = [&](int64_t i, int64_t out[2]) -> int {
-      const int64_t x0 = i % F.P, x1 = (i / F.P) % F.P, x2 = i / (F.P * F.P);
-      const int64_t o0 = x0 & 1, o1 = x1 & 1, o2 = x2 & 1;
-      if (!(o0 | o1 | o2)) {
-        out[0] = (x0 >> 1) + C.P * ((x1 >> 1) + C.P * (x2 >> 1));
-        return 1;
-      }
-      out[0] = ((x0 - o0) >> 1) + C.P * (((x1 - o1) >> 1) + C.P * ((x2 - o2) >> 1));
-      out[1] = ((x0 + o0) >> 1) + C.P * (((x1 + o1) >> 1) + C.P * ((x2 + o2) >> 1));
-      return 2;
-    };
-    const unsigned T = nthreads();
-    std::vector<std::vector<int32_t>> part_idx(T);
-    std::vector<std::vector<int64_t>> part_len(T);
-    std::vector<int64_t> part_begin(T, C.nn);
-    parallel_for(C.nn, [&](int64_t a, int64_t b, int t) {
-      std::vector<int32_t> stamp(C.nn, -1);
-      std::vector<int32_t> row;
-      auto& idx = part_idx[t];
-      auto& len = part_len[t];
-      part_begin[t] = a;
-      for (int64_t I = a; I < b; ++I) {
-        row.clear();
-        const int64_t X0 = I % C.P, X1 = (I / C.P) % C.P, X2 = I / (C.P * C.P);
-        for (int64_t d2 = -1; d2 <= 1; ++d2)
-          for (int64_t d1 = -1; d1 <= 1; ++d1)
-            for (int64_t d0 = -1; d0 <= 1; ++d0) {
-              const int64_t f0 = 2 * X0 + d0, f1 = 2 * X1 + d1, f2 = 2 * X2 + d2;
-              if (f0 < 0 || f1 < 0 || f2 < 0 || f0 >= F.P || f1 >= F.P || f2 >= F.P) continue;
-              const int64_t f = f0 + F.P * (f1 + F.P * f2);
-              int64_t pf[2];
-              const int nf = par(f, pf);
-              if (pf[0] != I && (nf < 2 || pf[1] != I)) continue;
-              for (int64_t q = F.h_ptr[f]; q < F.h_ptr[f + 1]; ++q) {
-                int64_t pj[2];
-                const int nj = par(F.h_idx[q], pj);
-                for (int k = 0; k < nj; ++k)
-                  if (stamp[pj[k]] != (int32_t)I) {
-                    stamp[pj[k]] = (int32_t)I;
-                    row.push_back((int32_t)pj[k]);
-                  }
-              }
-            }
-        std::sort(row.begin(), row.end());
-        idx.insert(idx.end(), row.begin(), row.end());
-        len.push_back((int64_t)row.size());
-      }
-    });
     C.h_ptr.assign(C.nn + 1, 0);
-    C.h_idx.clear();
-    for (unsigned t = 0; t < T; ++t) {
-      int64_t I = part_begin[t];
-      for (int64_t l : part_len[t]) {
-        C.h_ptr[I + 1] = C.h_ptr[I] + l;
-        C.maxdeg = std::max(C.maxdeg, l);
-        ++I;
+    C.h_idx.reserve(15 * C.nn);
+    for (int64_t I = 0; I < C.nn; ++I) {
+      const int64_t X[3] = {I % C.P, (I / C.P) % C.P, I / (C.P * C.P)};
+      for (const auto& o : kuhn) {
+        const int64_t Y[3] = {X[0] + o[0], X[1] + o[1], X[2] + o[2]};
+        if (Y[0] < 0 || Y[1] < 0 || Y[2] < 0 || Y[0] >= C.P || Y[1] >= C.P || Y[2] >= C.P) continue;
+        C.h_idx.push_back((int32_t)(Y[0] + C.P * (Y[1] + C.P * Y[2])));
       }
-      C.h_idx.insert(C.h_idx.end(), part_idx[t].begin(), part_idx[t].end());
+      C.h_ptr[I + 1] = (int64_t)C.h_idx.size();
+      std::sort(C.h_idx.begin() + C.h_ptr[I], C.h_idx.end());
+      C.maxdeg = std::max(C.maxdeg, C.h_ptr[I + 1] - C.h_ptr[I]);
     }
     C.nblk = C.h_ptr[C.nn];
     g->lv.push_back(std::move(C));
@@ -623,11 +603,12 @@ int mg_setup(Ctx* c) {
     for (DevBuf* b : {&L.x, &L.x2, &L.b, &L.r})
       if ((rc = alloc(c, *b, L.nn * 3 * 8))) return rc;
     if ((rc = alloc(c, L.binv, L.nn * 9 * 8))) return rc;
-    if (l > 0) {  // host copies only needed to build the next level
-      g->lv[l - 1].h_ptr.clear();
-      g->lv[l - 1].h_ptr.shrink_to_fit();
-      g->lv[l - 1].h_idx.clear();
-      g->lv[l - 1].h_idx.shrink_to_fit();
+    if (l == 0 && FEDD_MG_FP32_FINE && (rc = alloc(c, L.vals32, L.nblk * 9 * 4))) return rc;
+    if (l > 0) {  // host copies are only needed for the upload
+      L.h_ptr.clear();
+      L.h_ptr.shrink_to_fit();
+      L.h_idx.clear();
+      L.h_idx.shrink_to_fit();
     }
   }
   g->ncoarse = 3 * g->lv.back().nn;
@@ -655,6 +636,9 @@ static int refresh_eager(Ctx* c) {
     }
     k_mg_block_diag<<<nbk(L.nn, 256), 256, 0, c->stream>>>(L.nn, L.d_nptr, L.d_nidx, L.d_vals,
                                                            L.binv.as<double>());
+    if (L.vals32.p)
+      k_mg_to_f32<<<(unsigned)std::min<int64_t>(nbk(L.nblk * 9, 256), (int64_t)c->num_cu * 8), 256, 0, c->stream>>>(
+          L.nblk * 9, L.d_vals, L.vals32.as<float>());
     FEDD_HIP(c, hipGetLastError());
   }
   const MgLevel& Cl = g->lv.back();
@@ -680,6 +664,18 @@ static int refresh_eager(Ctx* c) {
   return FEDD_OK;
 }
 
+template <int G, class VT>
+static void launch_sweep(hipStream_t st, unsigned gs, int mode, const MgLevel& L, const VT* vals, const double* b,
+                         const double* xin, double* out, double w) {
+  const double* bi = L.binv.as<double>();
+  if (mode == 0)
+    k_mg_sweep<G, 0, VT><<<gs, 256, 0, st>>>(L.nn, L.d_nptr, L.d_nidx, vals, bi, b, xin, out, w);
+  else if (mode == 1)
+    k_mg_sweep<G, 1, VT><<<gs, 256, 0, st>>>(L.nn, L.d_nptr, L.d_nidx, vals, bi, b, xin, out, w);
+  else
+    k_mg_sweep<G, 2, VT><<<gs, 256, 0, st>>>(L.nn, L.d_nptr, L.d_nidx, vals, bi, b, xin, out, w);
+}
+
 static void vcycle(Ctx* c, size_t l, const double* b, double* x) {
   MgHost* g = c->mg;
   MgLevel& L = g->lv[l];
@@ -690,27 +686,12 @@ static void vcycle(Ctx* c, size_t l, const double* b, double* x) {
   const double w = FEDD_MG_OMEGA;
   const unsigned gs = l == 0 ? nbk(L.nn * 32, 256) : nbk(L.nn * 16, 256);
   auto sweep = [&](int mode, const double* xin, double* out) {
-    if (l == 0) {
-      if (mode == 0)
-        k_mg_sweep<32, 0><<<gs, 256, 0, c->stream>>>(L.nn, L.d_nptr, L.d_nidx, L.d_vals, L.binv.as<double>(), b,
-                                                      xin, out, w);
-      else if (mode == 1)
-        k_mg_sweep<32, 1><<<gs, 256, 0, c->stream>>>(L.nn, L.d_nptr, L.d_nidx, L.d_vals, L.binv.as<double>(), b,
-                                                      xin, out, w);
-      else
-        k_mg_sweep<32, 2><<<gs, 256, 0, c->stream>>>(L.nn, L.d_nptr, L.d_nidx, L.d_vals, L.binv.as<double>(), b,
-                                                      xin, out, w);
-    } else {
-      if (mode == 0)
-        k_mg_sweep<16, 0><<<gs, 256, 0, c->stream>>>(L.nn, L.d_nptr, L.d_nidx, L.d_vals, L.binv.as<double>(), b,
-                                                      xin, out, w);
-      else if (mode == 1)
-        k_mg_sweep<16, 1><<<gs, 256, 0, c->stream>>>(L.nn, L.d_nptr, L.d_nidx, L.d_vals, L.binv.as<double>(), b,
-                                                      xin, out, w);
-      else
-        k_mg_sweep<16, 2><<<gs, 256, 0, c->stream>>>(L.nn, L.d_nptr, L.d_nidx, L.d_vals, L.binv.as<double>(), b,
-                                                      xin, out, w);
-    }
+    if (l == 0 && L.vals32.p)
+      launch_sweep<32>(c->stream, gs, mode, L, L.vals32.as<float>(), b, xin, out, w);
+    else if (l == 0)
+      launch_sweep<32>(c->stream, gs, mode, L, (const double*)L.d_vals, b, xin, out, w);
+    else
+      launch_sweep<16>(c->stream, gs, mode, L, (const double*)L.d_vals, b, xin, out, w);
   };
   // The sweeps alternate between x and the level's spare buffer.  Starting the
   // pre-smoothing in the spare buffer makes the nu post-sweeps end in x for any nu.
@@ -770,6 +751,18 @@ int mg_refresh(Ctx* c) {
     if (rc) return rc;
   }
   FEDD_HIP(c, hipGraphLaunch(g->rexec, c->stream));
+  return FEDD_OK;
+}
+
+int mg_refresh_fine(Ctx* c) {
+  MgHost* g = c->mg;
+  if (!g) return fail(c, FEDD_ERR_STATE, "multigrid not set up");
+  MgLevel& L = g->lv[0];
+  k_mg_block_diag<<<nbk(L.nn, 256), 256, 0, c->stream>>>(L.nn, L.d_nptr, L.d_nidx, L.d_vals, L.binv.as<double>());
+  if (L.vals32.p)
+    k_mg_to_f32<<<(unsigned)std::min<int64_t>(nbk(L.nblk * 9, 256), (int64_t)c->num_cu * 8), 256, 0, c->stream>>>(
+        L.nblk * 9, L.d_vals, L.vals32.as<float>());
+  FEDD_HIP(c, hipGetLastError());
   return FEDD_OK;
 }
 

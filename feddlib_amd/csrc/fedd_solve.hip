@@ -390,13 +390,15 @@ int pcg_solve(Ctx* c, double rtol, double atol, int maxit, int* iters, double* r
 // bnorm_known > 0: ||b|| is known to the caller (Newton: ||R||), which saves a
 // synchronisation.  The convergence test reads r.r of iteration t-1 from pinned host
 // memory while iteration t is already queued (at most one iteration beyond the target).
-int pcg_mg_solve(Ctx* c, double rtol, double atol, int maxit, int* iters, double* relres, bool refresh,
+// refresh: 0 keep the hierarchy, 1 rebuild it for the current A, 2 finest level only.
+int pcg_mg_solve(Ctx* c, double rtol, double atol, int maxit, int* iters, double* relres, int refresh,
                  double bnorm_known) {
   const Block& A = c->blk[0];
   const int64_t n = A.nrows, nnode = n / 3;
   int rc;
   if ((rc = mg_setup(c))) return rc;
-  if (refresh && (rc = mg_refresh(c))) return rc;
+  if (refresh == 1 && (rc = mg_refresh(c))) return rc;
+  if (refresh == 2 && (rc = mg_refresh_fine(c))) return rc;
   for (int k = 0; k < 4; ++k)
     if ((rc = ensure(c, c->d_kw[k], n * 8))) return rc;
   if ((rc = ensure(c, c->d_scal, PCG_SCAL * 8))) return rc;
@@ -576,7 +578,7 @@ int fedd_linear_solve(fedd_ctx* cc, double rtol, int maxit, int* iters, double* 
   if (c->graph_kind != FEDD_GRAPH_VECTOR) return fail(c, FEDD_ERR_STATE, "needs a vector graph");
   if (c->vec_n[FEDD_VEC_B] != c->blk[0].nrows) return fail(c, FEDD_ERR_STATE, "right-hand side not set");
   (void)hipSetDevice(c->device);
-  if (c->precond == 1) return pcg_mg_solve(c, rtol, 0.0, maxit, iters, relres, true, 0.0);
+  if (c->precond == 1) return pcg_mg_solve(c, rtol, 0.0, maxit, iters, relres, 1, 0.0);
   return pcg_solve(c, rtol, 0.0, maxit, iters, relres);
 }
 
@@ -641,6 +643,8 @@ int fedd_newton_elasticity(fedd_ctx* cc, const fedd_newton_params* P, fedd_newto
     bool surrogate = P->tangent == 1;
     double prev = INFINITY, norm = 0.0;
     bool converged = false;
+    bool coarse_current = false;  // coarse levels rebuilt during this load step
+    int base_iters = 0, last_iters = 0;
     for (int it = 0; it <= P->max_newton; ++it) {
       FEDD_HIP(c, hipEventRecord(ea, c->stream));
       if ((rc = launch_elasticity(c, model, P->E, P->nu, P->precision, 1, surrogate ? 2 : 1))) return rc;
@@ -689,15 +693,25 @@ int fedd_newton_elasticity(fedd_ctx* cc, const fedd_newton_params* P, fedd_newto
       eta = std::max(eta, 0.1 * P->abs_tol / std::max(norm, 1e-300));
       int iters = 0;
       double relres = 0.0;
+      // mg_reuse: within a load step the coarse levels keep the operators of the step's
+      // first tangent until a solve needs more than 2 n0 + 2 iterations (n0: iterations
+      // of the solve right after the rebuild); the finest level always follows A.
+      const int refresh =
+          P->mg_reuse && coarse_current && last_iters <= 2 * base_iters + 2 ? 2 : 1;
       FEDD_HIP(c, hipEventRecord(sa, c->stream));
       if (P->preconditioner == 1)
-        rc = pcg_mg_solve(c, eta, 0.0, P->lin_maxit, &iters, &relres, true, norm);  // b = -R, ||b|| = norm
+        rc = pcg_mg_solve(c, eta, 0.0, P->lin_maxit, &iters, &relres, refresh, norm);  // b = -R, ||b|| = norm
       else
         rc = pcg_solve(c, eta, 0.0, P->lin_maxit, &iters, &relres);
       if (rc) return rc;
       FEDD_HIP(c, hipEventRecord(sb, c->stream));
       solve_pending = true;  // resolved after the next synchronisation
       st->krylov_iters += iters;
+      if (refresh == 1) {
+        base_iters = iters;
+        coarse_current = true;
+      }
+      last_iters = iters;
       FEDD_HIP(c, hipMemcpyAsync(u_prev.p, u, n * 8, hipMemcpyDeviceToDevice, c->stream));
       k_axpy<<<nb(n, 256), 256, 0, c->stream>>>(n, 1.0, c->vec[FEDD_VEC_X].as<double>(), u);
       FEDD_HIP(c, hipGetLastError());

@@ -37,7 +37,7 @@ class NewtonParams(ctypes.Structure):
                 ("load_steps", ctypes.c_int), ("abs_tol", ctypes.c_double), ("max_newton", ctypes.c_int),
                 ("lin_rtol", ctypes.c_double), ("lin_maxit", ctypes.c_int), ("tangent", ctypes.c_int),
                 ("precision", ctypes.c_int), ("stall_ratio", ctypes.c_double), ("collect_samples", ctypes.c_int),
-                ("preconditioner", ctypes.c_int)]
+                ("preconditioner", ctypes.c_int), ("mg_reuse", ctypes.c_int)]
 
 
 class NewtonStats(ctypes.Structure):
@@ -336,9 +336,9 @@ class Context:
 
     def newton_elasticity(self, model=1, E=1000.0, nu=0.3, load_steps=5, abs_tol=1e-8, max_newton=30,
                           lin_rtol=1e-10, lin_maxit=20000, tangent=0, precision=FP64, stall_ratio=0.5,
-                          history_cap=512, collect_samples=False, preconditioner=0):
+                          history_cap=512, collect_samples=False, preconditioner=0, mg_reuse=False):
         P = NewtonParams(model, E, nu, load_steps, abs_tol, max_newton, lin_rtol, lin_maxit, tangent, precision,
-                         stall_ratio, 1 if collect_samples else 0, preconditioner)
+                         stall_ratio, 1 if collect_samples else 0, preconditioner, 1 if mg_reuse else 0)
         st = NewtonStats()
         hist = np.zeros(history_cap)
         self._chk(self._L.fedd_newton_elasticity(self.h, ctypes.byref(P), ctypes.byref(st), _p(hist, _dp),

@@ -97,3 +97,25 @@ def test_mg_needs_the_lattice(gpu_ctx):
         assert e.value.code == -3
     finally:
         gpu_ctx.set_preconditioner(0)
+
+
+def test_mg_reuse_of_coarse_levels(gpu_ctx):
+    """mg_reuse keeps the coarse levels within a load step: the same converged
+    displacement, every Newton step still solved to its tolerance."""
+    pts, el, dofs = _problem(6)
+    t = np.array([40.0, 0.0, 60.0])
+    out = {}
+    for reuse in (False, True):
+        gpu_ctx.mesh_upload(N.MESH_PRIMARY, pts, el)
+        gpu_ctx.graph_build(N.GRAPH_VECTOR)
+        gpu_ctx.vector_upload(N.VEC_U, np.zeros(3 * pts.shape[0]))
+        gpu_ctx.dirichlet_set(dofs)
+        gpu_ctx.traction_load(2, 1.0, t)
+        st, h = gpu_ctx.newton_elasticity(model=1, E=EMOD, nu=NU, load_steps=3, abs_tol=1e-8, preconditioner=1,
+                                          mg_reuse=reuse)
+        assert st["converged"] == 1 and st["final_residual"] < 1e-8
+        out[reuse] = (st, h, gpu_ctx.vector_download(N.VEC_U, 3 * pts.shape[0]))
+    u0, u1 = out[False][2], out[True][2]
+    assert np.max(np.abs(u1 - u0)) <= 1e-7 * np.max(np.abs(u0))
+    assert out[True][0]["newton_iters"] == out[False][0]["newton_iters"]
+    assert out[True][0]["krylov_iters"] <= 2 * out[False][0]["krylov_iters"]
