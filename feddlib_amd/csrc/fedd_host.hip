@@ -61,17 +61,17 @@ void ktimer_resolve(Ctx* c) {
   c->kpending.clear();
 }
 
-void ktimer_begin(Ctx* c) {
+void ktimer_begin(Ctx* c, hipStream_t s) {
   if (!c->timing || c->capturing) return;
   if (c->kpending.size() >= 4096) ktimer_resolve(c);
   c->kopen = kevent(c);
-  (void)hipEventRecord(c->kopen, c->stream);
+  (void)hipEventRecord(c->kopen, s ? s : c->stream);
 }
 
-void ktimer_end(Ctx* c, const char* name) {
+void ktimer_end(Ctx* c, const char* name, hipStream_t s) {
   if (!c->timing || c->capturing || !c->kopen) return;
   hipEvent_t b = kevent(c);
-  (void)hipEventRecord(b, c->stream);
+  (void)hipEventRecord(b, s ? s : c->stream);
   c->kpending.push_back({name, c->kopen, b});
   c->kopen = nullptr;
 }
@@ -375,7 +375,13 @@ int fedd_ctx_create(int device, fedd_ctx** out) {
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
       c->num_cu = prop.multiProcessorCount;
   }
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->side) (void)hipStreamDestroy(c->side);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     delete c;
     return FEDD_ERR_HIP;
   }
@@ -393,6 +399,7 @@ void fedd_ctx_destroy(fedd_ctx* c) {
     if (g) (void)hipGraphDestroy(g);
   }
   (void)hipStreamSynchronize(c->stream);
+  (void)hipStreamSynchronize(c->side);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
   mlp_free(c);
@@ -402,6 +409,9 @@ void fedd_ctx_destroy(fedd_ctx* c) {
   for (auto& e : c->ev_rr)
     if (e) (void)hipEventDestroy(e);
   if (c->h_rr) (void)hipHostFree(c->h_rr);
+  (void)hipEventDestroy(c->ev_fork);
+  (void)hipEventDestroy(c->ev_join);
+  (void)hipStreamDestroy(c->side);
   ktimer_resolve(c);
   for (auto e : c->kpool) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(c->stream);

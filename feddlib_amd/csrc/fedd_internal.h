@@ -96,6 +96,10 @@ struct Ctx {
   int device = 0;
   int num_cu = 256;  // compute units of the device (hipDeviceProp_t::multiProcessorCount)
   hipStream_t stream = nullptr;
+  // second stream for independent work that overlaps the main stream (fork/join by
+  // events; default priority like the main stream)
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::string err;
   Mesh mesh[2];
   int graph_kind = GK_NONE;
@@ -165,8 +169,9 @@ int upload(Ctx* c, DevBuf& b, const void* host, size_t bytes);
 
 // Kernel-timing bracket: events recorded on the context stream around a launch
 // and resolved lazily (no synchronisation inside the timed region).
-void ktimer_begin(Ctx* c);
-void ktimer_end(Ctx* c, const char* name);
+// Kernel timers (events around a launch) on the context stream, or on stream s.
+void ktimer_begin(Ctx* c, hipStream_t s = nullptr);
+void ktimer_end(Ctx* c, const char* name, hipStream_t s = nullptr);
 void ktimer_resolve(Ctx* c);
 
 // Launchers implemented in the .hip files -------------------------------------

@@ -879,6 +879,23 @@ int launch_ns_assembly(Ctx* c, const fedd_ns_params* p, int mode, int reassemble
     const unsigned grid = (unsigned)(FEDD_ROW_BLOCKS_PER_CU > 0 ? std::min<int64_t>(c->vplan.nchunks,
                                                                                  (int64_t)c->num_cu * FEDD_ROW_BLOCKS_PER_CU)
                                                                 : c->vplan.nchunks);
+    // B does not depend on the iterate: its rows are assembled on the side stream,
+    // overlapping the velocity rows.
+    const bool fork = !reassemble;
+    if (fork) {
+      NsArgs b = ns_args(c, &q, c->pplan);
+      const size_t ldsb = (size_t)c->pplan.lds_doubles * 8;
+      FEDD_HIP(c, hipEventRecord(c->ev_fork, c->stream));
+      FEDD_HIP(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
+      ktimer_begin(c, c->side);
+      if (f32)
+        k_ns_pres_rows<float><<<(unsigned)c->pplan.nchunks, 256, ldsb, c->side>>>(b);
+      else
+        k_ns_pres_rows<double><<<(unsigned)c->pplan.nchunks, 256, ldsb, c->side>>>(b);
+      FEDD_HIP(c, hipGetLastError());
+      ktimer_end(c, "ns_pres_rows", c->side);
+      FEDD_HIP(c, hipEventRecord(c->ev_join, c->side));
+    }
     ktimer_begin(c);
     k_pad_u<<<nblk(n2, 256), 256, 0, c->stream>>>(n2, a.u, c->d_upad.as<double>());
     FEDD_HIP(c, hipGetLastError());
@@ -890,17 +907,7 @@ int launch_ns_assembly(Ctx* c, const fedd_ns_params* p, int mode, int reassemble
       k_ns_vel_rows<double><<<grid, FEDD_ROW_BLOCK, lds, c->stream>>>(a);
     FEDD_HIP(c, hipGetLastError());
     ktimer_end(c, "ns_vel_rows");
-    if (!reassemble) {
-      NsArgs b = ns_args(c, &q, c->pplan);
-      const size_t ldsb = (size_t)c->pplan.lds_doubles * 8;
-      ktimer_begin(c);
-      if (f32)
-        k_ns_pres_rows<float><<<(unsigned)c->pplan.nchunks, 256, ldsb, c->stream>>>(b);
-      else
-        k_ns_pres_rows<double><<<(unsigned)c->pplan.nchunks, 256, ldsb, c->stream>>>(b);
-      FEDD_HIP(c, hipGetLastError());
-      ktimer_end(c, "ns_pres_rows");
-    }
+    if (fork) FEDD_HIP(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
   } else {
     // The Newton residual stays FP64 in both precision modes (FP32-local applies to
     // the element matrices only).
