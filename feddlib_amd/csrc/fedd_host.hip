@@ -557,8 +557,6 @@ int fedd_graph_build(fedd_ctx* c, int kind) {
     vs.rows_per_node = 3;
     vs.comps_per_col = 3;
   }
-  if (kind == FEDD_GRAPH_VECTOR && m.npe != 10)
-    return fail(c, FEDD_ERR_ARG, "the vector graph needs a P2 mesh (tet-10 elasticity)");
   if (kind == FEDD_GRAPH_NS) {
     vs.second = &g.bt;
     vs.second_rows = 3;
@@ -925,6 +923,7 @@ static int elasticity_common(fedd_ctx* c, int model, double E, double nu, int pr
                              const double* Evec = nullptr) {
   if (!c) return FEDD_ERR_ARG;
   if (c->graph_kind != FEDD_GRAPH_VECTOR) return fail(c, FEDD_ERR_STATE, "needs a vector (P2) graph");
+  if (c->mesh[0].npe != 10) return fail(c, FEDD_ERR_STATE, "the hyperelastic elements need a P2 (tet-10) mesh");
   if (precision != FEDD_FP64 && precision != FEDD_FP32_LOCAL) return fail(c, FEDD_ERR_ARG, "bad precision");
   if (!(E > 0) || !(nu > -1.0 && nu < 0.5)) return fail(c, FEDD_ERR_ARG, "E must be > 0 and -1 < nu < 0.5");
   if (c->vec_n[FEDD_VEC_U] != 3 * c->mesh[0].npts) return fail(c, FEDD_ERR_STATE, "displacement not uploaded");
@@ -942,6 +941,15 @@ static int elasticity_common(fedd_ctx* c, int model, double E, double nu, int pr
 
 int fedd_assembly_linear_elasticity(fedd_ctx* c, double E, double nu, int precision) {
   return elasticity_common(c, 0, E, nu, precision, 0);
+}
+
+int fedd_assembly_linear_elasticity_xdim(fedd_ctx* c, double lambda, double mu, int precision) {
+  if (!c) return FEDD_ERR_ARG;
+  if (c->graph_kind != FEDD_GRAPH_VECTOR) return fail(c, FEDD_ERR_STATE, "needs a vector graph");
+  if (precision != FEDD_FP64 && precision != FEDD_FP32_LOCAL) return fail(c, FEDD_ERR_ARG, "bad precision");
+  if (!(mu > 0) || !(lambda > -2.0 / 3.0 * mu)) return fail(c, FEDD_ERR_ARG, "need mu > 0 and lambda > -2 mu / 3");
+  (void)hipSetDevice(c->device);
+  return launch_linelas(c, lambda, mu, precision);
 }
 
 int fedd_assembly_nonlinear_elasticity(fedd_ctx* c, int neo_hooke_model, double E, double nu, int precision) {

@@ -180,6 +180,7 @@ int expand_graph(Ctx* c, int kind, const int32_t* aidx, const int32_t* btidx, co
 int launch_ns_assembly(Ctx* c, const fedd_ns_params* p, int mode, int reassemble, int precision);
 int launch_laplace(Ctx* c, int precision);
 int launch_rhs(Ctx* c, double f);
+int launch_linelas(Ctx* c, double lam, double mu, int precision);
 int launch_stream_copy(Ctx* c, int64_t bytes, int iters, float* ms);
 int launch_geometry(Ctx* c);
 // tangent: 1 = exact law, 2 = learned surrogate (the network writes the tangent slot)

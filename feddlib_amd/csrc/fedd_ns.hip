@@ -199,6 +199,7 @@ struct NsTensors<0> {
   static constexpr const double (&TT)[3][10][10] = ft::T_V;
   static constexpr const double (&L)[4][10] = ft::L_V;
   static constexpr const double (&RV)[3][4] = ft::RV_V;
+  static constexpr const double (&RG)[3][3][10] = ft::RG_V;
   static constexpr int row = 0;
 };
 template <>
@@ -207,6 +208,7 @@ struct NsTensors<1> {
   static constexpr const double (&TT)[3][10][10] = ft::T_E;
   static constexpr const double (&L)[4][10] = ft::L_E;
   static constexpr const double (&RV)[3][4] = ft::RV_E;
+  static constexpr const double (&RG)[3][3][10] = ft::RG_E;
   static constexpr int row = 4;
 };
 
@@ -765,6 +767,88 @@ __global__ __launch_bounds__(256) void k_laplace_rows(LapArgs a) {
   for (int k = threadIdx.x; k < len; k += blockDim.x) a.vals[na0 + k] = lds[k];
 }
 
+// FE::assemblyLinElasXDim (FE_def.hpp:4701-5004), 3D: the element entry for rows (r, a),
+// columns (c, b) is int 2 mu eps(phi_r e_a) : eps(phi_c e_b) + lambda div div
+//   = |det| (mu delta_ab tr M + mu M[b][a] + lambda M[a][b]),
+// M[p][q] = int_ref d_p phi_r d_q phi_c after the affine map (exact reference tensors;
+// the reference's 1-point P1 and 5-point P2 rules integrate these products exactly).
+template <typename T, int NPE, int CLS>
+__device__ __forceinline__ void linelas_row(const LapArgs& a, int32_t e, const uint8_t* m, double* __restrict__ row,
+                                            int deg, T lam, T mu) {
+  const int li = m[0];
+  int32_t nodes[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+    nodes[c] = a.conn[(int64_t)(NPE == 4 ? c_perm1[li][c] : c_perm2[li][c]) * a.nelem + e];
+  Geo g;
+  geometry(a.xyz, a.npts, nodes, g);
+  T Bi[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Bi[i][j] = (T)g.Bi[i][j];
+#pragma unroll
+  for (int c = 0; c < NPE; ++c) {
+    T t[3][3];  // t[ax][q] = sum_ay RG[ax][ay][c] Bi[ay][q]
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        T v = 0;
+#pragma unroll
+        for (int ay = 0; ay < 3; ++ay) {
+          const double r = NPE == 4 ? ft::RG1[ax][ay][c] : NsTensors<CLS>::RG[ax][ay][c];
+          if (r != 0.0) v += (T)r * Bi[ay][q];
+        }
+        t[ax][q] = v;
+      }
+    T M[3][3];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) M[p][q] = Bi[0][p] * t[0][q] + Bi[1][p] * t[1][q] + Bi[2][p] * t[2][q];
+    const T tr = M[0][0] + M[1][1] + M[2][2];
+#pragma unroll
+    for (int d1 = 0; d1 < 3; ++d1)
+#pragma unroll
+      for (int d2 = 0; d2 < 3; ++d2) {
+        const T v = lam * M[d1][d2] + mu * M[d2][d1] + (d1 == d2 ? mu * tr : (T)0);
+        atomicAdd(row + d1 * 3 * deg + 3 * m[2 + c] + d2, (double)v * g.adet);
+      }
+  }
+}
+
+template <typename T, int NPE>
+__global__ __launch_bounds__(256) void k_linelas_rows(LapArgs a, double lam, double mu) {
+  extern __shared__ double lds[];
+  __shared__ int2 tab[256];  // per chunk node: row offset (9 per node block), degree
+  const int ch = (int)xcd_chunk(blockIdx.x, gridDim.x);
+  const int n0 = a.chunk_node[ch], n1 = a.chunk_node[ch + 1];
+  const int q0 = a.chunk_pair[ch], q1 = a.chunk_pair[ch + 1], qs = a.chunk_split[ch];
+  const int64_t na0 = a.nptr[n0];
+  const int len = (int)(9 * (a.nptr[n1] - na0));
+  for (int t = threadIdx.x; t < n1 - n0; t += blockDim.x) {
+    const int64_t ai = a.nptr[n0 + t];
+    tab[t] = make_int2((int)(9 * (ai - na0)), (int)(a.nptr[n0 + t + 1] - ai));
+  }
+  for (int k = threadIdx.x; k < len; k += blockDim.x) lds[k] = 0.0;
+  __syncthreads();
+  for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+    const int32_t e = a.pair_elem[q];
+    const uint4 mv = a.pair_meta[q];
+    const uint8_t* m = reinterpret_cast<const uint8_t*>(&mv);
+    const int2 nd = tab[m[1]];
+    if (NPE == 4)
+      linelas_row<T, 4, 0>(a, e, m, lds + nd.x, nd.y, (T)lam, (T)mu);
+    else if (q < qs)
+      linelas_row<T, 10, 0>(a, e, m, lds + nd.x, nd.y, (T)lam, (T)mu);
+    else
+      linelas_row<T, 10, 1>(a, e, m, lds + nd.x, nd.y, (T)lam, (T)mu);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < len; k += blockDim.x) __builtin_nontemporal_store(lds[k], a.vals + 9 * na0 + k);
+}
+
 __global__ __launch_bounds__(256) void k_rhs_rows(LapArgs a) {
   extern __shared__ double lds[];
   const int ch = (int)xcd_chunk(blockIdx.x, gridDim.x);
@@ -952,6 +1036,28 @@ int launch_laplace(Ctx* c, int precision) {
     k_laplace_rows<double><<<(unsigned)c->vplan.nchunks, 256, lds, c->stream>>>(a);
   FEDD_HIP(c, hipGetLastError());
   ktimer_end(c, "laplace_rows");
+  return FEDD_OK;
+}
+
+int launch_linelas(Ctx* c, double lam, double mu, int precision) {
+  LapArgs a = lap_args(c);
+  const size_t lds = (size_t)c->vplan.lds_doubles * 8;
+  const bool p1 = c->mesh[0].npe == 4;
+  const unsigned g = (unsigned)c->vplan.nchunks;
+  ktimer_begin(c);
+  if (precision == FEDD_FP32_LOCAL) {
+    if (p1)
+      k_linelas_rows<float, 4><<<g, 256, lds, c->stream>>>(a, lam, mu);
+    else
+      k_linelas_rows<float, 10><<<g, 256, lds, c->stream>>>(a, lam, mu);
+  } else {
+    if (p1)
+      k_linelas_rows<double, 4><<<g, 256, lds, c->stream>>>(a, lam, mu);
+    else
+      k_linelas_rows<double, 10><<<g, 256, lds, c->stream>>>(a, lam, mu);
+  }
+  FEDD_HIP(c, hipGetLastError());
+  ktimer_end(c, "linelas_rows");
   return FEDD_OK;
 }
 

@@ -601,7 +601,8 @@ int fedd_newton_elasticity(fedd_ctx* cc, const fedd_newton_params* P, fedd_newto
                            double* history, int history_cap) {
   Ctx* c = cc;
   if (!c || !P || !st) return FEDD_ERR_ARG;
-  if (c->graph_kind != FEDD_GRAPH_VECTOR) return fail(c, FEDD_ERR_STATE, "needs a vector (P2) graph");
+  if (c->graph_kind != FEDD_GRAPH_VECTOR || c->mesh[0].npe != 10)
+    return fail(c, FEDD_ERR_STATE, "needs a vector graph on a P2 (tet-10) mesh");
   if (P->load_steps < 1 || P->max_newton < 1 || !(P->abs_tol > 0) || P->preconditioner < 0 ||
       P->preconditioner > 1)
     return fail(c, FEDD_ERR_ARG, "bad Newton parameters");

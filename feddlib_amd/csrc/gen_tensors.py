@@ -118,6 +118,9 @@ def main():
         # Newton term: L[m][j] = int lambda_m phi_r phi_j
         l = [(P1[m] * P2[r] * P2[j]).integral() for m in range(4) for j in range(10)]
         print(arr("L_" + cls, l, (4, 10)))
+        # Elasticity: RG[a][b][j] = int d_a phi_r d_b phi_j (reference gradients, P2)
+        rg = [(dP2[r][a] * dP2[j][b]).integral() for a in range(3) for b in range(3) for j in range(10)]
+        print(arr("RG_" + cls, rg, (3, 3, 10)))
         # Divergence, velocity row r: R[a][m] = int psi_m d_a phi_r
         rr = [(P1[m] * dP2[r][a]).integral() for a in range(3) for m in range(4)]
         print(arr("RV_" + cls, rr, (3, 4)))
@@ -138,6 +141,9 @@ def main():
                 v += (dP1[0][b] * dP1[j][a]).integral()
             ks1.append(v)
     print(arr("KS1", ks1, (6, 4)))
+    # P1 elasticity row 0: RG1[a][b][j] = int d_a lambda_0 d_b lambda_j
+    rg1 = [(dP1[0][a] * dP1[j][b]).integral() for a in range(3) for b in range(3) for j in range(4)]
+    print(arr("RG1", rg1, (3, 3, 4)))
     # Volume integrals of the basis (loads): MV2[k] = int phi_k (P2), MV1 = int lambda (P1)
     print(arr("MV2", [p.integral() for p in P2], (10,)))
     print(arr("MV1", [p.integral() for p in P1], (4,)))

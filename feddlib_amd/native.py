@@ -77,6 +77,7 @@ _SIGS = {
     "fedd_ns_merged_info": (ctypes.c_int, [_vp, _lp, _lp]),
     "fedd_ns_merged_export": (ctypes.c_int, [_vp, _lp, _ip, _dp]),
     "fedd_write_matrix_market": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_char_p]),
+    "fedd_assembly_linear_elasticity_xdim": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_int]),
     "fedd_hdf5_read_vector": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _dp, ctypes.c_int64, _lp,
                                              ctypes.POINTER(ctypes.c_int)]),
     "fedd_colour_build": (ctypes.c_int, [_vp, _ip]),
@@ -306,6 +307,9 @@ class Context:
 
     def assembly_linear_elasticity(self, E, nu, precision=FP64):
         self._chk(self._L.fedd_assembly_linear_elasticity(self.h, E, nu, precision))
+
+    def assembly_linear_elasticity_xdim(self, lam, mu, precision=FP64):
+        self._chk(self._L.fedd_assembly_linear_elasticity_xdim(self.h, lam, mu, precision))
 
     def assembly_nonlinear_elasticity(self, neo_hooke_model, E, nu, precision=FP64):
         self._chk(self._L.fedd_assembly_nonlinear_elasticity(self.h, neo_hooke_model, E, nu, precision))

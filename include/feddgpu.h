@@ -180,6 +180,11 @@ int fedd_assembly_navier_stokes(fedd_ctx* ctx, const fedd_ns_params* params, int
  * the displacement VEC_U; element matrix = -s (AssembleFE_LinElas_def.hpp:109-119).
  * Fills A of a GRAPH_VECTOR graph on a P2 mesh.  Parameter/E, Parameter/Poisson Ratio. */
 int fedd_assembly_linear_elasticity(fedd_ctx* ctx, double E, double nu, int precision);
+/* FE::assemblyLinElasXDim (FE_def.hpp:4701-5004), 3D, as LinElas::assemble calls it
+ * (LinElas_def.hpp:64-84): the linear operator int 2 mu eps(u):eps(v) + lambda div u div v
+ * into A of a GRAPH_VECTOR graph on a P1 or P2 mesh (1-point / 5-point rules, both exact
+ * here).  Needs no displacement. */
+int fedd_assembly_linear_elasticity_xdim(fedd_ctx* ctx, double lambda, double mu, int precision);
 /* FE::assemblyNonLinearElasticity (FE_def.hpp:449-523): tangent into A and residual
  * (rhs = p of the element, AssembleFE_NonLinElas_def.hpp:131-173) into VEC_RES_U.
  * neo_hooke_model = Parameter/"Neo-Hooke Modell": 1 -> skr2, otherwise skr3. */

@@ -15,6 +15,7 @@ SRC = "/root/reference/feddlib/problems/tests/UnitTest/ReferenceSolutions"
 FILES = [
     "solution_laplace_3d_P1_8cores.h5",        # laplaceUnitTest.cpp, dim 3, P1, m=6 on 2x2x2 ranks
     "solution_laplace_3d_P2_8cores.h5",        # laplaceUnitTest.cpp, dim 3, P2
+    "solution_linElas_3d_P1_8cores.h5",        # linElasUnitTest.cpp, dim 3, P1, m=4
     "solution_linElas_3d_P2_8cores.h5",        # linElasUnitTest.cpp, dim 3, P2, m=4
     "solution_stokes_velocity_3d_P2_8cores.h5",  # stokesUnitTest.cpp, dim 3, P2-P1, m=3
     "solution_stokes_pressure_3d_P2_8cores.h5",

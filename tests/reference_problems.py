@@ -28,10 +28,12 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref
 
 LAPLACE = {1: ("solution_laplace_3d_P1_8cores.h5", 12), 2: ("solution_laplace_3d_P2_8cores.h5", 12)}
 LINELAS = ("solution_linElas_3d_P2_8cores.h5", 8)
+LINELAS_FILES = {1: "solution_linElas_3d_P1_8cores.h5", 2: "solution_linElas_3d_P2_8cores.h5"}
 STOKES = ("solution_stokes_velocity_3d_P2_8cores.h5", "solution_stokes_pressure_3d_P2_8cores.h5", 6)
 
 LINELAS_MU, LINELAS_NU, LINELAS_FORCE = 3.0, 0.4, 0.5
 LINELAS_E = LINELAS_MU * 2.0 * (1.0 + LINELAS_NU)  # LinElas_def.hpp:75
+LINELAS_LAMBDA = LINELAS_NU * LINELAS_E / ((1.0 + LINELAS_NU) * (1.0 - 2.0 * LINELAS_NU))  # LinElas_def.hpp:76
 STOKES_NU, STOKES_RHO = 0.01, 1.0
 
 

@@ -97,3 +97,65 @@ def test_gpu_stokes_reproduces_reference_solution(gpu_ctx, prec):
     tol = 1e-9 if prec == N.FP64 else 1e-5  # stokesUnitTest.cpp:230
     assert np.abs(x[:nv] - gv).max() <= tol * np.abs(gv).max()
     assert np.abs(x[nv:] - gp).max() <= tol * np.abs(gp).max()
+
+
+@pytest.mark.parametrize("fe", [1, 2])
+@pytest.mark.parametrize("prec", [N.FP64, N.FP32_LOCAL])
+def test_gpu_linelas_xdim_matches_oracle(gpu_ctx, fe, prec):
+    """fedd_assembly_linear_elasticity_xdim against the oracle's FE::assemblyLinElasXDim,
+    entry by entry on the vector graph (FP64 1e-12 / FP32-local 1e-5 parity rule)."""
+    import linelas as LE
+    from helpers import parity_ratio
+    pts, _, el = N.mesh_structured(3, fe)
+    n = 3 * len(pts)
+    gpu_ctx.mesh_upload(N.MESH_PRIMARY, pts, el)
+    gpu_ctx.graph_build(N.GRAPH_VECTOR)
+    gpu_ctx.assembly_linear_elasticity_xdim(RP.LINELAS_LAMBDA, RP.LINELAS_MU, prec)
+    rp, ci = gpu_ctx.block_graph(N.BLOCK_A)
+    v = gpu_ctx.block_values(N.BLOCK_A)
+    K = LE.linelas_xdim(fe, pts, el, RP.LINELAS_LAMBDA, RP.LINELAS_MU)
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    ref = np.asarray(K[rows, ci]).ravel()
+    assert K.nnz <= len(ci)  # no oracle entry outside the GPU graph
+    tol = dict(rtol=1e-12, atol_row=1e-14) if prec == N.FP64 else dict(rtol=1e-5, atol_row=1e-6)
+    assert parity_ratio(v, ref, rp, **tol) <= 1.0
+
+
+@pytest.mark.parametrize("fe,pc", [(1, 0), (2, 1)])
+def test_gpu_linelas_xdim_reproduces_reference_solution(gpu_ctx, fe, pc):
+    """linElasUnitTest.cpp end to end on the GPU: assemblyLinElasXDim, volume force,
+    Dirichlet rows, CG (block Jacobi for P1, multigrid for P2)."""
+    _, M = RP.LINELAS
+    pts, _, el = N.mesh_structured(M, fe)
+    n = 3 * len(pts)
+    gpu_ctx.mesh_upload(N.MESH_PRIMARY, pts, el)
+    gpu_ctx.graph_build(N.GRAPH_SCALAR)
+    gpu_ctx.assembly_rhs(1.0)
+    bs = gpu_ctx.vector_download(N.VEC_RHS, len(pts))
+    gpu_ctx.graph_build(N.GRAPH_VECTOR)
+    gpu_ctx.assembly_linear_elasticity_xdim(RP.LINELAS_LAMBDA, RP.LINELAS_MU)
+    rows = RP.node_dofs(np.where(RP.flags(pts) == 1)[0]).astype(np.int32)
+    gpu_ctx.dirichlet_set(rows)
+    gpu_ctx.apply_dirichlet()
+    b = np.zeros(n)
+    b[0::3] = RP.LINELAS_FORCE * bs
+    b[rows] = 0.0
+    gpu_ctx.set_preconditioner(pc)
+    try:
+        gpu_ctx.vector_upload(N.VEC_B, b)
+        it, rr = gpu_ctx.linear_solve(1e-13, 5000)
+    finally:
+        gpu_ctx.set_preconditioner(0)
+    u = gpu_ctx.vector_download(N.VEC_X, n)
+    g = N.hdf5_read_vector(RP.golden(RP.LINELAS_FILES[fe]), "solution")
+    assert rr <= 1e-13
+    assert np.abs(u - g).max() <= 1e-11  # linElasUnitTest.cpp tolerance
+
+
+def test_gpu_hyperelastic_needs_p2(gpu_ctx):
+    pts, _, el = N.mesh_structured(2, 1)
+    gpu_ctx.mesh_upload(N.MESH_PRIMARY, pts, el)
+    gpu_ctx.graph_build(N.GRAPH_VECTOR)
+    gpu_ctx.vector_upload(N.VEC_U, np.zeros(3 * len(pts)))
+    with pytest.raises(N.FeddError, match="P2"):
+        gpu_ctx.assembly_nonlinear_elasticity(1, 1000.0, 0.3)

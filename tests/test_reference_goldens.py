@@ -114,3 +114,47 @@ def test_hdf5_reader_survives_corrupted_metadata(tmp_path):
             assert v.ndim == 1
         except N.FeddError:
             pass
+
+
+def test_oracle_linelas_xdim_vectorised_equals_literal():
+    """The vectorised assembly of oracle/linelas.py equals the literal epsilon-tensor
+    element loop of FE::assemblyLinElasXDim on single elements (P1 and P2)."""
+    import linelas as LE
+    for fe in (1, 2):
+        pts, _, el = O.mesh(1, fe)
+        for e in (0, 3, 5):
+            T = el[e]
+            Kl = LE.element_matrix(fe, pts[T], RP.LINELAS_LAMBDA, RP.LINELAS_MU)
+            Kv = LE.linelas_xdim(fe, pts, el[e:e + 1], RP.LINELAS_LAMBDA, RP.LINELAS_MU).toarray()
+            dofs = (3 * T[:, None] + np.arange(3)).ravel()
+            assert np.abs(Kv[np.ix_(dofs, dofs)] - Kl).max() <= 1e-13 * np.abs(Kl).max()
+
+
+def test_oracle_linelas_xdim_equals_hyperelastic_tangent_at_rest():
+    """Two independent restatements: assemblyLinElasXDim (lambda, mu) and the tangent at
+    u = 0 of the AceGen laws with E = 2 mu (1 + nu)."""
+    import linelas as LE
+    pts, _, el = O.mesh(2, 2)
+    el = np.ascontiguousarray(el)
+    O.correct_orientation(pts, el)
+    n = 3 * len(pts)
+    Kx = LE.linelas_xdim(2, pts, el, RP.LINELAS_LAMBDA, RP.LINELAS_MU)
+    for model in (0, 1, 2):
+        K, _ = O.elasticity(model, RP.LINELAS_E, RP.LINELAS_NU, pts, el, np.zeros(n))
+        K = RP.csr(K, (n, n))
+        assert abs(K - Kx).max() <= 1e-12 * abs(Kx).max()
+
+
+@pytest.mark.parametrize("fe", [1, 2])
+def test_oracle_linelas_xdim_reproduces_reference_solution(fe):
+    import linelas as LE
+    _, M = RP.LINELAS
+    pts, _, el = O.mesh(M, fe)
+    n = 3 * len(pts)
+    K = LE.linelas_xdim(fe, pts, el, RP.LINELAS_LAMBDA, RP.LINELAS_MU)
+    b = np.zeros(n)
+    b[0::3] = RP.LINELAS_FORCE * O.rhs(fe, pts, el, 1.0)
+    rows = RP.node_dofs(np.where(RP.flags(pts) == 1)[0])
+    u = RP.solve_with_dirichlet(K, b, rows, 0.0)
+    g = N.hdf5_read_vector(RP.golden(RP.LINELAS_FILES[fe]), "solution")
+    assert np.abs(u - g).max() <= 1e-11  # linElasUnitTest.cpp tolerance
