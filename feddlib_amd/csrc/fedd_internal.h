@@ -195,7 +195,8 @@ bool mlp_matches(const Ctx* c, int model, double Emod, double nu);
 int mlp_collect_qp(Ctx* c);
 // Geometric multigrid (fedd_mg.hip): hierarchy for the current mesh / Dirichlet set,
 // coarse operators for the current A, and one V-cycle x = M^-1 b on the finest level.
-int mg_setup(Ctx* c);
+// mode 0: geometric hierarchy of the structured lattice; 1: aggregation (any mesh).
+int mg_setup(Ctx* c, int mode);
 int mg_refresh(Ctx* c);
 // Finest level only (smoother blocks and FP32 copy); coarse levels keep their operators.
 int mg_refresh_fine(Ctx* c);

@@ -48,11 +48,18 @@ struct MgLevel {
   double* d_vals = nullptr;
   DevBuf binv, free, x, x2, b, r;
   DevBuf vals32;  // finest level: FP32 copy of the operator used by the smoother
+  // aggregation hierarchy: transfer from the next finer level to this one
+  DevBuf agg;         // int32, per finer-level node: its aggregate (a node of this level)
+  DevBuf mptr, midx;  // int64 / int32: members (finer-level nodes) of every node
   std::vector<int64_t> h_ptr;
   std::vector<int32_t> h_idx;
+  std::vector<int32_t> h_agg;
+  std::vector<int64_t> h_mptr;
+  std::vector<int32_t> h_midx;
 };
 
 struct MgHost {
+  int mode = 0;  // 0: lattice (geometric), 1: aggregation (algebraic)
   std::vector<MgLevel> lv;
   DevBuf dense, inv;  // coarsest level: Gauss-Jordan ping-pong buffers (n x n); inverse in inv
   int64_t ncoarse = 0;
@@ -203,6 +210,111 @@ __global__ __launch_bounds__(256) void k_mg_prolong(int64_t Pf, int64_t Pc, cons
     double v = 0.0;
     for (int k = 0; k < n; ++k) v += freec[3 * par[k] + d] ? w[k] * xc[3 * par[k] + d] : 0.0;
     xf[3 * f + d] += v;
+  }
+}
+
+// Frobenius norm of every 3x3 node block of a node-block operator (strength of connection).
+__global__ __launch_bounds__(256) void k_amg_block_norm(int64_t nn, const int64_t* __restrict__ nptr,
+                                                        const double* __restrict__ vals, float* __restrict__ out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  const int64_t k0 = nptr[i];
+  const int deg = (int)(nptr[i + 1] - k0);
+  const double* v = vals + 9 * k0;
+  for (int s = 0; s < deg; ++s) {
+    double a = 0.0;
+#pragma unroll
+    for (int d1 = 0; d1 < 3; ++d1)
+#pragma unroll
+      for (int d2 = 0; d2 < 3; ++d2) {
+        const double x = v[3 * deg * d1 + 3 * s + d2];
+        a += x * x;
+      }
+    out[k0 + s] = (float)sqrt(a);
+  }
+}
+
+// Aggregation transfers (piecewise constant over the aggregates, per dof component):
+// restriction sums the free fine dofs of each aggregate, prolongation adds the
+// aggregate's correction to its free fine dofs.
+__global__ __launch_bounds__(256) void k_amg_restrict(int64_t nc, const int64_t* __restrict__ mptr,
+                                                      const int32_t* __restrict__ midx, const double* __restrict__ rf,
+                                                      const uint8_t* __restrict__ freef,
+                                                      const uint8_t* __restrict__ freec, double* __restrict__ bc) {
+  const int64_t I = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (I >= nc) return;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int64_t k = mptr[I]; k < mptr[I + 1]; ++k) {
+    const int64_t i = midx[k];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) acc[d] += freef[3 * i + d] ? rf[3 * i + d] : 0.0;
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) bc[3 * I + d] = freec[3 * I + d] ? acc[d] : 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_amg_prolong(int64_t nf, const int32_t* __restrict__ agg,
+                                                     const double* __restrict__ xc, const uint8_t* __restrict__ freef,
+                                                     double* __restrict__ xf, double alpha) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nf) return;
+  const int64_t I = agg[i];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    if (freef[3 * i + d]) xf[3 * i + d] += alpha * xc[3 * I + d];
+}
+
+// Galerkin product P^T A P for aggregation (P boolean per dof, masked by the free fine
+// dofs): coarse block (I, J) sums the fine blocks (i, j), i in I, j in J.  One
+// workgroup per coarse row, accumulated in LDS; constrained coarse dofs get identity rows.
+__global__ __launch_bounds__(64) void k_amg_galerkin(int64_t nc, const int64_t* __restrict__ mptr,
+                                                     const int32_t* __restrict__ midx, const int32_t* __restrict__ agg,
+                                                     const int64_t* __restrict__ fptr, const int32_t* __restrict__ fidx,
+                                                     const double* __restrict__ fval, const uint8_t* __restrict__ ffree,
+                                                     const int64_t* __restrict__ cptr, const int32_t* __restrict__ cidx,
+                                                     double* __restrict__ cval, const uint8_t* __restrict__ cfree) {
+  extern __shared__ double acc[];
+  for (int64_t I = blockIdx.x; I < nc; I += gridDim.x) {
+    const int64_t c0 = cptr[I];
+    const int degc = (int)(cptr[I + 1] - c0);
+    const int32_t* crow = cidx + c0;
+    for (int k = threadIdx.x; k < 9 * degc; k += blockDim.x) acc[k] = 0.0;
+    __syncthreads();
+    uint8_t cf[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) cf[d] = cfree[3 * I + d];
+    for (int64_t a = mptr[I]; a < mptr[I + 1]; ++a) {
+      const int64_t f = midx[a];
+      const int64_t f0 = fptr[f];
+      const int degf = (int)(fptr[f + 1] - f0);
+      uint8_t fr[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) fr[d] = ffree[3 * f + d] & cf[d];
+      const double* fv = fval + 9 * f0;
+      for (int k = threadIdx.x; k < degf; k += blockDim.x) {
+        const int32_t j = fidx[f0 + k];
+        const int32_t J = agg[j];
+        const int slot = find_slot(crow, degc, J);
+        if (slot < 0) continue;  // cannot happen: the coarse graph holds every coupling
+#pragma unroll
+        for (int d1 = 0; d1 < 3; ++d1)
+#pragma unroll
+          for (int d2 = 0; d2 < 3; ++d2) {
+            const bool on = fr[d1] && ffree[3 * (int64_t)j + d2] && cfree[3 * (int64_t)J + d2];
+            const double v = on ? fv[3 * degf * d1 + 3 * k + d2] : 0.0;
+            if (v != 0.0) atomicAdd(acc + 3 * degc * d1 + 3 * slot + d2, v);
+          }
+      }
+      __syncthreads();
+    }
+    const int self = find_slot(crow, degc, (int32_t)I);
+    for (int k = threadIdx.x; k < 9 * degc; k += blockDim.x) {
+      const int d1 = k / (3 * degc), q = k - 3 * degc * d1, slot = q / 3, d2 = q - 3 * slot;
+      double v = acc[k];
+      if (!cf[d1] && slot == self && d2 == d1) v = 1.0;
+      cval[9 * c0 + k] = v;
+    }
+    __syncthreads();
   }
 }
 
@@ -530,54 +642,212 @@ static bool lattice_mesh(const Mesh& m, int64_t& P) {
   return ok.load();
 }
 
-int mg_setup(Ctx* c) {
-  if (c->mg) return FEDD_OK;
+// Greedy aggregation of a node graph (rows include the node itself): pass 1 makes every
+// node whose neighbours are all unaggregated the root of a new aggregate {i} + neighbours;
+// pass 2 joins each remaining node to the aggregate of its first pass-1 neighbour; pass 3
+// makes singletons of the rest.  Returns the number of aggregates.
+static int64_t aggregate(int64_t n, const std::vector<int64_t>& ptr, const std::vector<int32_t>& idx,
+                         const std::vector<uint8_t>* strong, std::vector<int32_t>& agg) {
+  auto st = [&](int64_t k) { return !strong || (*strong)[k]; };
+  agg.assign(n, -1);
+  int64_t nc = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    if (agg[i] >= 0) continue;
+    bool free_nb = true;
+    for (int64_t k = ptr[i]; k < ptr[i + 1] && free_nb; ++k) free_nb = !st(k) || agg[idx[k]] < 0;
+    if (!free_nb) continue;
+    for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k)
+      if (st(k)) agg[idx[k]] = (int32_t)nc;
+    agg[i] = (int32_t)nc++;
+  }
+  std::vector<int32_t> first(agg);
+  for (int64_t i = 0; i < n; ++i) {
+    if (agg[i] >= 0) continue;
+    for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k)
+      if (st(k) && first[idx[k]] >= 0) {
+        agg[i] = first[idx[k]];
+        break;
+      }
+  }
+  for (int64_t i = 0; i < n; ++i)
+    if (agg[i] < 0) agg[i] = (int32_t)nc++;
+  return nc;
+}
+
+#define FEDD_AMG_COARSEST 64  // nodes on the coarsest aggregation level (at most)
+// Scaling of the aggregation coarse-grid correction (over-correction compensates the
+// low energy of piecewise-constant interpolation; the V-cycle stays symmetric).
+#ifndef FEDD_AMG_ALPHA
+#define FEDD_AMG_ALPHA 1.8
+#endif
+// Strong coupling on the finest level: ||A_ij||_F >= theta sqrt(||A_ii||_F ||A_jj||_F).
+#ifndef FEDD_AMG_THETA
+#define FEDD_AMG_THETA 0.1
+#endif
+
+// Aggregation hierarchy from the node graph of level 0 (any mesh).
+static int build_aggregation(Ctx* c, MgHost* g, std::vector<std::vector<uint8_t>>& frees) {
+  const int64_t n0 = c->mesh[0].npts;
+  std::vector<int64_t> ptr(n0 + 1);
+  FEDD_HIP(c, hipMemcpy(ptr.data(), c->d_nptr.p, (n0 + 1) * 8, hipMemcpyDeviceToHost));
+  std::vector<int32_t> idx(ptr[n0]);
+  FEDD_HIP(c, hipMemcpy(idx.data(), c->d_nidx.p, idx.size() * 4, hipMemcpyDeviceToHost));
+  // strength of the finest level's couplings, from the current operator
+  std::vector<uint8_t> strong(idx.size(), 1);
+  {
+    DevBuf d_nrm;
+    int rc = alloc(c, d_nrm, std::max<size_t>(idx.size(), 1) * 4);
+    if (rc) return rc;
+    k_amg_block_norm<<<nbk(n0, 256), 256, 0, c->stream>>>(n0, c->d_nptr.as<int64_t>(), c->blk[0].d_vals.as<double>(),
+                                                          d_nrm.as<float>());
+    FEDD_HIP(c, hipGetLastError());
+    std::vector<float> nrm(idx.size());
+    FEDD_HIP(c, hipMemcpyAsync(nrm.data(), d_nrm.p, nrm.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    FEDD_HIP(c, hipStreamSynchronize(c->stream));
+    std::vector<float> diag(n0, 0.f);
+    for (int64_t i = 0; i < n0; ++i)
+      for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k)
+        if (idx[k] == i) diag[i] = nrm[k];
+    for (int64_t i = 0; i < n0; ++i)
+      for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k)
+        strong[k] = idx[k] == i || nrm[k] >= (float)FEDD_AMG_THETA * std::sqrt(diag[i] * diag[idx[k]]);
+  }
+  bool first_level = true;
+  while (g->lv.back().nn > FEDD_AMG_COARSEST && g->lv.size() < 16) {
+    const MgLevel& F = g->lv.back();
+    std::vector<int32_t> agg;
+    const int64_t nc = aggregate(F.nn, ptr, idx, first_level ? &strong : nullptr, agg);
+    first_level = false;
+    if (nc >= F.nn) break;  // no coarsening left
+    MgLevel C;
+    C.nn = nc;
+    C.h_mptr.assign(nc + 1, 0);
+    for (int64_t i = 0; i < F.nn; ++i) C.h_mptr[agg[i] + 1]++;
+    for (int64_t I = 0; I < nc; ++I) C.h_mptr[I + 1] += C.h_mptr[I];
+    C.h_midx.resize(F.nn);
+    {
+      std::vector<int64_t> fill(C.h_mptr.begin(), C.h_mptr.end() - 1);
+      for (int64_t i = 0; i < F.nn; ++i) C.h_midx[fill[agg[i]]++] = (int32_t)i;
+    }
+    // coarse graph: J neighbours I iff a member of I couples to a member of J
+    const unsigned T = nthreads();
+    std::vector<std::vector<int32_t>> part_idx(T);
+    std::vector<std::vector<int64_t>> part_len(T);
+    std::vector<int64_t> part_begin(T, nc);
+    parallel_for(nc, [&](int64_t a, int64_t b, int t) {
+      std::vector<int32_t> stamp(nc, -1), row;
+      part_begin[t] = a;
+      for (int64_t I = a; I < b; ++I) {
+        row.clear();
+        for (int64_t q = C.h_mptr[I]; q < C.h_mptr[I + 1]; ++q) {
+          const int32_t i = C.h_midx[q];
+          for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+            const int32_t J = agg[idx[k]];
+            if (stamp[J] != (int32_t)I) {
+              stamp[J] = (int32_t)I;
+              row.push_back(J);
+            }
+          }
+        }
+        std::sort(row.begin(), row.end());
+        part_idx[t].insert(part_idx[t].end(), row.begin(), row.end());
+        part_len[t].push_back((int64_t)row.size());
+      }
+    });
+    C.h_ptr.assign(nc + 1, 0);
+    for (unsigned t = 0; t < T; ++t) {
+      int64_t I = part_begin[t];
+      for (int64_t l : part_len[t]) {
+        C.h_ptr[I + 1] = C.h_ptr[I] + l;
+        C.maxdeg = std::max(C.maxdeg, l);
+        ++I;
+      }
+      C.h_idx.insert(C.h_idx.end(), part_idx[t].begin(), part_idx[t].end());
+    }
+    C.nblk = C.h_ptr[nc];
+    if (9 * C.maxdeg * 8 > 65536) return fail(c, FEDD_ERR_LIMIT, "aggregation: a coarse row exceeds the LDS budget");
+    // coarse dof free iff some member's dof is free
+    std::vector<uint8_t> cf(3 * nc, 0);
+    const std::vector<uint8_t>& ff = frees.back();
+    for (int64_t i = 0; i < F.nn; ++i)
+      for (int d = 0; d < 3; ++d) cf[3 * (int64_t)agg[i] + d] |= ff[3 * i + d];
+    frees.push_back(std::move(cf));
+    C.h_agg = std::move(agg);
+    ptr = C.h_ptr;
+    idx = C.h_idx;
+    g->lv.push_back(std::move(C));
+  }
+  if (g->lv.back().nn > 2048) return fail(c, FEDD_ERR_LIMIT, "aggregation stalled above 2048 coarsest nodes");
+  return FEDD_OK;
+}
+
+int mg_setup(Ctx* c, int mode) {
+  if (c->mg && c->mg->mode == mode) return FEDD_OK;
+  mg_free(c);
   const Mesh& m = c->mesh[0];
   int64_t P0 = 0;
-  if (!lattice_mesh(m, P0))
+  if (mode == 0 && !lattice_mesh(m, P0))
     return fail(c, FEDD_ERR_STATE, "multigrid needs the structured P2 lattice mesh of fedd_mesh_structured");
   MgHost* g = new MgHost();
   c->mg = g;
+  g->mode = mode;
   // level 0: the context's operator (9 values per node block)
   {
     MgLevel L0;
     L0.P = P0;
-    L0.nn = P0 * P0 * P0;
+    L0.nn = m.npts;
     L0.nblk = c->blk[0].nnz / 9;
     g->lv.push_back(std::move(L0));
   }
-  // Coarse levels: P1 lattices.  The Galerkin product of the midpoint prolongation on the
-  // Kuhn lattice couples a coarse node exactly with itself and its 14 Kuhn neighbours
-  // I +- e_S (the edges of the P1 Kuhn mesh), from the P2 level as from every P1 level.
-  static const int kuhn[15][3] = {{0, 0, 0},  {1, 0, 0},  {-1, 0, 0},  {0, 1, 0},   {0, -1, 0},
-                                  {0, 0, 1},  {0, 0, -1}, {1, 1, 0},   {-1, -1, 0}, {1, 0, 1},
-                                  {-1, 0, -1}, {0, 1, 1}, {0, -1, -1}, {1, 1, 1},   {-1, -1, -1}};
-  while (g->lv.back().P >= 5 && ((g->lv.back().P - 1) & 1) == 0) {
-    MgLevel C;
-    C.P = (g->lv.back().P + 1) / 2;
-    C.nn = C.P * C.P * C.P;
-    C.h_ptr.assign(C.nn + 1, 0);
-    C.h_idx.reserve(15 * C.nn);
-    for (int64_t I = 0; I < C.nn; ++I) {
-      const int64_t X[3] = {I % C.P, (I / C.P) % C.P, I / (C.P * C.P)};
-      for (const auto& o : kuhn) {
-        const int64_t Y[3] = {X[0] + o[0], X[1] + o[1], X[2] + o[2]};
-        if (Y[0] < 0 || Y[1] < 0 || Y[2] < 0 || Y[0] >= C.P || Y[1] >= C.P || Y[2] >= C.P) continue;
-        C.h_idx.push_back((int32_t)(Y[0] + C.P * (Y[1] + C.P * Y[2])));
-      }
-      C.h_ptr[I + 1] = (int64_t)C.h_idx.size();
-      std::sort(C.h_idx.begin() + C.h_ptr[I], C.h_idx.end());
-      C.maxdeg = std::max(C.maxdeg, C.h_ptr[I + 1] - C.h_ptr[I]);
+  std::vector<std::vector<uint8_t>> frees(1, std::vector<uint8_t>(3 * m.npts, 1));
+  for (size_t k = 0; k < c->bcflag_host.size() && k < frees[0].size(); ++k) frees[0][k] = c->bcflag_host[k] ? 0 : 1;
+  int rc;
+  if (mode == 1) {
+    if ((rc = build_aggregation(c, g, frees))) {
+      mg_free(c);
+      return rc;
     }
-    C.nblk = C.h_ptr[C.nn];
-    g->lv.push_back(std::move(C));
+  } else {
+    // Coarse levels: P1 lattices.  The Galerkin product of the midpoint prolongation on the
+    // Kuhn lattice couples a coarse node exactly with itself and its 14 Kuhn neighbours
+    // I +- e_S (the edges of the P1 Kuhn mesh), from the P2 level as from every P1 level.
+    static const int kuhn[15][3] = {{0, 0, 0},  {1, 0, 0},  {-1, 0, 0},  {0, 1, 0},   {0, -1, 0},
+                                    {0, 0, 1},  {0, 0, -1}, {1, 1, 0},   {-1, -1, 0}, {1, 0, 1},
+                                    {-1, 0, -1}, {0, 1, 1}, {0, -1, -1}, {1, 1, 1},   {-1, -1, -1}};
+    while (g->lv.back().P >= 5 && ((g->lv.back().P - 1) & 1) == 0) {
+      const MgLevel& F = g->lv.back();
+      MgLevel C;
+      C.P = (F.P + 1) / 2;
+      C.nn = C.P * C.P * C.P;
+      C.h_ptr.assign(C.nn + 1, 0);
+      C.h_idx.reserve(15 * C.nn);
+      for (int64_t I = 0; I < C.nn; ++I) {
+        const int64_t X[3] = {I % C.P, (I / C.P) % C.P, I / (C.P * C.P)};
+        for (const auto& o : kuhn) {
+          const int64_t Y[3] = {X[0] + o[0], X[1] + o[1], X[2] + o[2]};
+          if (Y[0] < 0 || Y[1] < 0 || Y[2] < 0 || Y[0] >= C.P || Y[1] >= C.P || Y[2] >= C.P) continue;
+          C.h_idx.push_back((int32_t)(Y[0] + C.P * (Y[1] + C.P * Y[2])));
+        }
+        C.h_ptr[I + 1] = (int64_t)C.h_idx.size();
+        std::sort(C.h_idx.begin() + C.h_ptr[I], C.h_idx.end());
+        C.maxdeg = std::max(C.maxdeg, C.h_ptr[I + 1] - C.h_ptr[I]);
+      }
+      C.nblk = C.h_ptr[C.nn];
+      // coarse dof free iff the fine dof at the same lattice point is free
+      const std::vector<uint8_t>& ff = frees.back();
+      std::vector<uint8_t> cf(3 * C.nn);
+      for (int64_t I = 0; I < C.nn; ++I) {
+        const int64_t x0 = I % C.P, x1 = (I / C.P) % C.P, x2 = I / (C.P * C.P);
+        const int64_t f = 2 * x0 + F.P * (2 * x1 + F.P * 2 * x2);
+        for (int d = 0; d < 3; ++d) cf[3 * I + d] = ff[3 * f + d];
+      }
+      frees.push_back(std::move(cf));
+      g->lv.push_back(std::move(C));
+    }
   }
   // device arrays and Dirichlet masks
-  std::vector<uint8_t> fr(3 * g->lv[0].nn, 1);
-  for (size_t k = 0; k < c->bcflag_host.size() && k < fr.size(); ++k) fr[k] = c->bcflag_host[k] ? 0 : 1;
   for (size_t l = 0; l < g->lv.size(); ++l) {
     MgLevel& L = g->lv[l];
-    int rc;
     if (l == 0) {
       L.d_nptr = c->d_nptr.as<int64_t>();
       L.d_nidx = c->d_nidx.as<int32_t>();
@@ -589,30 +859,25 @@ int mg_setup(Ctx* c) {
       L.d_nptr = L.nptr.as<int64_t>();
       L.d_nidx = L.nidx.as<int32_t>();
       L.d_vals = L.vals.as<double>();
-      // coarse dof free iff the fine dof at the same lattice point is free
-      const MgLevel& F = g->lv[l - 1];
-      std::vector<uint8_t> cf(3 * L.nn);
-      for (int64_t I = 0; I < L.nn; ++I) {
-        const int64_t x0 = I % L.P, x1 = (I / L.P) % L.P, x2 = I / (L.P * L.P);
-        const int64_t f = 2 * x0 + F.P * (2 * x1 + F.P * 2 * x2);
-        for (int d = 0; d < 3; ++d) cf[3 * I + d] = fr[3 * f + d];
+      if (mode == 1) {
+        if ((rc = upload(c, L.agg, L.h_agg.data(), L.h_agg.size() * 4))) return rc;
+        if ((rc = upload(c, L.mptr, L.h_mptr.data(), L.h_mptr.size() * 8))) return rc;
+        if ((rc = upload(c, L.midx, L.h_midx.data(), L.h_midx.size() * 4))) return rc;
       }
-      fr.swap(cf);
     }
-    if ((rc = upload(c, L.free, fr.data(), fr.size()))) return rc;
+    if ((rc = upload(c, L.free, frees[l].data(), frees[l].size()))) return rc;
     for (DevBuf* b : {&L.x, &L.x2, &L.b, &L.r})
       if ((rc = alloc(c, *b, L.nn * 3 * 8))) return rc;
     if ((rc = alloc(c, L.binv, L.nn * 9 * 8))) return rc;
     if (l == 0 && FEDD_MG_FP32_FINE && (rc = alloc(c, L.vals32, L.nblk * 9 * 4))) return rc;
-    if (l > 0) {  // host copies are only needed for the upload
-      L.h_ptr.clear();
-      L.h_ptr.shrink_to_fit();
-      L.h_idx.clear();
-      L.h_idx.shrink_to_fit();
-    }
+    // host copies are only needed for the upload
+    std::vector<int64_t>().swap(L.h_ptr);
+    std::vector<int32_t>().swap(L.h_idx);
+    std::vector<int32_t>().swap(L.h_agg);
+    std::vector<int64_t>().swap(L.h_mptr);
+    std::vector<int32_t>().swap(L.h_midx);
   }
   g->ncoarse = 3 * g->lv.back().nn;
-  int rc;
   if ((rc = alloc(c, g->inv, g->ncoarse * g->ncoarse * 8))) return rc;
   if ((rc = alloc(c, g->dense, g->ncoarse * g->ncoarse * 8))) return rc;
   return FEDD_OK;
@@ -628,9 +893,14 @@ static int refresh_eager(Ctx* c) {
       const MgLevel& F = g->lv[l - 1];
       const unsigned grid = (unsigned)std::min<int64_t>(L.nn, (int64_t)c->num_cu * 16);
       ktimer_begin(c);
-      k_mg_galerkin<<<grid, 64, (size_t)9 * L.maxdeg * 8, c->stream>>>(
-          F.P, L.P, F.d_nptr, F.d_nidx, F.d_vals, F.free.as<uint8_t>(), L.d_nptr, L.d_nidx, L.d_vals,
-          L.free.as<uint8_t>());
+      if (g->mode == 1)
+        k_amg_galerkin<<<grid, 64, (size_t)9 * L.maxdeg * 8, c->stream>>>(
+            L.nn, L.mptr.as<int64_t>(), L.midx.as<int32_t>(), L.agg.as<int32_t>(), F.d_nptr, F.d_nidx, F.d_vals,
+            F.free.as<uint8_t>(), L.d_nptr, L.d_nidx, L.d_vals, L.free.as<uint8_t>());
+      else
+        k_mg_galerkin<<<grid, 64, (size_t)9 * L.maxdeg * 8, c->stream>>>(
+            F.P, L.P, F.d_nptr, F.d_nidx, F.d_vals, F.free.as<uint8_t>(), L.d_nptr, L.d_nidx, L.d_vals,
+            L.free.as<uint8_t>());
       FEDD_HIP(c, hipGetLastError());
       ktimer_end(c, l == 1 ? "mg_galerkin_1" : "mg_galerkin_coarse");
     }
@@ -707,11 +977,20 @@ static void vcycle(Ctx* c, size_t l, const double* b, double* x) {
   // coarse correction
   sweep(2, cur, L.r.as<double>());
   MgLevel& C = g->lv[l + 1];
-  k_mg_restrict<<<nbk(C.nn, 256), 256, 0, c->stream>>>(L.P, C.P, L.r.as<double>(), L.free.as<uint8_t>(),
-                                                       C.free.as<uint8_t>(), C.b.as<double>());
+  if (g->mode == 1)
+    k_amg_restrict<<<nbk(C.nn, 256), 256, 0, c->stream>>>(C.nn, C.mptr.as<int64_t>(), C.midx.as<int32_t>(),
+                                                          L.r.as<double>(), L.free.as<uint8_t>(),
+                                                          C.free.as<uint8_t>(), C.b.as<double>());
+  else
+    k_mg_restrict<<<nbk(C.nn, 256), 256, 0, c->stream>>>(L.P, C.P, L.r.as<double>(), L.free.as<uint8_t>(),
+                                                         C.free.as<uint8_t>(), C.b.as<double>());
   vcycle(c, l + 1, C.b.as<double>(), C.x.as<double>());
-  k_mg_prolong<<<nbk(L.nn, 256), 256, 0, c->stream>>>(L.P, C.P, C.x.as<double>(), L.free.as<uint8_t>(),
-                                                      C.free.as<uint8_t>(), cur);
+  if (g->mode == 1)
+    k_amg_prolong<<<nbk(L.nn, 256), 256, 0, c->stream>>>(L.nn, C.agg.as<int32_t>(), C.x.as<double>(),
+                                                         L.free.as<uint8_t>(), cur, FEDD_AMG_ALPHA);
+  else
+    k_mg_prolong<<<nbk(L.nn, 256), 256, 0, c->stream>>>(L.P, C.P, C.x.as<double>(), L.free.as<uint8_t>(),
+                                                        C.free.as<uint8_t>(), cur);
   for (int s = 0; s < FEDD_MG_NU; ++s) {
     double* nxt = cur == xa ? xb : xa;
     sweep(0, cur, nxt);

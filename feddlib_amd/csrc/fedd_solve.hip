@@ -392,11 +392,11 @@ int pcg_solve(Ctx* c, double rtol, double atol, int maxit, int* iters, double* r
 // memory while iteration t is already queued (at most one iteration beyond the target).
 // refresh: 0 keep the hierarchy, 1 rebuild it for the current A, 2 finest level only.
 int pcg_mg_solve(Ctx* c, double rtol, double atol, int maxit, int* iters, double* relres, int refresh,
-                 double bnorm_known) {
+                 double bnorm_known, int mode) {
   const Block& A = c->blk[0];
   const int64_t n = A.nrows, nnode = n / 3;
   int rc;
-  if ((rc = mg_setup(c))) return rc;
+  if ((rc = mg_setup(c, mode))) return rc;
   if (refresh == 1 && (rc = mg_refresh(c))) return rc;
   if (refresh == 2 && (rc = mg_refresh_fine(c))) return rc;
   for (int k = 0; k < 4; ++k)
@@ -578,13 +578,13 @@ int fedd_linear_solve(fedd_ctx* cc, double rtol, int maxit, int* iters, double* 
   if (c->graph_kind != FEDD_GRAPH_VECTOR) return fail(c, FEDD_ERR_STATE, "needs a vector graph");
   if (c->vec_n[FEDD_VEC_B] != c->blk[0].nrows) return fail(c, FEDD_ERR_STATE, "right-hand side not set");
   (void)hipSetDevice(c->device);
-  if (c->precond == 1) return pcg_mg_solve(c, rtol, 0.0, maxit, iters, relres, 1, 0.0);
+  if (c->precond >= 1) return pcg_mg_solve(c, rtol, 0.0, maxit, iters, relres, 1, 0.0, c->precond - 1);
   return pcg_solve(c, rtol, 0.0, maxit, iters, relres);
 }
 
 int fedd_set_preconditioner(fedd_ctx* cc, int kind) {
   Ctx* c = cc;
-  if (!c || kind < 0 || kind > 1) return FEDD_ERR_ARG;
+  if (!c || kind < 0 || kind > 2) return FEDD_ERR_ARG;
   c->precond = kind;
   return FEDD_OK;
 }
@@ -604,7 +604,7 @@ int fedd_newton_elasticity(fedd_ctx* cc, const fedd_newton_params* P, fedd_newto
   if (c->graph_kind != FEDD_GRAPH_VECTOR || c->mesh[0].npe != 10)
     return fail(c, FEDD_ERR_STATE, "needs a vector graph on a P2 (tet-10) mesh");
   if (P->load_steps < 1 || P->max_newton < 1 || !(P->abs_tol > 0) || P->preconditioner < 0 ||
-      P->preconditioner > 1)
+      P->preconditioner > 2)
     return fail(c, FEDD_ERR_ARG, "bad Newton parameters");
   const int model = P->neo_hooke_model == 1 ? 1 : 2;
   if ((P->tangent == 1 || P->collect_samples) && !c->mlp_ready)
@@ -700,8 +700,8 @@ int fedd_newton_elasticity(fedd_ctx* cc, const fedd_newton_params* P, fedd_newto
       const int refresh =
           P->mg_reuse && coarse_current && last_iters <= 2 * base_iters + 2 ? 2 : 1;
       FEDD_HIP(c, hipEventRecord(sa, c->stream));
-      if (P->preconditioner == 1)
-        rc = pcg_mg_solve(c, eta, 0.0, P->lin_maxit, &iters, &relres, refresh, norm);  // b = -R, ||b|| = norm
+      if (P->preconditioner >= 1)  // b = -R, ||b|| = norm
+        rc = pcg_mg_solve(c, eta, 0.0, P->lin_maxit, &iters, &relres, refresh, norm, P->preconditioner - 1);
       else
         rc = pcg_solve(c, eta, 0.0, P->lin_maxit, &iters, &relres);
       if (rc) return rc;

@@ -209,8 +209,10 @@ int fedd_traction_load(fedd_ctx* ctx, int axis, double value, const double* trac
 int fedd_linear_solve(fedd_ctx* ctx, double rtol, int maxit, int* iters, double* relres);
 /* 0: 3x3 block Jacobi (any mesh); 1: geometric multigrid V-cycle on the structured
  * lattice (P2 -> P1 -> coarser P1, Galerkin coarse operators, damped block-Jacobi
- * smoothing, dense coarsest solve) standing in for the reference's FROSch/MueLu
- * preconditioners (Preconditioner_def.hpp:206-291). */
+ * smoothing, dense coarsest solve); 2: aggregation multigrid on any mesh (greedy node
+ * aggregates, piecewise-constant transfers per dof component, Galerkin coarse operators,
+ * the same smoother and coarsest solve).  1 and 2 stand in for the reference's
+ * FROSch/MueLu preconditioners (Preconditioner_def.hpp:206-291). */
 int fedd_set_preconditioner(fedd_ctx* ctx, int kind);
 
 typedef struct {
@@ -228,8 +230,9 @@ typedef struct {
                            the step first when ||R_k+1|| > ||R_k|| or is not finite */
   int collect_samples;  /* nonzero: append the quadrature-point states of every Newton iterate
                            to the surrogate's training set (needs fedd_mlp_init) */
-  int preconditioner;   /* 0: 3x3 block Jacobi; 1: geometric multigrid V-cycle (structured
-                           lattice meshes of fedd_mesh_structured only) */
+  int preconditioner;   /* as fedd_set_preconditioner: 0 block Jacobi, 1 geometric multigrid
+                           (structured lattice meshes of fedd_mesh_structured only),
+                           2 aggregation multigrid (any mesh) */
   int mg_reuse;         /* multigrid only.  0: rebuild the hierarchy for every tangent;
                            1: within a load step keep the coarse levels of the step's first
                            tangent (the finest level always follows the current tangent)

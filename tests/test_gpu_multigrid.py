@@ -119,3 +119,77 @@ def test_mg_reuse_of_coarse_levels(gpu_ctx):
     assert np.max(np.abs(u1 - u0)) <= 1e-7 * np.max(np.abs(u0))
     assert out[True][0]["newton_iters"] == out[False][0]["newton_iters"]
     assert out[True][0]["krylov_iters"] <= 2 * out[False][0]["krylov_iters"]
+
+
+def _relabelled(M, seed=0):
+    """The lattice mesh with permuted node ids and jittered interior nodes (edge nodes kept
+    at the midpoints): not a lattice, so only the aggregation hierarchy applies."""
+    pts, el, dofs = _problem(M)
+    rng = np.random.default_rng(seed)
+    h = 1.0 / (2 * M)
+    inner = np.all((pts > 1e-12) & (pts < 1 - 1e-12), axis=1)
+    verts = np.unique(el[:, :4])
+    jit = np.zeros_like(pts)
+    jit[verts] = 0.1 * h * rng.uniform(-1, 1, (len(verts), 3))
+    jit[~inner] = 0.0
+    pts = pts + jit
+    for k, (a, b) in enumerate([(0, 1), (1, 2), (0, 2), (0, 3), (1, 3), (2, 3)]):
+        pts[el[:, 4 + k]] = 0.5 * (pts[el[:, a]] + pts[el[:, b]])
+    perm = rng.permutation(len(pts))
+    inv = np.argsort(perm)
+    el2 = np.ascontiguousarray(inv[el].astype(np.int32))
+    base = np.where(np.abs(pts[perm][:, 2]) < 1e-12)[0]
+    dofs2 = np.sort((3 * base[:, None] + np.arange(3)).ravel()).astype(np.int32)
+    return pts[perm], el2, dofs2
+
+
+@pytest.mark.parametrize("M,relabel", [(3, False), (6, False), (6, True)])
+def test_amg_linear_solve(gpu_ctx, M, relabel):
+    """Aggregation AMG (preconditioner 2): the direct solution, on the lattice and on a
+    relabelled, jittered mesh, in far fewer iterations than block Jacobi."""
+    pts, el, dofs = _relabelled(M) if relabel else _problem(M)
+    n = 3 * pts.shape[0]
+    gpu_ctx.mesh_upload(N.MESH_PRIMARY, pts, el)
+    gpu_ctx.graph_build(N.GRAPH_VECTOR)
+    gpu_ctx.vector_upload(N.VEC_U, np.zeros(n))
+    gpu_ctx.assembly_nonlinear_elasticity(1, EMOD, NU)
+    gpu_ctx.dirichlet_set(dofs)
+    gpu_ctx.apply_dirichlet()
+    b = np.random.default_rng(M).standard_normal(n)
+    b[dofs] = 0
+    rp, ci = gpu_ctx.block_graph(N.BLOCK_A)
+    A = sp.csr_matrix((gpu_ctx.block_values(N.BLOCK_A), ci, rp), shape=(n, n))
+    xd = spla.spsolve(A.tocsc(), b)
+    its = {}
+    try:
+        for pc in (0, 2):
+            gpu_ctx.set_preconditioner(pc)
+            gpu_ctx.vector_upload(N.VEC_B, b)
+            it, rr = gpu_ctx.linear_solve(1e-11, 20000)
+            x = gpu_ctx.vector_download(N.VEC_X, n)
+            assert rr <= 1e-11
+            assert np.linalg.norm(x - xd) <= 1e-8 * np.linalg.norm(xd)
+            its[pc] = it
+    finally:
+        gpu_ctx.set_preconditioner(0)
+    if M >= 6:
+        assert its[2] * 3 < its[0], its
+
+
+def test_amg_newton_on_unstructured_mesh(gpu_ctx):
+    pts, el, dofs = _relabelled(6, seed=1)
+    t = np.array([40.0, 0.0, 60.0])
+    out = {}
+    for pc in (0, 2):
+        gpu_ctx.mesh_upload(N.MESH_PRIMARY, pts, el)
+        gpu_ctx.graph_build(N.GRAPH_VECTOR)
+        gpu_ctx.vector_upload(N.VEC_U, np.zeros(3 * pts.shape[0]))
+        gpu_ctx.dirichlet_set(dofs)
+        gpu_ctx.traction_load(2, 1.0, t)
+        st, _ = gpu_ctx.newton_elasticity(model=1, E=EMOD, nu=NU, load_steps=3, abs_tol=1e-8, preconditioner=pc,
+                                          mg_reuse=pc == 2)
+        assert st["converged"] == 1 and st["final_residual"] < 1e-8
+        out[pc] = (st, gpu_ctx.vector_download(N.VEC_U, 3 * pts.shape[0]))
+    u0, u2 = out[0][1], out[2][1]
+    assert np.max(np.abs(u2 - u0)) <= 1e-7 * np.max(np.abs(u0))
+    assert out[2][0]["krylov_iters"] * 3 < out[0][0]["krylov_iters"]
