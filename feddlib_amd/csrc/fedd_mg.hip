@@ -180,18 +180,37 @@ __global__ __launch_bounds__(256) void k_mg_to_f32(int64_t n, const double* __re
     b[i] = (float)a[i];
 }
 
+// Restriction on the lattice: 16 lanes per coarse node, lane 0 the coincident fine node
+// (weight 1), lanes 1..14 the midpoints of the 14 Kuhn edges +-e_S (weight 1/2), then a
+// shuffle reduction.  (One thread per coarse node leaves too few waves in flight on the
+// 8x smaller coarse level.)
 __global__ __launch_bounds__(256) void k_mg_restrict(int64_t Pf, int64_t Pc, const double* __restrict__ rf,
                                                      const uint8_t* __restrict__ freef,
                                                      const uint8_t* __restrict__ freec, double* __restrict__ bc) {
-  const int64_t I = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (I >= Pc * Pc * Pc) return;
-  int64_t ch[15];
-  double w[15];
-  const int n = children(I, Pc, Pf, ch, w);
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t I = t >> 4;
+  const int lane = (int)(t & 15);
+  const int64_t nc = Pc * Pc * Pc;
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int k = 0; k < n; ++k)
+  if (I < nc && lane < 15) {
+    int64_t X[3];
+    lattice(I, Pc, X);
+    const int s = (lane + 1) >> 1;              // axis set of the Kuhn edge (0: the node itself)
+    const int sg = (lane & 1) ? 1 : -1;         // direction along it
+    const int64_t f0 = 2 * X[0] + sg * (s & 1), f1 = 2 * X[1] + sg * ((s >> 1) & 1),
+                  f2 = 2 * X[2] + sg * ((s >> 2) & 1);
+    if (f0 >= 0 && f1 >= 0 && f2 >= 0 && f0 < Pf && f1 < Pf && f2 < Pf) {
+      const int64_t f = f0 + Pf * (f1 + Pf * f2);
+      const double w = lane == 0 ? 1.0 : 0.5;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) acc[d] += freef[3 * ch[k] + d] ? w[k] * rf[3 * ch[k] + d] : 0.0;
+      for (int d = 0; d < 3; ++d) acc[d] = freef[3 * f + d] ? w * rf[3 * f + d] : 0.0;
+    }
+  }
+#pragma unroll
+  for (int o = 8; o > 0; o >>= 1)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) acc[d] += __shfl_xor(acc[d], o, 16);
+  if (I >= nc || lane != 0) return;
 #pragma unroll
   for (int d = 0; d < 3; ++d) bc[3 * I + d] = freec[3 * I + d] ? acc[d] : 0.0;
 }
@@ -982,8 +1001,8 @@ static void vcycle(Ctx* c, size_t l, const double* b, double* x) {
                                                           L.r.as<double>(), L.free.as<uint8_t>(),
                                                           C.free.as<uint8_t>(), C.b.as<double>());
   else
-    k_mg_restrict<<<nbk(C.nn, 256), 256, 0, c->stream>>>(L.P, C.P, L.r.as<double>(), L.free.as<uint8_t>(),
-                                                         C.free.as<uint8_t>(), C.b.as<double>());
+    k_mg_restrict<<<nbk(C.nn * 16, 256), 256, 0, c->stream>>>(L.P, C.P, L.r.as<double>(), L.free.as<uint8_t>(),
+                                                              C.free.as<uint8_t>(), C.b.as<double>());
   vcycle(c, l + 1, C.b.as<double>(), C.x.as<double>());
   if (g->mode == 1)
     k_amg_prolong<<<nbk(L.nn, 256), 256, 0, c->stream>>>(L.nn, C.agg.as<int32_t>(), C.x.as<double>(),
