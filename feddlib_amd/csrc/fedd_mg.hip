@@ -32,6 +32,9 @@
 // The smoother and residual of the finest level read an FP32 copy of the operator
 // (refreshed with it): half the bytes of the HBM-bound sweeps.  The preconditioner stays
 // a fixed symmetric linear operator, so CG's FP64 iteration is unaffected.
+#ifndef FEDD_MG_G0
+#define FEDD_MG_G0 32  // lanes per node in the finest level's sweeps
+#endif
 #ifndef FEDD_MG_FP32_FINE
 #define FEDD_MG_FP32_FINE 1
 #endif
@@ -953,6 +956,7 @@ static int refresh_eager(Ctx* c) {
   return FEDD_OK;
 }
 
+// The first sweep (from x = 0) reads no matrix row: one lane per node.
 template <int G, class VT>
 static void launch_sweep(hipStream_t st, unsigned gs, int mode, const MgLevel& L, const VT* vals, const double* b,
                          const double* xin, double* out, double w) {
@@ -960,7 +964,7 @@ static void launch_sweep(hipStream_t st, unsigned gs, int mode, const MgLevel& L
   if (mode == 0)
     k_mg_sweep<G, 0, VT><<<gs, 256, 0, st>>>(L.nn, L.d_nptr, L.d_nidx, vals, bi, b, xin, out, w);
   else if (mode == 1)
-    k_mg_sweep<G, 1, VT><<<gs, 256, 0, st>>>(L.nn, L.d_nptr, L.d_nidx, vals, bi, b, xin, out, w);
+    k_mg_sweep<1, 1, VT><<<nbk(L.nn, 256), 256, 0, st>>>(L.nn, L.d_nptr, L.d_nidx, vals, bi, b, xin, out, w);
   else
     k_mg_sweep<G, 2, VT><<<gs, 256, 0, st>>>(L.nn, L.d_nptr, L.d_nidx, vals, bi, b, xin, out, w);
 }
@@ -973,12 +977,12 @@ static void vcycle(Ctx* c, size_t l, const double* b, double* x) {
     return;
   }
   const double w = FEDD_MG_OMEGA;
-  const unsigned gs = l == 0 ? nbk(L.nn * 32, 256) : nbk(L.nn * 16, 256);
+  const unsigned gs = l == 0 ? nbk(L.nn * FEDD_MG_G0, 256) : nbk(L.nn * 16, 256);
   auto sweep = [&](int mode, const double* xin, double* out) {
     if (l == 0 && L.vals32.p)
-      launch_sweep<32>(c->stream, gs, mode, L, L.vals32.as<float>(), b, xin, out, w);
+      launch_sweep<FEDD_MG_G0>(c->stream, gs, mode, L, L.vals32.as<float>(), b, xin, out, w);
     else if (l == 0)
-      launch_sweep<32>(c->stream, gs, mode, L, (const double*)L.d_vals, b, xin, out, w);
+      launch_sweep<FEDD_MG_G0>(c->stream, gs, mode, L, (const double*)L.d_vals, b, xin, out, w);
     else
       launch_sweep<16>(c->stream, gs, mode, L, (const double*)L.d_vals, b, xin, out, w);
   };
