@@ -51,14 +51,17 @@ struct MgLevel {
   double* d_vals = nullptr;
   DevBuf binv, free, x, x2, b, r;
   DevBuf vals32;  // finest level: FP32 copy of the operator used by the smoother
-  // aggregation hierarchy: transfer from the next finer level to this one
-  DevBuf agg;         // int32, per finer-level node: its aggregate (a node of this level)
-  DevBuf mptr, midx;  // int64 / int32: members (finer-level nodes) of every node
+  // aggregation hierarchy: transfer from the next finer level F to this level
+  DevBuf agg;                // int32, per F node: its aggregate (a node of this level)
+  DevBuf pptr, pidx, pval;   // smoothed prolongator, rows = F nodes, 3x3 blocks (row-major)
+  DevBuf tptr, tidx, tpos;   // its transpose: per node of this level, (F node, block index in pval)
+  DevBuf aptr, aidx, aval;   // A_F P, rows = F nodes
+  int64_t maxdeg_ap = 0;
   std::vector<int64_t> h_ptr;
   std::vector<int32_t> h_idx;
   std::vector<int32_t> h_agg;
-  std::vector<int64_t> h_mptr;
-  std::vector<int32_t> h_midx;
+  std::vector<int64_t> h_pptr, h_tptr, h_tpos, h_aptr;
+  std::vector<int32_t> h_pidx, h_tidx, h_aidx;
 };
 
 struct MgHost {
@@ -256,45 +259,108 @@ __global__ __launch_bounds__(256) void k_amg_block_norm(int64_t nn, const int64_
   }
 }
 
-// Aggregation transfers (piecewise constant over the aggregates, per dof component):
-// restriction sums the free fine dofs of each aggregate, prolongation adds the
-// aggregate's correction to its free fine dofs.
-__global__ __launch_bounds__(256) void k_amg_restrict(int64_t nc, const int64_t* __restrict__ mptr,
-                                                      const int32_t* __restrict__ midx, const double* __restrict__ rf,
-                                                      const uint8_t* __restrict__ freef,
-                                                      const uint8_t* __restrict__ freec, double* __restrict__ bc) {
-  const int64_t I = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (I >= nc) return;
-  double acc[3] = {0.0, 0.0, 0.0};
-  for (int64_t k = mptr[I]; k < mptr[I + 1]; ++k) {
-    const int64_t i = midx[k];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) acc[d] += freef[3 * i + d] ? rf[3 * i + d] : 0.0;
-  }
-#pragma unroll
-  for (int d = 0; d < 3; ++d) bc[3 * I + d] = freec[3 * I + d] ? acc[d] : 0.0;
-}
-
-__global__ __launch_bounds__(256) void k_amg_prolong(int64_t nf, const int32_t* __restrict__ agg,
-                                                     const double* __restrict__ xc, const uint8_t* __restrict__ freef,
-                                                     double* __restrict__ xf, double alpha) {
+// Smoothed aggregation.  The tentative prolongator maps coarse dof (J, d) to the free fine
+// dofs (i, d) with agg(i) = J; the prolongator is P = (I - omega D^-1 A) P_tent, D the 3x3
+// block diagonal (binv holds its inverse).  Row i of P has a 3x3 block for every aggregate
+// of i's neighbours; rows of constrained fine dofs are zero.
+__global__ __launch_bounds__(256) void k_sa_smooth_p(int64_t nf, const int64_t* __restrict__ fptr,
+                                                     const int32_t* __restrict__ fidx, const double* __restrict__ fval,
+                                                     const double* __restrict__ binv, const uint8_t* __restrict__ ffree,
+                                                     const int32_t* __restrict__ agg, const int64_t* __restrict__ pptr,
+                                                     const int32_t* __restrict__ pidx, double* __restrict__ pval,
+                                                     double omega) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= nf) return;
-  const int64_t I = agg[i];
+  const int64_t p0 = pptr[i];
+  const int np = (int)(pptr[i + 1] - p0);
+  const int32_t* prow = pidx + p0;
+  double* pv = pval + 9 * p0;
+  for (int k = 0; k < 9 * np; ++k) pv[k] = 0.0;
+  const int64_t f0 = fptr[i];
+  const int degf = (int)(fptr[i + 1] - f0);
+  const double* fv = fval + 9 * f0;
+  for (int s = 0; s < degf; ++s) {  // S = A P_tent
+    const int32_t j = fidx[f0 + s];
+    const int slot = find_slot(prow, np, agg[j]);
+    if (slot < 0) continue;  // cannot happen: row i holds the aggregate of every neighbour
 #pragma unroll
-  for (int d = 0; d < 3; ++d)
-    if (freef[3 * i + d]) xf[3 * i + d] += alpha * xc[3 * I + d];
+    for (int d1 = 0; d1 < 3; ++d1)
+#pragma unroll
+      for (int d2 = 0; d2 < 3; ++d2)
+        if (ffree[3 * (int64_t)j + d2]) pv[9 * slot + 3 * d1 + d2] += fv[3 * degf * d1 + 3 * s + d2];
+  }
+  const double* B = binv + 9 * i;
+  const int32_t own = agg[i];
+  for (int slot = 0; slot < np; ++slot) {
+    double t[3][3];
+#pragma unroll
+    for (int d1 = 0; d1 < 3; ++d1)
+#pragma unroll
+      for (int d2 = 0; d2 < 3; ++d2)
+        t[d1][d2] = B[3 * d1] * pv[9 * slot + d2] + B[3 * d1 + 1] * pv[9 * slot + 3 + d2] +
+                    B[3 * d1 + 2] * pv[9 * slot + 6 + d2];
+#pragma unroll
+    for (int d1 = 0; d1 < 3; ++d1)
+#pragma unroll
+      for (int d2 = 0; d2 < 3; ++d2) {
+        double v = -omega * t[d1][d2];
+        if (prow[slot] == own && d1 == d2) v += 1.0;
+        pv[9 * slot + 3 * d1 + d2] = ffree[3 * i + d1] ? v : 0.0;
+      }
+  }
 }
 
-// Galerkin product P^T A P for aggregation (P boolean per dof, masked by the free fine
-// dofs): coarse block (I, J) sums the fine blocks (i, j), i in I, j in J.  One
-// workgroup per coarse row, accumulated in LDS; constrained coarse dofs get identity rows.
-__global__ __launch_bounds__(64) void k_amg_galerkin(int64_t nc, const int64_t* __restrict__ mptr,
-                                                     const int32_t* __restrict__ midx, const int32_t* __restrict__ agg,
-                                                     const int64_t* __restrict__ fptr, const int32_t* __restrict__ fidx,
-                                                     const double* __restrict__ fval, const uint8_t* __restrict__ ffree,
-                                                     const int64_t* __restrict__ cptr, const int32_t* __restrict__ cidx,
-                                                     double* __restrict__ cval, const uint8_t* __restrict__ cfree) {
+// AP = A_F P: one workgroup (64 threads) per fine row, accumulated in LDS by aggregate.
+__global__ __launch_bounds__(64) void k_sa_ap(int64_t nf, const int64_t* __restrict__ fptr,
+                                              const int32_t* __restrict__ fidx, const double* __restrict__ fval,
+                                              const int64_t* __restrict__ pptr, const int32_t* __restrict__ pidx,
+                                              const double* __restrict__ pval, const int64_t* __restrict__ aptr,
+                                              const int32_t* __restrict__ aidx, double* __restrict__ aval) {
+  extern __shared__ double acc[];
+  for (int64_t i = blockIdx.x; i < nf; i += gridDim.x) {
+    const int64_t a0 = aptr[i];
+    const int na = (int)(aptr[i + 1] - a0);
+    const int32_t* arow = aidx + a0;
+    for (int k = threadIdx.x; k < 9 * na; k += blockDim.x) acc[k] = 0.0;
+    __syncthreads();
+    const int64_t f0 = fptr[i];
+    const int degf = (int)(fptr[i + 1] - f0);
+    const double* fv = fval + 9 * f0;
+    for (int s = threadIdx.x; s < degf; s += blockDim.x) {
+      const int32_t j = fidx[f0 + s];
+      double A[3][3];
+#pragma unroll
+      for (int d1 = 0; d1 < 3; ++d1)
+#pragma unroll
+        for (int e = 0; e < 3; ++e) A[d1][e] = fv[3 * degf * d1 + 3 * s + e];
+      for (int64_t k = pptr[j]; k < pptr[j + 1]; ++k) {
+        const int slot = find_slot(arow, na, pidx[k]);
+        if (slot < 0) continue;  // cannot happen: the AP graph holds every product
+        const double* P = pval + 9 * k;
+#pragma unroll
+        for (int d1 = 0; d1 < 3; ++d1)
+#pragma unroll
+          for (int d2 = 0; d2 < 3; ++d2) {
+            const double v = A[d1][0] * P[d2] + A[d1][1] * P[3 + d2] + A[d1][2] * P[6 + d2];
+            if (v != 0.0) atomicAdd(acc + 9 * slot + 3 * d1 + d2, v);
+          }
+      }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < 9 * na; k += blockDim.x) aval[9 * a0 + k] = acc[k];
+    __syncthreads();
+  }
+}
+
+// Coarse operator P^T (A P): one workgroup per coarse row I over the fine rows i with a
+// block of P in column I (the transpose lists), written in the node-block layout;
+// constrained coarse dofs (all-zero columns of P) get identity rows.
+__global__ __launch_bounds__(64) void k_sa_ptap(int64_t nc, const int64_t* __restrict__ tptr,
+                                                const int32_t* __restrict__ tidx, const int64_t* __restrict__ tpos,
+                                                const double* __restrict__ pval, const int64_t* __restrict__ aptr,
+                                                const int32_t* __restrict__ aidx, const double* __restrict__ aval,
+                                                const int64_t* __restrict__ cptr, const int32_t* __restrict__ cidx,
+                                                double* __restrict__ cval, const uint8_t* __restrict__ cfree) {
   extern __shared__ double acc[];
   for (int64_t I = blockIdx.x; I < nc; I += gridDim.x) {
     const int64_t c0 = cptr[I];
@@ -302,42 +368,77 @@ __global__ __launch_bounds__(64) void k_amg_galerkin(int64_t nc, const int64_t* 
     const int32_t* crow = cidx + c0;
     for (int k = threadIdx.x; k < 9 * degc; k += blockDim.x) acc[k] = 0.0;
     __syncthreads();
-    uint8_t cf[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) cf[d] = cfree[3 * I + d];
-    for (int64_t a = mptr[I]; a < mptr[I + 1]; ++a) {
-      const int64_t f = midx[a];
-      const int64_t f0 = fptr[f];
-      const int degf = (int)(fptr[f + 1] - f0);
-      uint8_t fr[3];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) fr[d] = ffree[3 * f + d] & cf[d];
-      const double* fv = fval + 9 * f0;
-      for (int k = threadIdx.x; k < degf; k += blockDim.x) {
-        const int32_t j = fidx[f0 + k];
-        const int32_t J = agg[j];
-        const int slot = find_slot(crow, degc, J);
-        if (slot < 0) continue;  // cannot happen: the coarse graph holds every coupling
+    for (int64_t q = tptr[I] + threadIdx.x; q < tptr[I + 1]; q += blockDim.x) {
+      const int64_t i = tidx[q];
+      const double* P = pval + 9 * tpos[q];  // block (i, I)
+      for (int64_t a = aptr[i]; a < aptr[i + 1]; ++a) {
+        const int slot = find_slot(crow, degc, aidx[a]);
+        if (slot < 0) continue;  // cannot happen: the coarse graph holds every product
+        const double* Q = aval + 9 * a;
 #pragma unroll
         for (int d1 = 0; d1 < 3; ++d1)
 #pragma unroll
           for (int d2 = 0; d2 < 3; ++d2) {
-            const bool on = fr[d1] && ffree[3 * (int64_t)j + d2] && cfree[3 * (int64_t)J + d2];
-            const double v = on ? fv[3 * degf * d1 + 3 * k + d2] : 0.0;
+            const double v = P[d1] * Q[d2] + P[3 + d1] * Q[3 + d2] + P[6 + d1] * Q[6 + d2];  // (P^T Q)[d1][d2]
             if (v != 0.0) atomicAdd(acc + 3 * degc * d1 + 3 * slot + d2, v);
           }
       }
-      __syncthreads();
     }
+    __syncthreads();
     const int self = find_slot(crow, degc, (int32_t)I);
     for (int k = threadIdx.x; k < 9 * degc; k += blockDim.x) {
-      const int d1 = k / (3 * degc), q = k - 3 * degc * d1, slot = q / 3, d2 = q - 3 * slot;
+      const int d1 = k / (3 * degc), r = k - 3 * degc * d1, slot = r / 3, d2 = r - 3 * slot;
       double v = acc[k];
-      if (!cf[d1] && slot == self && d2 == d1) v = 1.0;
+      if (!cfree[3 * I + d1] || !cfree[3 * (int64_t)crow[slot] + d2]) v = 0.0;
+      if (!cfree[3 * I + d1] && slot == self && d2 == d1) v = 1.0;
       cval[9 * c0 + k] = v;
     }
     __syncthreads();
   }
+}
+
+// b_c = P^T r: one wave per coarse node over its transpose list, shuffle reduction.
+__global__ __launch_bounds__(256) void k_sa_restrict(int64_t nc, const int64_t* __restrict__ tptr,
+                                                     const int32_t* __restrict__ tidx,
+                                                     const int64_t* __restrict__ tpos, const double* __restrict__ pval,
+                                                     const double* __restrict__ rf, const uint8_t* __restrict__ cfree,
+                                                     double* __restrict__ bc) {
+  const int64_t I = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (I < nc)
+    for (int64_t q = tptr[I] + lane; q < tptr[I + 1]; q += 64) {
+      const int64_t i = tidx[q];
+      const double* P = pval + 9 * tpos[q];
+      const double r0 = rf[3 * i], r1 = rf[3 * i + 1], r2 = rf[3 * i + 2];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) acc[d] += P[d] * r0 + P[3 + d] * r1 + P[6 + d] * r2;
+    }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) acc[d] += __shfl_xor(acc[d], o, 64);
+  if (I >= nc || lane != 0) return;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) bc[3 * I + d] = cfree[3 * I + d] ? acc[d] : 0.0;
+}
+
+// x_f += P x_c (rows of constrained fine dofs are zero).
+__global__ __launch_bounds__(256) void k_sa_prolong(int64_t nf, const int64_t* __restrict__ pptr,
+                                                    const int32_t* __restrict__ pidx, const double* __restrict__ pval,
+                                                    const double* __restrict__ xc, double* __restrict__ xf) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nf) return;
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int64_t k = pptr[i]; k < pptr[i + 1]; ++k) {
+    const double* P = pval + 9 * k;
+    const int64_t J = pidx[k];
+    const double x0 = xc[3 * J], x1 = xc[3 * J + 1], x2 = xc[3 * J + 2];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) s[d] += P[3 * d] * x0 + P[3 * d + 1] * x1 + P[3 * d + 2] * x2;
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) xf[3 * i + d] += s[d];
 }
 
 // Galerkin product on node blocks: one workgroup (64 threads) per coarse node row,
@@ -697,15 +798,52 @@ static int64_t aggregate(int64_t n, const std::vector<int64_t>& ptr, const std::
 }
 
 #define FEDD_AMG_COARSEST 64  // nodes on the coarsest aggregation level (at most)
-// Scaling of the aggregation coarse-grid correction (over-correction compensates the
-// low energy of piecewise-constant interpolation; the V-cycle stays symmetric).
-#ifndef FEDD_AMG_ALPHA
-#define FEDD_AMG_ALPHA 1.8
+// Damping of the prolongator smoothing P = (I - omega D^-1 A) P_tent.
+#ifndef FEDD_SA_OMEGA
+#define FEDD_SA_OMEGA 0.67
 #endif
 // Strong coupling on the finest level: ||A_ij||_F >= theta sqrt(||A_ii||_F ||A_jj||_F).
 #ifndef FEDD_AMG_THETA
 #define FEDD_AMG_THETA 0.1
 #endif
+
+// Sorted, duplicate-free rows: row r holds every column that rowfn(r, emit) emits
+// (columns < ncols).  Built over host threads with a stamp array per thread.
+template <class RowFn>
+static void union_rows(int64_t nrows, int64_t ncols, RowFn rowfn, std::vector<int64_t>& optr,
+                       std::vector<int32_t>& oidx, int64_t* maxdeg) {
+  const unsigned T = nthreads();
+  std::vector<std::vector<int32_t>> part_idx(T);
+  std::vector<std::vector<int64_t>> part_len(T);
+  std::vector<int64_t> part_begin(T, nrows);
+  parallel_for(nrows, [&](int64_t a, int64_t b, int t) {
+    std::vector<int32_t> stamp(ncols, -1), row;
+    part_begin[t] = a;
+    for (int64_t r = a; r < b; ++r) {
+      row.clear();
+      rowfn(r, [&](int32_t col) {
+        if (stamp[col] != (int32_t)r) {
+          stamp[col] = (int32_t)r;
+          row.push_back(col);
+        }
+      });
+      std::sort(row.begin(), row.end());
+      part_idx[t].insert(part_idx[t].end(), row.begin(), row.end());
+      part_len[t].push_back((int64_t)row.size());
+    }
+  });
+  optr.assign(nrows + 1, 0);
+  oidx.clear();
+  for (unsigned t = 0; t < T; ++t) {
+    int64_t r = part_begin[t];
+    for (int64_t l : part_len[t]) {
+      optr[r + 1] = optr[r] + l;
+      *maxdeg = std::max(*maxdeg, l);
+      ++r;
+    }
+    oidx.insert(oidx.end(), part_idx[t].begin(), part_idx[t].end());
+  }
+}
 
 // Aggregation hierarchy from the node graph of level 0 (any mesh).
 static int build_aggregation(Ctx* c, MgHost* g, std::vector<std::vector<uint8_t>>& frees) {
@@ -743,51 +881,43 @@ static int build_aggregation(Ctx* c, MgHost* g, std::vector<std::vector<uint8_t>
     if (nc >= F.nn) break;  // no coarsening left
     MgLevel C;
     C.nn = nc;
-    C.h_mptr.assign(nc + 1, 0);
-    for (int64_t i = 0; i < F.nn; ++i) C.h_mptr[agg[i] + 1]++;
-    for (int64_t I = 0; I < nc; ++I) C.h_mptr[I + 1] += C.h_mptr[I];
-    C.h_midx.resize(F.nn);
+    // prolongator graph: row i holds the aggregates of i's neighbours
+    int64_t md = 0;
+    union_rows(F.nn, nc, [&](int64_t i, auto&& emit) {
+      for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k) emit(agg[idx[k]]);
+    }, C.h_pptr, C.h_pidx, &md);
+    // its transpose
+    C.h_tptr.assign(nc + 1, 0);
+    for (int32_t J : C.h_pidx) C.h_tptr[J + 1]++;
+    for (int64_t I = 0; I < nc; ++I) C.h_tptr[I + 1] += C.h_tptr[I];
+    C.h_tidx.resize(C.h_pidx.size());
+    C.h_tpos.resize(C.h_pidx.size());
     {
-      std::vector<int64_t> fill(C.h_mptr.begin(), C.h_mptr.end() - 1);
-      for (int64_t i = 0; i < F.nn; ++i) C.h_midx[fill[agg[i]]++] = (int32_t)i;
-    }
-    // coarse graph: J neighbours I iff a member of I couples to a member of J
-    const unsigned T = nthreads();
-    std::vector<std::vector<int32_t>> part_idx(T);
-    std::vector<std::vector<int64_t>> part_len(T);
-    std::vector<int64_t> part_begin(T, nc);
-    parallel_for(nc, [&](int64_t a, int64_t b, int t) {
-      std::vector<int32_t> stamp(nc, -1), row;
-      part_begin[t] = a;
-      for (int64_t I = a; I < b; ++I) {
-        row.clear();
-        for (int64_t q = C.h_mptr[I]; q < C.h_mptr[I + 1]; ++q) {
-          const int32_t i = C.h_midx[q];
-          for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k) {
-            const int32_t J = agg[idx[k]];
-            if (stamp[J] != (int32_t)I) {
-              stamp[J] = (int32_t)I;
-              row.push_back(J);
-            }
-          }
+      std::vector<int64_t> fill(C.h_tptr.begin(), C.h_tptr.end() - 1);
+      for (int64_t i = 0; i < F.nn; ++i)
+        for (int64_t k = C.h_pptr[i]; k < C.h_pptr[i + 1]; ++k) {
+          const int64_t q = fill[C.h_pidx[k]]++;
+          C.h_tidx[q] = (int32_t)i;
+          C.h_tpos[q] = k;
         }
-        std::sort(row.begin(), row.end());
-        part_idx[t].insert(part_idx[t].end(), row.begin(), row.end());
-        part_len[t].push_back((int64_t)row.size());
-      }
-    });
-    C.h_ptr.assign(nc + 1, 0);
-    for (unsigned t = 0; t < T; ++t) {
-      int64_t I = part_begin[t];
-      for (int64_t l : part_len[t]) {
-        C.h_ptr[I + 1] = C.h_ptr[I] + l;
-        C.maxdeg = std::max(C.maxdeg, l);
-        ++I;
-      }
-      C.h_idx.insert(C.h_idx.end(), part_idx[t].begin(), part_idx[t].end());
     }
+    // A P graph: row i holds the P columns of i's neighbours
+    union_rows(F.nn, nc, [&](int64_t i, auto&& emit) {
+      for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+        const int32_t j = idx[k];
+        for (int64_t q = C.h_pptr[j]; q < C.h_pptr[j + 1]; ++q) emit(C.h_pidx[q]);
+      }
+    }, C.h_aptr, C.h_aidx, &C.maxdeg_ap);
+    // coarse graph: row I holds the A P columns of every fine row with a block in column I
+    union_rows(nc, nc, [&](int64_t I, auto&& emit) {
+      for (int64_t q = C.h_tptr[I]; q < C.h_tptr[I + 1]; ++q) {
+        const int32_t i = C.h_tidx[q];
+        for (int64_t a = C.h_aptr[i]; a < C.h_aptr[i + 1]; ++a) emit(C.h_aidx[a]);
+      }
+    }, C.h_ptr, C.h_idx, &C.maxdeg);
     C.nblk = C.h_ptr[nc];
-    if (9 * C.maxdeg * 8 > 65536) return fail(c, FEDD_ERR_LIMIT, "aggregation: a coarse row exceeds the LDS budget");
+    if (9 * std::max(C.maxdeg, C.maxdeg_ap) * 8 > 65536)
+      return fail(c, FEDD_ERR_LIMIT, "aggregation: a coarse row exceeds the LDS budget");
     // coarse dof free iff some member's dof is free
     std::vector<uint8_t> cf(3 * nc, 0);
     const std::vector<uint8_t>& ff = frees.back();
@@ -882,9 +1012,17 @@ int mg_setup(Ctx* c, int mode) {
       L.d_nidx = L.nidx.as<int32_t>();
       L.d_vals = L.vals.as<double>();
       if (mode == 1) {
-        if ((rc = upload(c, L.agg, L.h_agg.data(), L.h_agg.size() * 4))) return rc;
-        if ((rc = upload(c, L.mptr, L.h_mptr.data(), L.h_mptr.size() * 8))) return rc;
-        if ((rc = upload(c, L.midx, L.h_midx.data(), L.h_midx.size() * 4))) return rc;
+        if ((rc = upload(c, L.agg, L.h_agg.data(), L.h_agg.size() * 4)) ||
+            (rc = upload(c, L.pptr, L.h_pptr.data(), L.h_pptr.size() * 8)) ||
+            (rc = upload(c, L.pidx, L.h_pidx.data(), std::max<size_t>(L.h_pidx.size(), 1) * 4)) ||
+            (rc = alloc(c, L.pval, std::max<size_t>(L.h_pidx.size(), 1) * 9 * 8)) ||
+            (rc = upload(c, L.tptr, L.h_tptr.data(), L.h_tptr.size() * 8)) ||
+            (rc = upload(c, L.tidx, L.h_tidx.data(), std::max<size_t>(L.h_tidx.size(), 1) * 4)) ||
+            (rc = upload(c, L.tpos, L.h_tpos.data(), std::max<size_t>(L.h_tpos.size(), 1) * 8)) ||
+            (rc = upload(c, L.aptr, L.h_aptr.data(), L.h_aptr.size() * 8)) ||
+            (rc = upload(c, L.aidx, L.h_aidx.data(), std::max<size_t>(L.h_aidx.size(), 1) * 4)) ||
+            (rc = alloc(c, L.aval, std::max<size_t>(L.h_aidx.size(), 1) * 9 * 8)))
+          return rc;
       }
     }
     if ((rc = upload(c, L.free, frees[l].data(), frees[l].size()))) return rc;
@@ -896,8 +1034,8 @@ int mg_setup(Ctx* c, int mode) {
     std::vector<int64_t>().swap(L.h_ptr);
     std::vector<int32_t>().swap(L.h_idx);
     std::vector<int32_t>().swap(L.h_agg);
-    std::vector<int64_t>().swap(L.h_mptr);
-    std::vector<int32_t>().swap(L.h_midx);
+    for (auto* v : {&L.h_pptr, &L.h_tptr, &L.h_tpos, &L.h_aptr}) std::vector<int64_t>().swap(*v);
+    for (auto* v : {&L.h_pidx, &L.h_tidx, &L.h_aidx}) std::vector<int32_t>().swap(*v);
   }
   g->ncoarse = 3 * g->lv.back().nn;
   if ((rc = alloc(c, g->inv, g->ncoarse * g->ncoarse * 8))) return rc;
@@ -915,11 +1053,20 @@ static int refresh_eager(Ctx* c) {
       const MgLevel& F = g->lv[l - 1];
       const unsigned grid = (unsigned)std::min<int64_t>(L.nn, (int64_t)c->num_cu * 16);
       ktimer_begin(c);
-      if (g->mode == 1)
-        k_amg_galerkin<<<grid, 64, (size_t)9 * L.maxdeg * 8, c->stream>>>(
-            L.nn, L.mptr.as<int64_t>(), L.midx.as<int32_t>(), L.agg.as<int32_t>(), F.d_nptr, F.d_nidx, F.d_vals,
-            F.free.as<uint8_t>(), L.d_nptr, L.d_nidx, L.d_vals, L.free.as<uint8_t>());
-      else
+      if (g->mode == 1) {
+        k_sa_smooth_p<<<nbk(F.nn, 256), 256, 0, c->stream>>>(F.nn, F.d_nptr, F.d_nidx, F.d_vals, F.binv.as<double>(),
+                                                            F.free.as<uint8_t>(), L.agg.as<int32_t>(),
+                                                            L.pptr.as<int64_t>(), L.pidx.as<int32_t>(),
+                                                            L.pval.as<double>(), FEDD_SA_OMEGA);
+        const unsigned gf = (unsigned)std::min<int64_t>(F.nn, (int64_t)c->num_cu * 16);
+        k_sa_ap<<<gf, 64, (size_t)9 * L.maxdeg_ap * 8, c->stream>>>(
+            F.nn, F.d_nptr, F.d_nidx, F.d_vals, L.pptr.as<int64_t>(), L.pidx.as<int32_t>(), L.pval.as<double>(),
+            L.aptr.as<int64_t>(), L.aidx.as<int32_t>(), L.aval.as<double>());
+        k_sa_ptap<<<grid, 64, (size_t)9 * L.maxdeg * 8, c->stream>>>(
+            L.nn, L.tptr.as<int64_t>(), L.tidx.as<int32_t>(), L.tpos.as<int64_t>(), L.pval.as<double>(),
+            L.aptr.as<int64_t>(), L.aidx.as<int32_t>(), L.aval.as<double>(), L.d_nptr, L.d_nidx, L.d_vals,
+            L.free.as<uint8_t>());
+      } else
         k_mg_galerkin<<<grid, 64, (size_t)9 * L.maxdeg * 8, c->stream>>>(
             F.P, L.P, F.d_nptr, F.d_nidx, F.d_vals, F.free.as<uint8_t>(), L.d_nptr, L.d_nidx, L.d_vals,
             L.free.as<uint8_t>());
@@ -1001,16 +1148,17 @@ static void vcycle(Ctx* c, size_t l, const double* b, double* x) {
   sweep(2, cur, L.r.as<double>());
   MgLevel& C = g->lv[l + 1];
   if (g->mode == 1)
-    k_amg_restrict<<<nbk(C.nn, 256), 256, 0, c->stream>>>(C.nn, C.mptr.as<int64_t>(), C.midx.as<int32_t>(),
-                                                          L.r.as<double>(), L.free.as<uint8_t>(),
-                                                          C.free.as<uint8_t>(), C.b.as<double>());
+    k_sa_restrict<<<nbk(C.nn * 64, 256), 256, 0, c->stream>>>(C.nn, C.tptr.as<int64_t>(), C.tidx.as<int32_t>(),
+                                                              C.tpos.as<int64_t>(), C.pval.as<double>(),
+                                                              L.r.as<double>(), C.free.as<uint8_t>(),
+                                                              C.b.as<double>());
   else
     k_mg_restrict<<<nbk(C.nn * 16, 256), 256, 0, c->stream>>>(L.P, C.P, L.r.as<double>(), L.free.as<uint8_t>(),
                                                               C.free.as<uint8_t>(), C.b.as<double>());
   vcycle(c, l + 1, C.b.as<double>(), C.x.as<double>());
   if (g->mode == 1)
-    k_amg_prolong<<<nbk(L.nn, 256), 256, 0, c->stream>>>(L.nn, C.agg.as<int32_t>(), C.x.as<double>(),
-                                                         L.free.as<uint8_t>(), cur, FEDD_AMG_ALPHA);
+    k_sa_prolong<<<nbk(L.nn, 256), 256, 0, c->stream>>>(L.nn, C.pptr.as<int64_t>(), C.pidx.as<int32_t>(),
+                                                        C.pval.as<double>(), C.x.as<double>(), cur);
   else
     k_mg_prolong<<<nbk(L.nn, 256), 256, 0, c->stream>>>(L.P, C.P, C.x.as<double>(), L.free.as<uint8_t>(),
                                                         C.free.as<uint8_t>(), cur);

@@ -209,9 +209,9 @@ int fedd_traction_load(fedd_ctx* ctx, int axis, double value, const double* trac
 int fedd_linear_solve(fedd_ctx* ctx, double rtol, int maxit, int* iters, double* relres);
 /* 0: 3x3 block Jacobi (any mesh); 1: geometric multigrid V-cycle on the structured
  * lattice (P2 -> P1 -> coarser P1, Galerkin coarse operators, damped block-Jacobi
- * smoothing, dense coarsest solve); 2: aggregation multigrid on any mesh (greedy node
- * aggregates, piecewise-constant transfers per dof component, Galerkin coarse operators,
- * the same smoother and coarsest solve).  1 and 2 stand in for the reference's
+ * smoothing, dense coarsest solve); 2: smoothed-aggregation multigrid on any mesh (greedy
+ * node aggregates, prolongator (I - omega D^-1 A) P_tent with P_tent piecewise constant per
+ * dof component, Galerkin coarse operators, the same smoother and coarsest solve).  1 and 2 stand in for the reference's
  * FROSch/MueLu preconditioners (Preconditioner_def.hpp:206-291). */
 int fedd_set_preconditioner(fedd_ctx* ctx, int kind);
 
