@@ -936,16 +936,25 @@ static int build_aggregation(Ctx* c, MgHost* g, std::vector<std::vector<uint8_t>
   return FEDD_OK;
 }
 
+static int mg_setup_levels(Ctx* c, MgHost* g, int mode, int64_t P0);
+
+// A failed setup leaves no hierarchy behind (the next call starts over).
 int mg_setup(Ctx* c, int mode) {
   if (c->mg && c->mg->mode == mode) return FEDD_OK;
   mg_free(c);
-  const Mesh& m = c->mesh[0];
   int64_t P0 = 0;
-  if (mode == 0 && !lattice_mesh(m, P0))
+  if (mode == 0 && !lattice_mesh(c->mesh[0], P0))
     return fail(c, FEDD_ERR_STATE, "multigrid needs the structured P2 lattice mesh of fedd_mesh_structured");
   MgHost* g = new MgHost();
   c->mg = g;
   g->mode = mode;
+  const int rc = mg_setup_levels(c, g, mode, P0);
+  if (rc) mg_free(c);
+  return rc;
+}
+
+static int mg_setup_levels(Ctx* c, MgHost* g, int mode, int64_t P0) {
+  const Mesh& m = c->mesh[0];
   // level 0: the context's operator (9 values per node block)
   {
     MgLevel L0;
@@ -958,10 +967,7 @@ int mg_setup(Ctx* c, int mode) {
   for (size_t k = 0; k < c->bcflag_host.size() && k < frees[0].size(); ++k) frees[0][k] = c->bcflag_host[k] ? 0 : 1;
   int rc;
   if (mode == 1) {
-    if ((rc = build_aggregation(c, g, frees))) {
-      mg_free(c);
-      return rc;
-    }
+    if ((rc = build_aggregation(c, g, frees))) return rc;
   } else {
     // Coarse levels: P1 lattices.  The Galerkin product of the midpoint prolongation on the
     // Kuhn lattice couples a coarse node exactly with itself and its 14 Kuhn neighbours
