@@ -805,10 +805,78 @@ static int64_t aggregate(int64_t n, const std::vector<int64_t>& ptr, const std::
 #ifndef FEDD_SA_OMEGA
 #define FEDD_SA_OMEGA 0.67
 #endif
-// Strong coupling on the finest level: ||A_ij||_F >= theta sqrt(||A_ii||_F ||A_jj||_F).
+// Levels whose average row holds at most this many blocks aggregate over strong couplings only.
+#ifndef FEDD_AMG_FILTER_DEG
+#define FEDD_AMG_FILTER_DEG 64
+#endif
+// Strong coupling: ||A_ij||_F >= theta sqrt(||A_ii||_F ||A_jj||_F).
 #ifndef FEDD_AMG_THETA
 #define FEDD_AMG_THETA 0.1
 #endif
+
+// Device arrays of level l (l = 0 uses the context's operator) and its Dirichlet mask;
+// the host copies of the level's graphs are released afterwards.
+static int level_device_setup(Ctx* c, MgLevel& L, size_t l, int mode, const std::vector<uint8_t>& fr) {
+  int rc;
+  if (l == 0) {
+    L.d_nptr = c->d_nptr.as<int64_t>();
+    L.d_nidx = c->d_nidx.as<int32_t>();
+    L.d_vals = c->blk[0].d_vals.as<double>();
+  } else {
+    if ((rc = upload(c, L.nptr, L.h_ptr.data(), L.h_ptr.size() * 8))) return rc;
+    if ((rc = upload(c, L.nidx, L.h_idx.data(), L.h_idx.size() * 4))) return rc;
+    if ((rc = alloc(c, L.vals, L.nblk * 9 * 8))) return rc;
+    L.d_nptr = L.nptr.as<int64_t>();
+    L.d_nidx = L.nidx.as<int32_t>();
+    L.d_vals = L.vals.as<double>();
+    if (mode == 1) {
+      if ((rc = upload(c, L.agg, L.h_agg.data(), L.h_agg.size() * 4)) ||
+          (rc = upload(c, L.pptr, L.h_pptr.data(), L.h_pptr.size() * 8)) ||
+          (rc = upload(c, L.pidx, L.h_pidx.data(), std::max<size_t>(L.h_pidx.size(), 1) * 4)) ||
+          (rc = alloc(c, L.pval, std::max<size_t>(L.h_pidx.size(), 1) * 9 * 8)) ||
+          (rc = upload(c, L.tptr, L.h_tptr.data(), L.h_tptr.size() * 8)) ||
+          (rc = upload(c, L.tidx, L.h_tidx.data(), std::max<size_t>(L.h_tidx.size(), 1) * 4)) ||
+          (rc = upload(c, L.tpos, L.h_tpos.data(), std::max<size_t>(L.h_tpos.size(), 1) * 8)) ||
+          (rc = upload(c, L.aptr, L.h_aptr.data(), L.h_aptr.size() * 8)) ||
+          (rc = upload(c, L.aidx, L.h_aidx.data(), std::max<size_t>(L.h_aidx.size(), 1) * 4)) ||
+          (rc = alloc(c, L.aval, std::max<size_t>(L.h_aidx.size(), 1) * 9 * 8)))
+        return rc;
+    }
+  }
+  if ((rc = upload(c, L.free, fr.data(), fr.size()))) return rc;
+  for (DevBuf* b : {&L.x, &L.x2, &L.b, &L.r})
+    if ((rc = alloc(c, *b, L.nn * 3 * 8))) return rc;
+  if ((rc = alloc(c, L.binv, L.nn * 9 * 8))) return rc;
+  if (l == 0 && FEDD_MG_FP32_FINE && (rc = alloc(c, L.vals32, L.nblk * 9 * 4))) return rc;
+  std::vector<int64_t>().swap(L.h_ptr);
+  std::vector<int32_t>().swap(L.h_idx);
+  std::vector<int32_t>().swap(L.h_agg);
+  for (auto* v : {&L.h_pptr, &L.h_tptr, &L.h_tpos, &L.h_aptr}) std::vector<int64_t>().swap(*v);
+  for (auto* v : {&L.h_pidx, &L.h_tidx, &L.h_aidx}) std::vector<int32_t>().swap(*v);
+  return FEDD_OK;
+}
+
+// Smoothed-aggregation products for level l from level l-1 (its operator and block
+// inverse current): P, A P, P^T A P.
+static int sa_galerkin(Ctx* c, MgHost* g, size_t l) {
+  const MgLevel& F = g->lv[l - 1];
+  MgLevel& L = g->lv[l];
+  k_sa_smooth_p<<<nbk(F.nn, 256), 256, 0, c->stream>>>(F.nn, F.d_nptr, F.d_nidx, F.d_vals, F.binv.as<double>(),
+                                                      F.free.as<uint8_t>(), L.agg.as<int32_t>(), L.pptr.as<int64_t>(),
+                                                      L.pidx.as<int32_t>(), L.pval.as<double>(), FEDD_SA_OMEGA);
+  const unsigned gf = (unsigned)std::min<int64_t>(F.nn, (int64_t)c->num_cu * 16);
+  k_sa_ap<<<gf, 64, (size_t)9 * L.maxdeg_ap * 8, c->stream>>>(F.nn, F.d_nptr, F.d_nidx, F.d_vals,
+                                                               L.pptr.as<int64_t>(), L.pidx.as<int32_t>(),
+                                                               L.pval.as<double>(), L.aptr.as<int64_t>(),
+                                                               L.aidx.as<int32_t>(), L.aval.as<double>());
+  const unsigned gc = (unsigned)std::min<int64_t>(L.nn, (int64_t)c->num_cu * 16);
+  k_sa_ptap<<<gc, 64, (size_t)9 * L.maxdeg * 8, c->stream>>>(
+      L.nn, L.tptr.as<int64_t>(), L.tidx.as<int32_t>(), L.tpos.as<int64_t>(), L.pval.as<double>(),
+      L.aptr.as<int64_t>(), L.aidx.as<int32_t>(), L.aval.as<double>(), L.d_nptr, L.d_nidx, L.d_vals,
+      L.free.as<uint8_t>());
+  FEDD_HIP(c, hipGetLastError());
+  return FEDD_OK;
+}
 
 // Sorted, duplicate-free rows: row r holds every column that rowfn(r, emit) emits
 // (columns < ncols).  Built over host threads with a stamp array per thread.
@@ -855,32 +923,41 @@ static int build_aggregation(Ctx* c, MgHost* g, std::vector<std::vector<uint8_t>
   FEDD_HIP(c, hipMemcpy(ptr.data(), c->d_nptr.p, (n0 + 1) * 8, hipMemcpyDeviceToHost));
   std::vector<int32_t> idx(ptr[n0]);
   FEDD_HIP(c, hipMemcpy(idx.data(), c->d_nidx.p, idx.size() * 4, hipMemcpyDeviceToHost));
-  // strength of the finest level's couplings, from the current operator
-  std::vector<uint8_t> strong(idx.size(), 1);
-  {
+  // Strength of the couplings of the current finest-so-far level F, from its operator on
+  // the device (level 0: the context's matrix; coarser: the products just formed).
+  auto strength = [&](const MgLevel& F, std::vector<uint8_t>& strong) -> int {
+    const int64_t nb = (int64_t)idx.size();
     DevBuf d_nrm;
-    int rc = alloc(c, d_nrm, std::max<size_t>(idx.size(), 1) * 4);
+    int rc = alloc(c, d_nrm, std::max<int64_t>(nb, 1) * 4);
     if (rc) return rc;
-    k_amg_block_norm<<<nbk(n0, 256), 256, 0, c->stream>>>(n0, c->d_nptr.as<int64_t>(), c->blk[0].d_vals.as<double>(),
-                                                          d_nrm.as<float>());
+    k_amg_block_norm<<<nbk(F.nn, 256), 256, 0, c->stream>>>(F.nn, F.d_nptr, F.d_vals, d_nrm.as<float>());
     FEDD_HIP(c, hipGetLastError());
-    std::vector<float> nrm(idx.size());
-    FEDD_HIP(c, hipMemcpyAsync(nrm.data(), d_nrm.p, nrm.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    std::vector<float> nrm(nb);
+    FEDD_HIP(c, hipMemcpyAsync(nrm.data(), d_nrm.p, nb * 4, hipMemcpyDeviceToHost, c->stream));
     FEDD_HIP(c, hipStreamSynchronize(c->stream));
-    std::vector<float> diag(n0, 0.f);
-    for (int64_t i = 0; i < n0; ++i)
+    std::vector<float> diag(F.nn, 0.f);
+    for (int64_t i = 0; i < F.nn; ++i)
       for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k)
         if (idx[k] == i) diag[i] = nrm[k];
-    for (int64_t i = 0; i < n0; ++i)
+    strong.assign(nb, 1);
+    for (int64_t i = 0; i < F.nn; ++i)
       for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k)
         strong[k] = idx[k] == i || nrm[k] >= (float)FEDD_AMG_THETA * std::sqrt(diag[i] * diag[idx[k]]);
-  }
-  bool first_level = true;
+    return FEDD_OK;
+  };
   while (g->lv.back().nn > FEDD_AMG_COARSEST && g->lv.size() < 16) {
     const MgLevel& F = g->lv.back();
+    int rc;
+    // F's block inverse (for the prolongator smoothing) and the strength of its couplings
+    k_mg_block_diag<<<nbk(F.nn, 256), 256, 0, c->stream>>>(F.nn, F.d_nptr, F.d_nidx, F.d_vals, F.binv.as<double>());
+    FEDD_HIP(c, hipGetLastError());
+    // Levels with wide rows (the smoothed products widen the coarse stencils) are
+    // aggregated over all their couplings, which keeps the hierarchy short.
+    std::vector<uint8_t> strong;
+    const bool filter = (int64_t)idx.size() <= FEDD_AMG_FILTER_DEG * F.nn;
+    if (filter && (rc = strength(F, strong))) return rc;
     std::vector<int32_t> agg;
-    const int64_t nc = aggregate(F.nn, ptr, idx, first_level ? &strong : nullptr, agg);
-    first_level = false;
+    const int64_t nc = aggregate(F.nn, ptr, idx, filter ? &strong : nullptr, agg);
     if (nc >= F.nn) break;  // no coarsening left
     MgLevel C;
     C.nn = nc;
@@ -931,6 +1008,9 @@ static int build_aggregation(Ctx* c, MgHost* g, std::vector<std::vector<uint8_t>
     ptr = C.h_ptr;
     idx = C.h_idx;
     g->lv.push_back(std::move(C));
+    const size_t l = g->lv.size() - 1;
+    if ((rc = level_device_setup(c, g->lv[l], l, 1, frees[l]))) return rc;
+    if ((rc = sa_galerkin(c, g, l))) return rc;
   }
   if (g->lv.back().nn > 2048) return fail(c, FEDD_ERR_LIMIT, "aggregation stalled above 2048 coarsest nodes");
   return FEDD_OK;
@@ -967,6 +1047,7 @@ static int mg_setup_levels(Ctx* c, MgHost* g, int mode, int64_t P0) {
   for (size_t k = 0; k < c->bcflag_host.size() && k < frees[0].size(); ++k) frees[0][k] = c->bcflag_host[k] ? 0 : 1;
   int rc;
   if (mode == 1) {
+    if ((rc = level_device_setup(c, g->lv[0], 0, mode, frees[0]))) return rc;
     if ((rc = build_aggregation(c, g, frees))) return rc;
   } else {
     // Coarse levels: P1 lattices.  The Galerkin product of the midpoint prolongation on the
@@ -1006,46 +1087,10 @@ static int mg_setup_levels(Ctx* c, MgHost* g, int mode, int64_t P0) {
       g->lv.push_back(std::move(C));
     }
   }
-  // device arrays and Dirichlet masks
-  for (size_t l = 0; l < g->lv.size(); ++l) {
-    MgLevel& L = g->lv[l];
-    if (l == 0) {
-      L.d_nptr = c->d_nptr.as<int64_t>();
-      L.d_nidx = c->d_nidx.as<int32_t>();
-      L.d_vals = c->blk[0].d_vals.as<double>();
-    } else {
-      if ((rc = upload(c, L.nptr, L.h_ptr.data(), L.h_ptr.size() * 8))) return rc;
-      if ((rc = upload(c, L.nidx, L.h_idx.data(), L.h_idx.size() * 4))) return rc;
-      if ((rc = alloc(c, L.vals, L.nblk * 9 * 8))) return rc;
-      L.d_nptr = L.nptr.as<int64_t>();
-      L.d_nidx = L.nidx.as<int32_t>();
-      L.d_vals = L.vals.as<double>();
-      if (mode == 1) {
-        if ((rc = upload(c, L.agg, L.h_agg.data(), L.h_agg.size() * 4)) ||
-            (rc = upload(c, L.pptr, L.h_pptr.data(), L.h_pptr.size() * 8)) ||
-            (rc = upload(c, L.pidx, L.h_pidx.data(), std::max<size_t>(L.h_pidx.size(), 1) * 4)) ||
-            (rc = alloc(c, L.pval, std::max<size_t>(L.h_pidx.size(), 1) * 9 * 8)) ||
-            (rc = upload(c, L.tptr, L.h_tptr.data(), L.h_tptr.size() * 8)) ||
-            (rc = upload(c, L.tidx, L.h_tidx.data(), std::max<size_t>(L.h_tidx.size(), 1) * 4)) ||
-            (rc = upload(c, L.tpos, L.h_tpos.data(), std::max<size_t>(L.h_tpos.size(), 1) * 8)) ||
-            (rc = upload(c, L.aptr, L.h_aptr.data(), L.h_aptr.size() * 8)) ||
-            (rc = upload(c, L.aidx, L.h_aidx.data(), std::max<size_t>(L.h_aidx.size(), 1) * 4)) ||
-            (rc = alloc(c, L.aval, std::max<size_t>(L.h_aidx.size(), 1) * 9 * 8)))
-          return rc;
-      }
-    }
-    if ((rc = upload(c, L.free, frees[l].data(), frees[l].size()))) return rc;
-    for (DevBuf* b : {&L.x, &L.x2, &L.b, &L.r})
-      if ((rc = alloc(c, *b, L.nn * 3 * 8))) return rc;
-    if ((rc = alloc(c, L.binv, L.nn * 9 * 8))) return rc;
-    if (l == 0 && FEDD_MG_FP32_FINE && (rc = alloc(c, L.vals32, L.nblk * 9 * 4))) return rc;
-    // host copies are only needed for the upload
-    std::vector<int64_t>().swap(L.h_ptr);
-    std::vector<int32_t>().swap(L.h_idx);
-    std::vector<int32_t>().swap(L.h_agg);
-    for (auto* v : {&L.h_pptr, &L.h_tptr, &L.h_tpos, &L.h_aptr}) std::vector<int64_t>().swap(*v);
-    for (auto* v : {&L.h_pidx, &L.h_tidx, &L.h_aidx}) std::vector<int32_t>().swap(*v);
-  }
+  // device arrays and Dirichlet masks (the aggregation builder has set its levels up)
+  if (mode == 0)
+    for (size_t l = 0; l < g->lv.size(); ++l)
+      if ((rc = level_device_setup(c, g->lv[l], l, mode, frees[l]))) return rc;
   g->ncoarse = 3 * g->lv.back().nn;
   if ((rc = alloc(c, g->inv, g->ncoarse * g->ncoarse * 8))) return rc;
   if ((rc = alloc(c, g->dense, g->ncoarse * g->ncoarse * 8))) return rc;
@@ -1063,18 +1108,8 @@ static int refresh_eager(Ctx* c) {
       const unsigned grid = (unsigned)std::min<int64_t>(L.nn, (int64_t)c->num_cu * 16);
       ktimer_begin(c);
       if (g->mode == 1) {
-        k_sa_smooth_p<<<nbk(F.nn, 256), 256, 0, c->stream>>>(F.nn, F.d_nptr, F.d_nidx, F.d_vals, F.binv.as<double>(),
-                                                            F.free.as<uint8_t>(), L.agg.as<int32_t>(),
-                                                            L.pptr.as<int64_t>(), L.pidx.as<int32_t>(),
-                                                            L.pval.as<double>(), FEDD_SA_OMEGA);
-        const unsigned gf = (unsigned)std::min<int64_t>(F.nn, (int64_t)c->num_cu * 16);
-        k_sa_ap<<<gf, 64, (size_t)9 * L.maxdeg_ap * 8, c->stream>>>(
-            F.nn, F.d_nptr, F.d_nidx, F.d_vals, L.pptr.as<int64_t>(), L.pidx.as<int32_t>(), L.pval.as<double>(),
-            L.aptr.as<int64_t>(), L.aidx.as<int32_t>(), L.aval.as<double>());
-        k_sa_ptap<<<grid, 64, (size_t)9 * L.maxdeg * 8, c->stream>>>(
-            L.nn, L.tptr.as<int64_t>(), L.tidx.as<int32_t>(), L.tpos.as<int64_t>(), L.pval.as<double>(),
-            L.aptr.as<int64_t>(), L.aidx.as<int32_t>(), L.aval.as<double>(), L.d_nptr, L.d_nidx, L.d_vals,
-            L.free.as<uint8_t>());
+        int rc = sa_galerkin(c, g, l);
+        if (rc) return rc;
       } else
         k_mg_galerkin<<<grid, 64, (size_t)9 * L.maxdeg * 8, c->stream>>>(
             F.P, L.P, F.d_nptr, F.d_nidx, F.d_vals, F.free.as<uint8_t>(), L.d_nptr, L.d_nidx, L.d_vals,
