@@ -78,7 +78,7 @@ def main():
                          "attainable bandwidth and the calibration of rocprofv3 byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (the elements would be "
+        print("bench.py measures one GPU only: there is no multi-GPU path (the elements would be "
               "partitioned into sub-cubes, one per GPU, with one interface-row exchange per assembly); "
               "run it with --gpus 1.", file=sys.stderr)
         sys.exit(2)
