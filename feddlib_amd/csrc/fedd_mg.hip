@@ -1,7 +1,10 @@
-// Geometric multigrid preconditioner for the P2 vector (elasticity) problem on the
-// structured unit-cube lattice of MeshStructured::buildMesh3D.
+// Multigrid preconditioners for the 3-dof-per-node (elasticity) problem:
+//   mode 0: geometric, on the structured unit-cube lattice of MeshStructured::buildMesh3D;
+//   mode 1: smoothed aggregation, on any mesh (described before build_aggregation below).
+// Both share the node-block level layout, the smoother, the coarsest solve and the
+// replayed graphs.
 //
-// Levels: the P2 space on mesh M (lattice of P0 = 2M+1 points per axis), then P1 on
+// Geometric levels: the P2 space on mesh M (lattice of P0 = 2M+1 points per axis), then P1 on
 // meshes M, M/2, M/4, ... (lattices P_(k+1) = (P_k + 1) / 2) while P_k - 1 is even and
 // P_k >= 5.  Every prolongation is the same lattice rule: a fine node whose lattice
 // coordinates are odd on the axis set S is the midpoint of the Kuhn edge e_S between
