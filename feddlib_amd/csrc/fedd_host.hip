@@ -578,10 +578,6 @@ int fedd_graph_build(fedd_ctx* c, int kind) {
       return rc;
     }
   }
-  if ((rc = launch_geometry(c))) {
-    c->graph_kind = GK_NONE;
-    return rc;
-  }
   FEDD_HIP(c, hipStreamSynchronize(c->stream));
   c->newton_step = 0;
   return FEDD_OK;

@@ -108,8 +108,6 @@ struct Ctx {
   DevBuf d_nptr, d_pptr, d_bptr;  // int64 node-level row pointers (A, B^T, B)
   DevBuf d_nidx;                  // int32 node-level column indices of A (node blocks)
   GatherPlan vplan, pplan;        // velocity (primary) and pressure row gathers
-  DevBuf d_geo;                   // per element: grad lambda_0..3 (12) + |det| + pad, 16 doubles
-  DevBuf d_upad;                  // velocity iterate padded to 4 doubles per node
   DevBuf d_emod;                  // per-element Young's modulus field (E-field elasticity overload)
   DevBuf d_qp;                    // elasticity quadrature-point records (52 doubles each)
   int qp_nq = 0;                  // points per element of the records in d_qp (0: none yet)
@@ -182,7 +180,6 @@ int launch_laplace(Ctx* c, int precision);
 int launch_rhs(Ctx* c, double f);
 int launch_linelas(Ctx* c, double lam, double mu, int precision);
 int launch_stream_copy(Ctx* c, int64_t bytes, int iters, float* ms);
-int launch_geometry(Ctx* c);
 // tangent: 1 = exact law, 2 = learned surrogate (the network writes the tangent slot)
 // Evec: device array of per-element Young's moduli (overrides Emod) or null.
 int launch_elasticity(Ctx* c, int model, double Emod, double nu, int precision, int with_res, int tangent,

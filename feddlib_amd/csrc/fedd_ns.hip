@@ -37,6 +37,14 @@ namespace fedd {
 
 __constant__ int8_t c_perm2[10][10] = FEDD_PERM_P2_INIT;
 __constant__ int8_t c_perm1[4][4] = FEDD_PERM_P1_INIT;
+// PERM rows packed as ten 4-bit fields: one load per pair instead of ten.
+constexpr uint64_t perm_code(int li) {
+  uint64_t v = 0;
+  for (int c = 0; c < 10; ++c) v |= (uint64_t)kPermP2[li][c] << (4 * c);
+  return v;
+}
+__constant__ uint64_t c_perm_code[10] = {perm_code(0), perm_code(1), perm_code(2), perm_code(3), perm_code(4),
+                                         perm_code(5), perm_code(6), perm_code(7), perm_code(8), perm_code(9)};
 
 namespace ft = fedd_tensors;
 
@@ -221,9 +229,6 @@ struct NsArgs {
   int64_t npts;
   const int32_t* conn;     // P2 connectivity SoA
   const int32_t* pconn;    // P1 connectivity SoA
-  const int32_t* conn16;   // P2 connectivity AoS, 16 ints per element
-  const double* geo;       // element geometry cache (16 doubles per element)
-  const double* upad;      // velocity padded to 4 doubles per node
   int64_t nelem;
   const double* u;         // velocity iterate, 3 per node
   const double* p;         // pressure iterate
@@ -255,36 +260,31 @@ template <typename T, int CLS>
 struct VelRow {
   // Computes the canonical row of one pair and adds it into LDS.
   //   e      element, m = pair meta (loc, node-in-chunk, 10 column slots, 4 pressure slots)
-  //   Binv'  rows grad lambda_{perm[1..3]} of the cached element geometry
-  //   u      the ten canonical nodes' velocities (padded copy, 2 loads per node)
+  //   Binv'  inverse of the relabelled element's affine map, from its vertex coordinates
+  //   u      the ten canonical nodes' velocities
   static __device__ __forceinline__ void run(const NsArgs& a, int32_t e, const uint8_t* m,
                                              double* __restrict__ rowA, int deg,
                                              double* __restrict__ rowBT, int pdeg) {
     using NT = NsTensors<CLS>;
-    const int li = m[0];
-    const int8_t* pm = c_perm2[li];
-    const int32_t* cn = a.conn16 + 16 * (int64_t)e;
-    const double* ge = a.geo + 16 * (int64_t)e;
+    const uint64_t code = c_perm_code[m[0]];
     int32_t nodes[10];
 #pragma unroll
-    for (int c = 0; c < 10; ++c) nodes[c] = cn[pm[c]];
+    for (int c = 0; c < 10; ++c) nodes[c] = a.conn[(int64_t)((code >> (4 * c)) & 15) * a.nelem + e];
+    Geo g;
+    geometry(a.xyz, a.npts, nodes, g);  // of the relabelled element
     T Bi[3][3];
 #pragma unroll
-    for (int ax = 0; ax < 3; ++ax) {
-      const double* gr = ge + 3 * pm[ax + 1];
+    for (int ax = 0; ax < 3; ++ax)
 #pragma unroll
-      for (int d = 0; d < 3; ++d) Bi[ax][d] = (T)gr[d];
-    }
-    const double adet = ge[12];
+      for (int d = 0; d < 3; ++d) Bi[ax][d] = (T)g.Bi[ax][d];
+    const double adet = g.adet;
     T U[10][3];
 #pragma unroll
     for (int k = 0; k < 10; ++k) {
-      const dbl2* up = reinterpret_cast<const dbl2*>(a.upad + 4 * (int64_t)nodes[k]);
-      const dbl2 v0 = up[0];
-      const dbl2 v1 = up[1];
-      U[k][0] = (T)v0.x;
-      U[k][1] = (T)v0.y;
-      U[k][2] = (T)v1.x;
+      const double* uk = a.u + 3 * (int64_t)nodes[k];
+      U[k][0] = (T)uk[0];
+      U[k][1] = (T)uk[1];
+      U[k][2] = (T)uk[2];
     }
     // S_c = nu rho A_rc + rho N_rc   (without |det|)
     T S[10];
@@ -467,50 +467,6 @@ __global__ __launch_bounds__(FEDD_ROW_BLOCK) __attribute__((amdgpu_waves_per_eu(
     }
     lds_sync();  // the rows are read out before the next chunk zeroes them
   }
-}
-
-// Element geometry cache: grad lambda_0..3 (rows of Binv and minus their sum) and
-// |det B| per element, from the first four (vertex) nodes.
-__global__ __launch_bounds__(256) void k_geometry(const double* __restrict__ xyz, int64_t npts,
-                                                   const int32_t* __restrict__ conn, int64_t nelem,
-                                                   double* __restrict__ geo) {
-  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e >= nelem) return;
-  int32_t nodes[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) nodes[k] = conn[(int64_t)k * nelem + e];
-  Geo g;
-  geometry(xyz, npts, nodes, g);
-  double* o = geo + 16 * e;
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    o[3 + d] = g.Bi[0][d];
-    o[6 + d] = g.Bi[1][d];
-    o[9 + d] = g.Bi[2][d];
-    o[d] = -(g.Bi[0][d] + g.Bi[1][d] + g.Bi[2][d]);
-  }
-  o[12] = g.adet;
-  o[13] = o[14] = o[15] = 0.0;
-}
-
-// Velocity iterate padded to 4 doubles per node (two 16-byte loads per node).
-__global__ __launch_bounds__(256) void k_pad_u(int64_t n, const double* __restrict__ u,
-                                                double* __restrict__ up) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  dbl2* o = reinterpret_cast<dbl2*>(up + 4 * i);
-  o[0] = dbl2{u[3 * i], u[3 * i + 1]};
-  o[1] = dbl2{u[3 * i + 2], 0.0};
-}
-
-int launch_geometry(Ctx* c) {
-  const Mesh& m = c->mesh[0];
-  int rc = alloc(c, c->d_geo, (size_t)m.nelem * 16 * 8);
-  if (rc) return rc;
-  k_geometry<<<nblk(m.nelem, 256), 256, 0, c->stream>>>(m.d_xyz.as<double>(), m.npts, m.d_conn.as<int32_t>(),
-                                                       m.nelem, c->d_geo.as<double>());
-  FEDD_HIP(c, hipGetLastError());
-  return FEDD_OK;
 }
 
 // Pressure rows of B (1,0): row p of element matrix rows 30+m (already scaled by -1).
@@ -918,9 +874,6 @@ static NsArgs ns_args(Ctx* c, const fedd_ns_params* p, const GatherPlan& pl) {
   a.npts = c->mesh[0].npts;
   a.conn = c->mesh[0].d_conn.as<int32_t>();
   a.pconn = c->mesh[1].d_conn.as<int32_t>();
-  a.conn16 = c->mesh[0].d_conn16.as<int32_t>();
-  a.geo = c->d_geo.as<double>();
-  a.upad = c->d_upad.as<double>();
   a.nelem = c->mesh[0].nelem;
   a.u = c->vec[FEDD_VEC_U].as<double>();
   a.p = c->vec[FEDD_VEC_P].as<double>();
@@ -950,11 +903,6 @@ int launch_ns_assembly(Ctx* c, const fedd_ns_params* p, int mode, int reassemble
   if (mode == FEDD_NS_JACOBIAN || mode == FEDD_NS_FIXEDPOINT) {
     fedd_ns_params q = *p;
     if (mode == FEDD_NS_FIXEDPOINT) q.newton = 0;  // the fixed-point matrix never holds W (:115-146)
-    const int64_t n2 = c->mesh[0].npts;
-    if (c->d_upad.bytes < (size_t)n2 * 32) {
-      int rc = alloc(c, c->d_upad, (size_t)n2 * 32);
-      if (rc) return rc;
-    }
     NsArgs a = ns_args(c, &q, c->vplan);
     a.with_bt = reassemble ? 0 : 1;
     const size_t lds = (size_t)c->vplan.lds_doubles * 8;
@@ -980,10 +928,6 @@ int launch_ns_assembly(Ctx* c, const fedd_ns_params* p, int mode, int reassemble
       ktimer_end(c, "ns_pres_rows", c->side);
       FEDD_HIP(c, hipEventRecord(c->ev_join, c->side));
     }
-    ktimer_begin(c);
-    k_pad_u<<<nblk(n2, 256), 256, 0, c->stream>>>(n2, a.u, c->d_upad.as<double>());
-    FEDD_HIP(c, hipGetLastError());
-    ktimer_end(c, "pad_u");
     ktimer_begin(c);
     if (f32)
       k_ns_vel_rows<float><<<grid, FEDD_ROW_BLOCK, lds, c->stream>>>(a);

@@ -94,7 +94,7 @@ def test_graph_replay_small(gpu_ctx):
     gpu_ctx.assembly_navier_stokes(NU, RHO, True, N.NS_JACOBIAN, reassemble=False)
     gpu_ctx.assembly_navier_stokes(NU, RHO, True, N.NS_RHS, reassemble=False)
     nodes = gpu_ctx.capture_end()
-    assert nodes >= 5
+    assert nodes >= 4  # B rows, velocity rows, velocity and pressure residual rows
     assert gpu_ctx.newton_step() == step0 + 1
     for rep in range(3):
         ur = u * (1.0 + 0.25 * rep)
