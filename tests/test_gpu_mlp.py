@@ -4,7 +4,7 @@ Forward and gradient kernels are compared with oracle/mlp.py (float64 network wi
 the same bfloat16 operand roundings; the oracle's float64 mode is pinned by finite
 differences and torch autograd in tests/test_mlp_oracle.py).  Training runs on the
 device with the library's own forward/backward/Adam kernels; the trained network is
-judged against the exact law on held-out deformation gradients and inside Newton,
+measured against the exact law on held-out deformation gradients and inside Newton,
 where it must beat the network-free surrogate (the constant small-strain tangent
 CC0, i.e. a zero network) and fall back to the exact tangent when it fails.
 """
