@@ -639,13 +639,13 @@ int fedd_newton_elasticity(fedd_ctx* cc, const fedd_newton_params* P, fedd_newto
   int hist = 0;
   double final_norm = 0.0;
   bool all_converged = true;
+  bool coarse_current = false;  // coarse levels built (mg_reuse)
+  int base_iters = 0, last_iters = 0;
   for (int step = 1; step <= P->load_steps; ++step) {
     const double lam = (double)step / P->load_steps;
     bool surrogate = P->tangent == 1;
     double prev = INFINITY, norm = 0.0;
     bool converged = false;
-    bool coarse_current = false;  // coarse levels rebuilt during this load step
-    int base_iters = 0, last_iters = 0;
     for (int it = 0; it <= P->max_newton; ++it) {
       FEDD_HIP(c, hipEventRecord(ea, c->stream));
       if ((rc = launch_elasticity(c, model, P->E, P->nu, P->precision, 1, surrogate ? 2 : 1))) return rc;
@@ -694,9 +694,10 @@ int fedd_newton_elasticity(fedd_ctx* cc, const fedd_newton_params* P, fedd_newto
       eta = std::max(eta, 0.1 * P->abs_tol / std::max(norm, 1e-300));
       int iters = 0;
       double relres = 0.0;
-      // mg_reuse: within a load step the coarse levels keep the operators of the step's
-      // first tangent until a solve needs more than 2 n0 + 2 iterations (n0: iterations
-      // of the solve right after the rebuild); the finest level always follows A.
+      // mg_reuse: the coarse levels keep the operators of the tangent they were built
+      // for, across Newton and load steps, until a solve needs more than 2 n0 + 2
+      // iterations (n0: iterations of the solve right after the rebuild); the finest
+      // level always follows A.
       const int refresh =
           P->mg_reuse && coarse_current && last_iters <= 2 * base_iters + 2 ? 2 : 1;
       FEDD_HIP(c, hipEventRecord(sa, c->stream));

@@ -234,12 +234,12 @@ typedef struct {
                            (structured lattice meshes of fedd_mesh_structured only),
                            2 aggregation multigrid (any mesh) */
   int mg_reuse;         /* multigrid only.  0: rebuild the hierarchy for every tangent;
-                           1: within a load step keep the coarse levels of the step's first
-                           tangent (the finest level always follows the current tangent)
-                           until a solve needs more than 2 n0 + 2 CG iterations, n0 those
-                           of the first solve after the rebuild.  Only the preconditioner
-                           changes: every solve still meets its tolerance for the current
-                           tangent. */
+                           1: keep the coarse levels built for an earlier tangent, across
+                           Newton and load steps (the finest level always follows the
+                           current tangent), until a solve needs more than 2 n0 + 2 CG
+                           iterations, n0 those of the first solve after the rebuild.  Only
+                           the preconditioner changes: every solve still meets its
+                           tolerance for the current tangent. */
 } fedd_newton_params;
 
 typedef struct {

@@ -100,7 +100,7 @@ def test_mg_needs_the_lattice(gpu_ctx):
 
 
 def test_mg_reuse_of_coarse_levels(gpu_ctx):
-    """mg_reuse keeps the coarse levels within a load step: the same converged
+    """mg_reuse keeps the coarse levels across Newton and load steps: the same converged
     displacement, every Newton step still solved to its tolerance."""
     pts, el, dofs = _problem(6)
     t = np.array([40.0, 0.0, 60.0])
