@@ -73,6 +73,8 @@ def main():
                     help="run only the exact-tangent solves of the Newton leg (profiling runs)")
     ap.add_argument("--large-M", type=int, default=64,
                     help="cells per edge of the graph-replayed FP32-local Jacobian + residual leg (0: skip)")
+    ap.add_argument("--elas-M", type=int, default=32,
+                    help="cells per edge of the hyperelastic assembly leg, jittered P2 cube (0: skip)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the STREAM copy kernel (1 GiB read + 1 GiB write, 20 times): the "
                          "attainable bandwidth and the calibration of rocprofv3 byte counters")
@@ -221,8 +223,45 @@ def main():
         out["newton"] = newton_leg(ctx, N, args.newton_M, exact_only=args.newton_exact_only)
     if args.large_M > 0:
         out["ns_large"] = large_leg(ctx, N, W, args.large_M)
+    if args.elas_M > 0:
+        out["elasticity"] = elasticity_leg(ctx, N, W, args.elas_M)
     ctx.close()
     print(json.dumps(out))
+
+
+def elasticity_leg(ctx, N, W, M, reps=10):
+    """FE::assemblyNonLinearElasticity (skr2 law, tangent + residual) on the jittered P2 cube
+    with M^3 x 6 tets and the smoothed seeded displacement, FP64 and FP32-local: elements/s
+    of the whole assembly (quadrature-point kernel + row gather), inputs resident."""
+    pts, _, el = W.jittered_p2_mesh(M, 0)
+    u = W.smoothed_displacement(pts, el, 1)
+    E = el.shape[0]
+    ctx.mesh_upload(N.MESH_PRIMARY, pts, el)
+    ctx.graph_build(N.GRAPH_VECTOR)
+    ctx.vector_upload(N.VEC_U, u.ravel())
+    nnz = ctx.block_info(N.BLOCK_A)[2]
+    n = pts.shape[0]
+    # values written once, connectivity (10 ids), coordinates and displacement read once,
+    # residual written once
+    alg = 8 * nnz + 4 * 10 * E + 24 * n + 24 * n + 24 * n
+    res = {"workload": "FE::assemblyNonLinearElasticity, Neo-Hooke (skr2) tangent + residual, unit cube %d^3x6 = "
+                       "%d P2 tets, interior vertices jittered by up to 0.2 h, displacement 0.05 N(0,1) smoothed once, "
+                       "E=1000 nu=0.3" % (M, E), "tets": E, "nnz": nnz, "algorithmic_bytes": alg}
+    for name, prec in (("fp64", N.FP64), ("fp32_local", N.FP32_LOCAL)):
+        for _ in range(2):
+            ctx.assembly_nonlinear_elasticity(1, EMOD, POISSON, prec)
+        ctx.synchronize()
+        ctx.event_record(0)
+        for _ in range(reps):
+            ctx.assembly_nonlinear_elasticity(1, EMOD, POISSON, prec)
+        ctx.event_record(1)
+        ctx.synchronize()
+        ms = ctx.event_elapsed(0, 1) / reps
+        res[name] = {"ms": ms, "elems_per_s": E / (ms * 1e-3),
+                     "hbm_frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    print("# elasticity %d tets: %.3f ms fp64, %.3f ms fp32-local" % (E, res["fp64"]["ms"], res["fp32_local"]["ms"]),
+          file=sys.stderr, flush=True)
+    return res
 
 
 def large_leg(ctx, N, W, M, iters=10):
