@@ -135,11 +135,18 @@ struct ElasArgs {
 };
 
 // One thread per (element, quadrature point), original labelling.
-__global__ __launch_bounds__(256) void k_elas_qp(ElasArgs a) {
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= a.nelem * a.nq) return;
-  const int64_t e = t / a.nq;
-  const int m = (int)(t - e * a.nq);
+__global__ __launch_bounds__(64) void k_elas_qp(ElasArgs a) {
+  // One wave per 64 consecutive records: each lane stages its record in LDS, in two
+  // halves of 26 doubles, and the wave writes the 64 records out as contiguous lines.
+  __shared__ double stage[26 * 64];
+  const int64_t nrec = a.nelem * a.nq;
+  const int64_t t0 = (int64_t)blockIdx.x * 64;
+  const int lane = threadIdx.x;
+  const int64_t t = t0 + lane;
+  const bool live = t < nrec;
+  const int64_t tt = live ? t : nrec - 1;  // idle lanes recompute the last record
+  const int64_t e = tt / a.nq;
+  const int m = (int)(tt - e * a.nq);
   const ElasRule R = elas_rule(a.model);
   const int32_t* cn = a.conn16 + 16 * e;
   double x[10][3], uu[10][3];
@@ -166,27 +173,44 @@ __global__ __launch_bounds__(256) void k_elas_qp(ElasArgs a) {
       for (int k = 0; k < 10; ++k) v += uu[k][i] * G[k][A];
       F[i][A] = v;
     }
-  double* o = a.qp + 52 * t;
   double S6[6], CC[21];
+#pragma unroll
+  for (int k = 0; k < 21; ++k) CC[k] = 0.0;  // the surrogate writes the tangent slot afterwards
   exact_law(a.model, a.Evec ? a.Evec[e] : a.Emod, a.nu, F, S6, a.exact_tangent ? CC : nullptr);
+  double rec[52];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
-    for (int A = 0; A < 3; ++A) o[3 * i + A] = F[i][A];
+    for (int A = 0; A < 3; ++A) rec[3 * i + A] = F[i][A];
 #pragma unroll
-  for (int I = 0; I < 6; ++I) o[9 + I] = S6[I];
-  if (a.exact_tangent)
+  for (int I = 0; I < 6; ++I) rec[9 + I] = S6[I];
 #pragma unroll
-    for (int k = 0; k < 21; ++k) o[15 + k] = CC[k];
-  o[36] = (m == 4 ? R.wc : R.wa) * det;
+  for (int k = 0; k < 21; ++k) rec[15 + k] = CC[k];
+  rec[36] = (m == 4 ? R.wc : R.wa) * det;
 #pragma unroll
   for (int A = 0; A < 3; ++A) {
-    o[37 + A] = -(Ji[0][A] + Ji[1][A] + Ji[2][A]);
-    o[40 + A] = Ji[0][A];
-    o[43 + A] = Ji[1][A];
-    o[46 + A] = Ji[2][A];
+    rec[37 + A] = -(Ji[0][A] + Ji[1][A] + Ji[2][A]);
+    rec[40 + A] = Ji[0][A];
+    rec[43 + A] = Ji[1][A];
+    rec[46 + A] = Ji[2][A];
   }
-  o[49] = o[50] = o[51] = 0.0;
+  rec[49] = rec[50] = rec[51] = 0.0;
+  const int64_t nout = (nrec - t0 < 64 ? nrec - t0 : 64) * 52;  // doubles this wave writes
+  double* out = a.qp + 52 * t0;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+#pragma unroll
+    for (int k = 0; k < 26; ++k) stage[lane * 26 + k] = rec[26 * h + k];
+    __syncthreads();
+    // record r's half h covers out[52 r + 26 h .. + 26)
+    for (int j = lane; j < 26 * 64; j += 64) {
+      const int r = j / 26, k = j - 26 * r;
+      const int64_t idx = 52 * (int64_t)r + 26 * h + k;
+      if (idx < nout) out[idx] = stage[j];
+    }
+    __syncthreads();
+  }
+  (void)live;
 }
 
 // Canonical-frame P2 reference gradients at the canonical points (compile-time).
@@ -414,7 +438,7 @@ int launch_elasticity(Ctx* c, int model, double Emod, double nu, int precision, 
   a.with_res = with_res;
   const int64_t nqp = m.nelem * a.nq;
   ktimer_begin(c);
-  k_elas_qp<<<(unsigned)((nqp + 255) / 256), 256, 0, c->stream>>>(a);
+  k_elas_qp<<<(unsigned)((nqp + 63) / 64), 64, 0, c->stream>>>(a);
   FEDD_HIP(c, hipGetLastError());
   ktimer_end(c, "elas_qp");
   if (!exact_tangent) {
