@@ -49,6 +49,22 @@ def profiled_traffic():
     return None
 
 
+FLOP_FILE = "profiles/assembly_valu_flop_counters.txt"
+
+
+def profiled_fp64_flops(kernel="fedd::k_ns_vel_rows<double>"):
+    """FP64 flops per launch of `kernel`, from the committed rocprofv3
+    SQ_INSTS_VALU_FLOPS_FP64 summary (third column of the kernel's line)."""
+    path = os.path.join(ROOT, FLOP_FILE)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.lstrip("# ").split()
+        if f and f[0] == kernel:
+            return float(f[2])
+    return None
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -161,6 +177,7 @@ def main():
     achieved = bytes_vel / (vel_avg_ms * 1e-3) / 1e9
 
     traffic = profiled_traffic() if M == 40 else None
+    flops = profiled_fp64_flops() if M == 40 else None
 
     cpu = None
     if args.cpu_sample > 0:
@@ -206,6 +223,10 @@ def main():
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "traffic_source": TRAFFIC_FILE if traffic else None,
             "algorithmic_bytes_per_launch": bytes_vel, "avg_launch_ms": vel_avg_ms,
+            "fp64_vector": None if flops is None else {
+                "flop_per_launch": flops, "tflops": flops / (vel_avg_ms * 1e-3) / 1e12,
+                "peak_tflops": FP64_PEAK_TFLOPS, "frac": flops / (vel_avg_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                "source": FLOP_FILE},
         },
         "step": {
             "algorithmic_bytes": bytes_step,
