@@ -890,6 +890,47 @@ int fedd_assembly_rhs(fedd_ctx* c, double f) {
   return launch_rhs(c, f);
 }
 
+int fedd_rhs_quadrature_points(fedd_ctx* c, double* xq, int64_t cap, int* nq) {
+  if (!c || !nq) return FEDD_ERR_ARG;
+  if (c->graph_kind != FEDD_GRAPH_SCALAR) return fail(c, FEDD_ERR_STATE, "needs a scalar graph");
+  const Mesh& m = c->mesh[0];
+  *nq = m.npe == 4 ? 1 : 5;
+  if (!xq) return FEDD_OK;
+  if (c->capturing) return fail(c, FEDD_ERR_STATE, "host transfers cannot be captured");
+  const int64_t n = m.nelem * *nq * 3;
+  if (cap < n) return fail(c, FEDD_ERR_ARG, "xq holds fewer than nelem * nq * 3 doubles");
+  (void)hipSetDevice(c->device);
+  if (c->d_rhsq.bytes < (size_t)n * 8) {
+    int rc = alloc(c, c->d_rhsq, (size_t)n * 8);
+    if (rc) return rc;
+  }
+  int rc = launch_rhs_qpoints(c, c->d_rhsq.as<double>(), nq);
+  if (rc) return rc;
+  FEDD_HIP(c, hipMemcpyAsync(xq, c->d_rhsq.p, n * 8, hipMemcpyDeviceToHost, c->stream));
+  FEDD_HIP(c, hipStreamSynchronize(c->stream));
+  return FEDD_OK;
+}
+
+int fedd_assembly_rhs_points(fedd_ctx* c, const double* fq, int64_t n, int ncomp) {
+  if (!c || !fq) return FEDD_ERR_ARG;
+  if (c->graph_kind != FEDD_GRAPH_SCALAR) return fail(c, FEDD_ERR_STATE, "needs a scalar graph");
+  if (ncomp != 1 && ncomp != 3) return fail(c, FEDD_ERR_ARG, "Invalid field type.");
+  if (c->capturing) return fail(c, FEDD_ERR_STATE, "host transfers cannot be captured");
+  const Mesh& m = c->mesh[0];
+  const int nq = m.npe == 4 ? 1 : 5;
+  if (n != m.nelem * nq * ncomp)
+    return fail(c, FEDD_ERR_ARG, "source values: need nelem * nq * ncomp entries");
+  (void)hipSetDevice(c->device);
+  int rc = upload(c, c->d_rhsq, fq, (size_t)n * 8);
+  if (rc) return rc;
+  const int64_t len = (int64_t)ncomp * m.npts;
+  if (c->vec_n[FEDD_VEC_RHS] != len) {
+    if ((rc = alloc(c, c->vec[FEDD_VEC_RHS], len * 8))) return rc;
+    c->vec_n[FEDD_VEC_RHS] = len;
+  }
+  return launch_rhs_points(c, c->d_rhsq.as<double>(), ncomp);
+}
+
 int fedd_assembly_navier_stokes(fedd_ctx* c, const fedd_ns_params* p, int mode, int reassemble,
                                 int precision) {
   if (!c || !p) return FEDD_ERR_ARG;

@@ -110,6 +110,7 @@ struct Ctx {
   GatherPlan vplan, pplan;        // velocity (primary) and pressure row gathers
   DevBuf d_emod;                  // per-element Young's modulus field (E-field elasticity overload)
   DevBuf d_qp;                    // elasticity quadrature-point records (52 doubles each)
+  DevBuf d_rhsq;                  // assemblyRHS: quadrature points / source values at them
   int qp_nq = 0;                  // points per element of the records in d_qp (0: none yet)
   // colouring
   std::vector<int64_t> colour_ptr;
@@ -178,6 +179,8 @@ int expand_graph(Ctx* c, int kind, const int32_t* aidx, const int32_t* btidx, co
 int launch_ns_assembly(Ctx* c, const fedd_ns_params* p, int mode, int reassemble, int precision);
 int launch_laplace(Ctx* c, int precision);
 int launch_rhs(Ctx* c, double f);
+int launch_rhs_qpoints(Ctx* c, double* d_xq, int* nq);
+int launch_rhs_points(Ctx* c, const double* d_fq, int ncomp);
 int launch_linelas(Ctx* c, double lam, double mu, int precision);
 int launch_stream_copy(Ctx* c, int64_t bytes, int iters, float* ms);
 // tangent: 1 = exact law, 2 = learned surrogate (the network writes the tangent slot)

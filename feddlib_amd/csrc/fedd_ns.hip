@@ -832,6 +832,113 @@ __global__ __launch_bounds__(256) void k_rhs_rows(LapArgs a) {
   for (int k = threadIdx.x; k < len; k += blockDim.x) a.rhs[n0 + k] = lds[k];
 }
 
+// FE::assemblyRHS with a general source (FE_def.hpp:7352-7391).  The rule is
+// determineDegree(dim, FEType, Std): P1 -> 1 point, P2 -> degree 2 raised to 3 -> the
+// 5-point rule, in the order of Helper::getQuadratureValues (Helper.cpp:1310-1350).
+struct RhsRule {
+  int nq;
+  double xi[5][3], w[5];
+};
+__host__ __device__ constexpr RhsRule rhs_rule(int npe) {
+  RhsRule r{};
+  if (npe == 4) {
+    r.nq = 1;
+    r.xi[0][0] = r.xi[0][1] = r.xi[0][2] = 0.25;
+    r.w[0] = 1.0 / 6.0;
+  } else {
+    r.nq = 5;
+    const double a = 1.0 / 6.0, b = 0.5;
+    const double pts[5][3] = {{0.25, 0.25, 0.25}, {a, a, a}, {a, a, b}, {a, b, a}, {b, a, a}};
+    for (int q = 0; q < 5; ++q)
+      for (int d = 0; d < 3; ++d) r.xi[q][d] = pts[q][d];
+    r.w[0] = -2.0 / 15.0;
+    for (int q = 1; q < 5; ++q) r.w[q] = 3.0 / 40.0;
+  }
+  return r;
+}
+// w_q phi_i(xi_q) of the rule (Helper::phi, Helper.cpp:660-710), node order of the mesh.
+struct RhsPhiW {
+  double v[10][5];
+};
+__host__ __device__ constexpr RhsPhiW rhs_phiw(int npe) {
+  const RhsRule r = rhs_rule(npe);
+  RhsPhiW t{};
+  for (int q = 0; q < r.nq; ++q) {
+    const double x = r.xi[q][0], y = r.xi[q][1], z = r.xi[q][2], l = 1.0 - x - y - z;
+    double ph[10] = {};
+    if (npe == 4) {
+      ph[0] = l; ph[1] = x; ph[2] = y; ph[3] = z;
+    } else {
+      ph[0] = l * (2 * l - 1); ph[1] = x * (2 * x - 1); ph[2] = y * (2 * y - 1); ph[3] = z * (2 * z - 1);
+      ph[4] = 4 * x * l; ph[5] = 4 * x * y; ph[6] = 4 * y * l; ph[7] = 4 * z * l; ph[8] = 4 * x * z;
+      ph[9] = 4 * y * z;
+    }
+    for (int i = 0; i < npe; ++i) t.v[i][q] = r.w[q] * ph[i];
+  }
+  return t;
+}
+__constant__ RhsRule c_rhs_rule[2] = {rhs_rule(4), rhs_rule(10)};
+__constant__ RhsPhiW c_rhs_phiw[2] = {rhs_phiw(4), rhs_phiw(10)};
+
+// Physical points x_q = x_0 + B xi_q of every element (quadPointsTrans, :7357-7366).
+__global__ __launch_bounds__(256) void k_rhs_qpoints(const double* __restrict__ xyz, int64_t npts,
+                                                      const int32_t* __restrict__ conn, int64_t nelem, int npe,
+                                                      double* __restrict__ xq) {
+  const RhsRule& R = c_rhs_rule[npe == 4 ? 0 : 1];
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nelem * R.nq) return;
+  const int64_t e = t / R.nq;
+  const int q = (int)(t - e * R.nq);
+  double x[4][3];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int32_t n = conn[(int64_t)k * nelem + e];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) x[k][d] = xyz[d * npts + n];
+  }
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    double v = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v += (x[k + 1][j] - x[0][j]) * R.xi[q][k];
+    xq[3 * t + j] = v + x[0][j];
+  }
+}
+
+// b_(i,d) += |det B| sum_q w_q phi_i(xi_q) f_d(x_q), fq[(e nq + q) ncomp + d], row gather
+// over the scalar plan; rows ncomp i + d.
+__global__ __launch_bounds__(256) void k_rhs_points_rows(LapArgs a, const double* __restrict__ fq, int ncomp) {
+  extern __shared__ double lds[];
+  const int sel = a.npe == 4 ? 0 : 1;
+  const int nq = c_rhs_rule[sel].nq;
+  const int ch = (int)xcd_chunk(blockIdx.x, gridDim.x);
+  const int n0 = a.chunk_node[ch], n1 = a.chunk_node[ch + 1];
+  const int q0 = a.chunk_pair[ch], q1 = a.chunk_pair[ch + 1];
+  const int len = ncomp * (n1 - n0);
+  for (int k = threadIdx.x; k < len; k += blockDim.x) lds[k] = 0.0;
+  __syncthreads();
+  for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+    const int32_t e = a.pair_elem[q];
+    const uint4 mv = a.pair_meta[q];
+    const uint8_t* m = reinterpret_cast<const uint8_t*>(&mv);
+    const int li = m[0];
+    int32_t nodes[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) nodes[c] = a.conn[(int64_t)c * a.nelem + e];
+    Geo g;
+    geometry(a.xyz, a.npts, nodes, g);
+    const double* pw = c_rhs_phiw[sel].v[li];
+    const double* fe = fq + (int64_t)e * nq * ncomp;
+    for (int d = 0; d < ncomp; ++d) {
+      double v = 0.0;
+      for (int k = 0; k < nq; ++k) v += pw[k] * fe[k * ncomp + d];
+      atomicAdd(lds + ncomp * m[1] + d, v * g.adet);
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < len; k += blockDim.x) a.rhs[(int64_t)ncomp * n0 + k] = lds[k];
+}
+
 // ---------------------------------------------------------------------------
 // STREAM-style copy (attainable HBM bandwidth; byte-counter calibration)
 // ---------------------------------------------------------------------------
@@ -1002,6 +1109,25 @@ int launch_linelas(Ctx* c, double lam, double mu, int precision) {
   }
   FEDD_HIP(c, hipGetLastError());
   ktimer_end(c, "linelas_rows");
+  return FEDD_OK;
+}
+
+int launch_rhs_qpoints(Ctx* c, double* d_xq, int* nq) {
+  const Mesh& m = c->mesh[0];
+  *nq = m.npe == 4 ? 1 : 5;
+  const int64_t n = m.nelem * *nq;
+  k_rhs_qpoints<<<nblk(n, 256), 256, 0, c->stream>>>(m.d_xyz.as<double>(), m.npts, m.d_conn.as<int32_t>(), m.nelem,
+                                                     m.npe, d_xq);
+  FEDD_HIP(c, hipGetLastError());
+  return FEDD_OK;
+}
+
+int launch_rhs_points(Ctx* c, const double* d_fq, int ncomp) {
+  LapArgs a = lap_args(c);
+  ktimer_begin(c);
+  k_rhs_points_rows<<<(unsigned)c->vplan.nchunks, 256, (size_t)ncomp * 256 * 8, c->stream>>>(a, d_fq, ncomp);
+  FEDD_HIP(c, hipGetLastError());
+  ktimer_end(c, "rhs_points_rows");
   return FEDD_OK;
 }
 

@@ -20,8 +20,8 @@
 // Differences a caller can see:
 //   * matrices: the assembled matrix replaces the contents of A (the reference's callers
 //     pass freshly created matrices, into which insertGlobalValues sums);
-//   * assemblyRHS: sources must be constant in space (FEDDLib's own tests use constant
-//     ones); a varying source raises std::logic_error instead of being integrated;
+//   * assemblyRHS evaluates the source callback on the host at the quadrature points the
+//     GPU reports, as the reference does inside its element loop;
 //   * one device context per domain (pair), created on first use.
 #pragma once
 
@@ -238,7 +238,9 @@ class FE {
   }
 
   // FE::assemblyRHS (FE_def.hpp:7303-7395): b_i += |det B| sum_w w phi_i(x_w) f(x_w),
-  // "Scalar" (one entry per node) or "Vector" (dim entries per node).
+  // "Scalar" (one entry per node) or "Vector" (dim entries per node).  func is called at
+  // the physical quadrature points of the reference's rule, on the host as in the
+  // reference; the integration and scatter run on the GPU.
   void assemblyRHS(int dim, std::string FEType, MultiVectorPtr_Type a, std::string fieldType, RhsFunc_Type func,
                    std::vector<double>& funcParameter) {
     if (FEType == "P0") throw std::logic_error("Not implemented for P0");
@@ -248,31 +250,23 @@ class FE {
     const Domain& d = *domainVec_[FEloc];
     if (fieldType != "Scalar" && fieldType != "Vector") throw std::logic_error("Invalid field type.");
     const int comps = fieldType == "Scalar" ? 1 : dim;
-    // The source at every element's centroid: the GPU integrates spatially constant sources.
-    std::vector<double> f0(3, 0.0), fv(3, 0.0), x(3);
-    double* paras = funcParameter.empty() ? nullptr : funcParameter.data();
-    const int npe = d.nodesPerElement;
-    for (int64_t e = 0; e < d.getNumElements(); ++e) {
-      for (int k = 0; k < 3; ++k) {
-        x[k] = 0.0;
-        for (int v = 0; v < 4; ++v) x[k] += 0.25 * d.points[3 * (size_t)d.elements[(size_t)e * npe + v] + k];
-      }
-      func(x.data(), e == 0 ? f0.data() : fv.data(), paras);
-      if (e > 0)
-        for (int k = 0; k < comps; ++k)
-          if (fv[k] != f0[k])
-            throw std::logic_error("assemblyRHS: the GPU path integrates sources that are constant in space");
-    }
-    const int64_t n = d.getNumPoints();
+    const int64_t n = d.getNumPoints(), E = d.getNumElements();
     if (a->getLocalLength() != comps * n) throw std::logic_error("assemblyRHS: vector length does not match the map");
     Slot& s = slot(FEloc, -1, FEDD_GRAPH_SCALAR);
-    std::vector<double> b((size_t)n);
-    double* out = a->getDataNonConst();
-    for (int k = 0; k < comps; ++k) {
-      feddCheck(s.ctx, fedd_assembly_rhs(s.ctx, f0[k]));
-      feddCheck(s.ctx, fedd_vector_download(s.ctx, FEDD_VEC_RHS, b.data(), n));
-      for (int64_t i = 0; i < n; ++i) out[comps * i + k] += b[(size_t)i];  // valuesRhs[row] += value
+    int nq = 0;
+    feddCheck(s.ctx, fedd_rhs_quadrature_points(s.ctx, nullptr, 0, &nq));
+    std::vector<double> xq((size_t)(E * nq * 3)), fq((size_t)(E * nq * comps)), val(3, 0.0);
+    feddCheck(s.ctx, fedd_rhs_quadrature_points(s.ctx, xq.data(), (int64_t)xq.size(), &nq));
+    double* paras = funcParameter.empty() ? nullptr : funcParameter.data();
+    for (int64_t k = 0; k < E * nq; ++k) {
+      func(&xq[3 * (size_t)k], val.data(), paras);
+      for (int c = 0; c < comps; ++c) fq[(size_t)(comps * k + c)] = val[(size_t)c];
     }
+    feddCheck(s.ctx, fedd_assembly_rhs_points(s.ctx, fq.data(), (int64_t)fq.size(), comps));
+    std::vector<double> b((size_t)(comps * n));
+    feddCheck(s.ctx, fedd_vector_download(s.ctx, FEDD_VEC_RHS, b.data(), comps * n));
+    double* out = a->getDataNonConst();
+    for (int64_t i = 0; i < comps * n; ++i) out[i] += b[(size_t)i];  // valuesRhs[row] += value
   }
 
   // FE::assemblyLaplaceAssFE (FE_def.hpp:1818-1853) through the "Laplace" element.

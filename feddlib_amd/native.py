@@ -87,6 +87,8 @@ _SIGS = {
     "fedd_vector_device_ptr": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_dp), _lp]),
     "fedd_assembly_laplace": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fedd_assembly_rhs": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "fedd_rhs_quadrature_points": (ctypes.c_int, [_vp, _dp, ctypes.c_int64, _ip]),
+    "fedd_assembly_rhs_points": (ctypes.c_int, [_vp, _dp, ctypes.c_int64, ctypes.c_int]),
     "fedd_assembly_navier_stokes": (ctypes.c_int, [_vp, ctypes.POINTER(NsParams), ctypes.c_int,
                                                    ctypes.c_int, ctypes.c_int]),
     "fedd_newton_step": (ctypes.c_int, [_vp, _ip]),
@@ -298,6 +300,21 @@ class Context:
 
     def assembly_rhs(self, f):
         self._chk(self._L.fedd_assembly_rhs(self.h, f))
+
+    def rhs_quadrature_points(self, nelem):
+        """Physical quadrature points of FE::assemblyRHS's rule: (nelem, nq, 3)."""
+        nq = ctypes.c_int()
+        self._chk(self._L.fedd_rhs_quadrature_points(self.h, None, 0, ctypes.byref(nq)))
+        xq = np.zeros(nelem * nq.value * 3)
+        self._chk(self._L.fedd_rhs_quadrature_points(self.h, _p(xq, _dp), xq.size, ctypes.byref(nq)))
+        return xq.reshape(nelem, nq.value, 3)
+
+    def assembly_rhs_points(self, fq):
+        """FE::assemblyRHS from source values at those points: fq (nelem, nq) ("Scalar")
+        or (nelem, nq, 3) ("Vector"); the result is in VEC_RHS."""
+        fq = np.ascontiguousarray(fq, np.float64)
+        ncomp = 1 if fq.ndim == 2 else fq.shape[2]
+        self._chk(self._L.fedd_assembly_rhs_points(self.h, _p(fq, _dp), fq.size, ncomp))
 
     def assembly_navier_stokes(self, viscosity, density, newton, mode, reassemble, precision=FP64,
                                coeff=(1.0, 1.0, 1.0, 1.0)):

@@ -168,6 +168,16 @@ int fedd_vector_device_ptr(fedd_ctx* ctx, int which, double** d_ptr, int64_t* n)
 int fedd_assembly_laplace(fedd_ctx* ctx, int precision);
 /* FE::assemblyRHS (FE_def.hpp:7303-7395), scalar field, constant f: writes VEC_RHS. */
 int fedd_assembly_rhs(fedd_ctx* ctx, double f);
+/* FE::assemblyRHS with a general source (FE_def.hpp:7352-7391).  The reference calls
+ * func at the physical quadrature points of the rule determineDegree(dim, FEType, Std)
+ * (P1: the centroid, P2: the 5-point rule, Helper.cpp:1310-1350) and adds
+ * |det B| sum_q w_q phi_i(q) f_d(x_q) at row node ("Scalar") or 3 node + d ("Vector").
+ * fedd_rhs_quadrature_points writes those points, xq[(e*nq + q)*3 + j] (cap doubles,
+ * xq may be NULL to query nq); the caller evaluates its source there and passes
+ * fq[(e*nq + q)*ncomp + d], ncomp 1 or 3, to fedd_assembly_rhs_points, which writes
+ * VEC_RHS with ncomp entries per node.  Needs GRAPH_SCALAR. */
+int fedd_rhs_quadrature_points(fedd_ctx* ctx, double* xq, int64_t cap, int* nq);
+int fedd_assembly_rhs_points(fedd_ctx* ctx, const double* fq, int64_t n, int ncomp);
 /* FE::assemblyNavierStokes (FE_def.hpp:1078-1239) with AssembleFENavierStokes
  * (AssembleFENavierStokes_def.hpp:67-224).  mode: FEDD_NS_*.  reassemble = 1 fills
  * only A(0,0) (addFeBlock), 0 fills A, B^T, B (and the empty C) (addFeBlockMatrix).

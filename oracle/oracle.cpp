@@ -899,6 +899,60 @@ void orc_assembly_rhs(int fe, const double* pts, int64_t npts, const int32_t* el
   }
 }
 
+// FE::assemblyRHS, general source (FE_def.hpp:7352-7391): the quadrature points of the
+// rule determineDegree(dim, FEType, Std) mapped to each element, quadPointsTrans[w][j] =
+// sum_k B[j][k] xi_w[k] + x_0[j] (:7357-7366).
+int orc_rhs_quadrature_points(int fe, const double* pts, const int32_t* elems, int64_t E, double* xq) {
+  const int n = nloc(fe);
+  vec2_d qp;
+  vec_d w;
+  const int nq = quadrature(degree_type(fe, 0), qp, w);
+  for (int64_t T = 0; T < E; ++T) {
+    const int32_t* nd = elems + (int64_t)n * T;
+    double xl[30];
+    for (int k = 0; k < n; ++k)
+      for (int d = 0; d < 3; ++d) xl[3 * k + d] = pts[3 * (int64_t)nd[k] + d];
+    M3 B;
+    build_transformation(xl, B);
+    for (int q = 0; q < nq; ++q)
+      for (int j = 0; j < 3; ++j) {
+        double v = 0.;
+        for (int k = 0; k < 3; ++k) v += B.v[j][k] * qp[q][k];
+        xq[(T * nq + q) * 3 + j] = v + xl[j];
+      }
+  }
+  return nq;
+}
+
+// ... and the load vector from the source values at those points, fq[(T nq + w) ncomp + d]:
+// value = |det B| sum_w w phi_i(w) f_d(x_w), added at row node ("Scalar", ncomp = 1) or
+// 3 node + d ("Vector", ncomp = 3) (:7367-7389).
+void orc_assembly_rhs_points(int fe, const double* pts, int64_t npts, const int32_t* elems, int64_t E,
+                             const double* fq, int ncomp, double* rhs) {
+  (void)npts;
+  const int n = nloc(fe);
+  vec2_d phiv;
+  vec_d w;
+  getPhi(phiv, w, fe, degree_type(fe, 0));
+  const int nq = (int)w.size();
+  for (int64_t T = 0; T < E; ++T) {
+    const int32_t* nd = elems + (int64_t)n * T;
+    double xl[30];
+    for (int k = 0; k < n; ++k)
+      for (int d = 0; d < 3; ++d) xl[3 * k + d] = pts[3 * (int64_t)nd[k] + d];
+    M3 B;
+    build_transformation(xl, B);
+    const double absDetB = std::fabs(compute_det(B));
+    for (int i = 0; i < n; ++i)
+      for (int d = 0; d < ncomp; ++d) {
+        double value = 0.;
+        for (int q = 0; q < nq; ++q) value += w[q] * phiv[q][i] * fq[(T * nq + q) * ncomp + d];
+        value *= absDetB;
+        rhs[(int64_t)ncomp * nd[i] + d] += value;
+      }
+  }
+}
+
 void orc_ns_element(const double* xl, const double* sol, double nu, double rho, int newton,
                     const double* coeff4, double* jac, double* fp, double* rhs) {
   static const double one4[4] = {1, 1, 1, 1};

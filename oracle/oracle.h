@@ -70,6 +70,14 @@ void* orc_assembly_laplace_assfe(int fe, int dofs, const double* pts, int64_t np
 void orc_assembly_rhs(int fe, const double* pts, int64_t npts, const int32_t* elems, int64_t E,
                       double f, double* rhs);
 
+/* FE::assemblyRHS with a general source (FE_def.hpp:7352-7391).  xq (E*nq*3) receives the
+ * physical quadrature points of the rule determineDegree(dim, FEType, Std) (P1: 1 point,
+ * P2: 5); returns nq.  Then rhs += |det B| sum_w w phi_i f_d(x_w) from fq[(T*nq+w)*ncomp+d],
+ * at row node (ncomp 1, "Scalar") or 3 node + d (ncomp 3, "Vector"). */
+int orc_rhs_quadrature_points(int fe, const double* pts, const int32_t* elems, int64_t E, double* xq);
+void orc_assembly_rhs_points(int fe, const double* pts, int64_t npts, const int32_t* elems, int64_t E,
+                             const double* fq, int ncomp, double* rhs);
+
 /* ---- Navier-Stokes P2-P1 (AssembleFENavierStokes_def.hpp, FE_def.hpp:1078-1239) ---- */
 /* Element-level: xl 10*3 (node-major), sol 34 (u node-interleaved then 4 p).
  * Fills jac (34*34 row-major) per assembleJacobian(:67-112) when jac != NULL,

@@ -46,6 +46,9 @@ def lib():
                                                  ctypes.c_int64]),
             "orc_assembly_rhs": (None, [ctypes.c_int, _dp, ctypes.c_int64, _ip, ctypes.c_int64,
                                         ctypes.c_double, _dp]),
+            "orc_rhs_quadrature_points": (ctypes.c_int, [ctypes.c_int, _dp, _ip, ctypes.c_int64, _dp]),
+            "orc_assembly_rhs_points": (None, [ctypes.c_int, _dp, ctypes.c_int64, _ip, ctypes.c_int64, _dp,
+                                               ctypes.c_int, _dp]),
             "orc_ns_element": (None, [_dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp,
                                       _dp, _dp, _dp]),
             "orc_assembly_ns": (None, [_dp, ctypes.c_int64, _ip, _ip, ctypes.c_int64, ctypes.c_int64, _dp,
@@ -126,6 +129,23 @@ def laplace_assfe(fe, dofs, pts, el):
 def rhs(fe, pts, el, f):
     b = np.zeros(pts.shape[0])
     lib().orc_assembly_rhs(fe, P(pts), pts.shape[0], P(el, _ip), el.shape[0], f, P(b))
+    return b
+
+
+def rhs_quadrature_points(fe, pts, el):
+    """Physical quadrature points of FE::assemblyRHS's rule per element: (E, nq, 3)."""
+    E = el.shape[0]
+    xq = np.zeros(E * 5 * 3)
+    nq = lib().orc_rhs_quadrature_points(fe, P(pts), P(el, _ip), E, P(xq))
+    return xq[:E * nq * 3].reshape(E, nq, 3)
+
+
+def rhs_points(fe, pts, el, fq):
+    """FE::assemblyRHS from source values fq (E, nq) ("Scalar") or (E, nq, 3) ("Vector")."""
+    fq = np.ascontiguousarray(fq, np.float64)
+    ncomp = 1 if fq.ndim == 2 else fq.shape[2]
+    b = np.zeros(ncomp * pts.shape[0])
+    lib().orc_assembly_rhs_points(fe, P(pts), pts.shape[0], P(el, _ip), el.shape[0], P(fq), ncomp, P(b))
     return b
 
 

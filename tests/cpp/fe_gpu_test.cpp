@@ -98,7 +98,7 @@ static void rhsX3D(double* x, double* res, double* parameters) {
 }
 static void rampFunc(double* x, double* res, double* parameters) {
   (void)parameters;
-  res[0] = x[0];
+  res[0] = 1.0 + x[0] * x[0] - 0.5 * x[1] * x[2];
 }
 
 // ------------------------------------------------------------ CPU cases
@@ -172,8 +172,22 @@ static void laplaceCases() {
   for (int64_t i = 0; i < n; ++i) want[3 * i] = 0.5 * rb[i];
   const double d = maxRelDiff(b->vector(), want);
   report("assemblyRHS Vector rhsX3D", d <= 1e-13, std::to_string(d));
-  report("assemblyRHS varying source raises logic_error",
-         throws<std::logic_error>([&] { fe.assemblyRHS(3, "P2", b, "Vector", rampFunc, prm); }, "constant in space"));
+  // a source varying in space, against the oracle's restated loop at its own points
+  for (int fe_ = 1; fe_ <= 2; ++fe_) {
+    const std::string T = fe_ == 1 ? "P1" : "P2";
+    auto dv = Domain::buildStructuredCube(3, T, false);
+    FE fv;
+    fv.addFE(dv);
+    const int64_t nv = dv->getNumPoints(), Ev = dv->getNumElements();
+    auto bs = std::make_shared<MultiVector>(nv);
+    fv.assemblyRHS(3, T, bs, "Scalar", rampFunc, prm);
+    std::vector<double> xq((size_t)(Ev * 5 * 3)), fq((size_t)(Ev * 5)), rr((size_t)nv, 0.0);
+    const int nq = orc_rhs_quadrature_points(fe_, dv->points.data(), dv->elements.data(), Ev, xq.data());
+    for (int64_t k = 0; k < Ev * nq; ++k) rampFunc(&xq[3 * k], &fq[k], nullptr);
+    orc_assembly_rhs_points(fe_, dv->points.data(), nv, dv->elements.data(), Ev, fq.data(), 1, rr.data());
+    const double dd = maxRelDiff(bs->vector(), rr);
+    report(("assemblyRHS varying source " + T).c_str(), dd <= 1e-13, std::to_string(dd));
+  }
   report("assemblyRHS invalid field type raises logic_error",
          throws<std::logic_error>([&] { fe.assemblyRHS(3, "P2", b, "Tensor", rhsX3D, prm); }, "Invalid field type."));
 }

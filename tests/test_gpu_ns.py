@@ -116,6 +116,38 @@ def test_laplace_and_rhs(gpu_ctx, fe, M):
     assert abs(b.sum() - 1.0) < 1e-12  # int_Omega f = |Omega| = 1
 
 
+@pytest.mark.parametrize("fe,M", [(1, 4), (2, 3)])
+def test_rhs_general_source(gpu_ctx, fe, M):
+    """assemblyRHS with a source varying in space (FE_def.hpp:7352-7391): the GPU's
+    quadrature points are the oracle's, a constant source reproduces fedd_assembly_rhs,
+    and scalar and vector sources match the oracle's restated loop to 1e-12."""
+    pts, _, el = N.mesh_structured(M, fe)
+    gpu_ctx.mesh_upload(N.MESH_PRIMARY, pts, el)
+    gpu_ctx.graph_build(N.GRAPH_SCALAR)
+    xq = gpu_ctx.rhs_quadrature_points(el.shape[0])
+    xo = O.rhs_quadrature_points(fe, pts, el)
+    assert xq.shape == xo.shape and np.max(np.abs(xq - xo)) <= 1e-15
+    n = pts.shape[0]
+    gpu_ctx.assembly_rhs_points(np.full(xq.shape[:2], 2.5))
+    b = gpu_ctx.vector_download(N.VEC_RHS, n)
+    gpu_ctx.assembly_rhs(2.5)
+    assert vec_rel_err(b, gpu_ctx.vector_download(N.VEC_RHS, n)) <= 1e-14
+
+    def f(x):
+        return 1.0 + x[..., 0] ** 2 - np.sin(3 * x[..., 1]) * x[..., 2]
+
+    gpu_ctx.assembly_rhs_points(f(xq))
+    b = gpu_ctx.vector_download(N.VEC_RHS, n)
+    assert vec_rel_err(b, O.rhs_points(fe, pts, el, f(xo))) <= 1e-12
+    fv = np.stack([f(xq), xq[..., 0] * xq[..., 1], -np.ones(xq.shape[:2])], axis=-1)
+    gpu_ctx.assembly_rhs_points(fv)
+    bv = gpu_ctx.vector_download(N.VEC_RHS, 3 * n)
+    fvo = np.stack([f(xo), xo[..., 0] * xo[..., 1], -np.ones(xo.shape[:2])], axis=-1)
+    assert vec_rel_err(bv, O.rhs_points(fe, pts, el, fvo)) <= 1e-12
+    with pytest.raises(N.FeddError, match="nelem \\* nq \\* ncomp"):
+        gpu_ctx.assembly_rhs_points(f(xq)[:-1])
+
+
 def _greedy_colours(el):
     """First-fit colouring in ascending element order (restated for the check)."""
     inc = {}

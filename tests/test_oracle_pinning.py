@@ -208,6 +208,48 @@ def test_rhs_closed_form(fe):
     assert np.max(np.abs(b - ref)) < 1e-14
 
 
+@pytest.mark.parametrize("fe", [1, 2])
+def test_rhs_points_pinned(fe):
+    """General-source assemblyRHS: the mapped points are the rule's points (P1 centroid,
+    P2 the 5-point orbits), a constant source gives the constant-f vector, the sum is the
+    rule applied to f over the cube (partition of unity; exact to degree 1 / 3), and the
+    first moment sum_i b_i x_i equals the rule applied to f x (P1 reproduces x, P2 too)."""
+    M = 3
+    pts, _, el = O.mesh(M, fe)
+    xq = O.rhs_quadrature_points(fe, pts, el)
+    X = pts[el[:, :4]]
+    if fe == 1:
+        assert xq.shape[1] == 1 and np.allclose(xq[:, 0], X.mean(axis=1), atol=1e-15)
+    else:
+        assert xq.shape[1] == 5 and np.allclose(xq[:, 0], X.mean(axis=1), atol=1e-15)
+        # the other four points: barycentric (1/2, 1/6, 1/6, 1/6) around each vertex
+        orbit = np.sort(np.einsum("ek,ekd->ed", np.array([0.5, 1 / 6, 1 / 6, 1 / 6])[None].repeat(len(el), 0), X))
+        cand = [np.sort(xq[:, k]) for k in range(1, 5)]
+        assert any(np.allclose(c, orbit, atol=1e-14) for c in cand)
+    assert np.max(np.abs(O.rhs_points(fe, pts, el, np.full(xq.shape[:2], 2.5)) - O.rhs(fe, pts, el, 2.5))) < 1e-15
+    x, y, z = xq[..., 0], xq[..., 1], xq[..., 2]
+    if fe == 1:
+        f = 1 + 2 * x + 3 * y - z
+        b = O.rhs_points(fe, pts, el, f)
+        assert abs(b.sum() - 3.0) < 1e-13  # midpoint rule is exact for linear f
+        vol = abs(np.linalg.det(np.stack([X[:, k] - X[:, 0] for k in (1, 2, 3)], 2))) / 6
+        mid = X.mean(axis=1)
+        ref = np.sum(vol * (1 + 2 * mid[:, 0] + 3 * mid[:, 1] - mid[:, 2]) * mid[:, 0])
+        assert abs(b @ pts[:, 0] - ref) < 1e-13
+    else:
+        f = x ** 2 * y + z ** 3
+        b = O.rhs_points(fe, pts, el, f)
+        assert abs(b.sum() - 5 / 12) < 1e-13  # int x^2 y + z^3 over the cube, degree 3
+        g = x + y
+        b = O.rhs_points(fe, pts, el, g)
+        assert abs(b @ pts[:, 0] - 7 / 12) < 1e-13  # int (x + y) x, degree 2
+    # "Vector": component d of node i at row 3 i + d
+    fv = np.stack([x, 2 * y, -z], axis=-1)
+    bv = O.rhs_points(fe, pts, el, fv)
+    for d, s in enumerate((1.0, 2.0, -1.0)):
+        assert np.max(np.abs(bv[d::3] - s * O.rhs_points(fe, pts, el, xq[..., d]))) < 1e-15
+
+
 # ------------------------------------------------------------- Navier-Stokes
 def _p2_poly_basis():
     """P2 basis (FEDDLib order) as callables of barycentrics, for exact integration."""
