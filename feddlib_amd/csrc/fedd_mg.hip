@@ -203,9 +203,13 @@ __global__ __launch_bounds__(256) void k_mg_to_f32(int64_t n, const double* __re
 // (weight 1), lanes 1..14 the midpoints of the 14 Kuhn edges +-e_S (weight 1/2), then a
 // shuffle reduction.  (One thread per coarse node leaves too few waves in flight on the
 // 8x smaller coarse level.)
+// With xc set, lane 0 also performs the coarse level's first smoothing sweep from zero,
+// xc = omega D^-1 bc (the arithmetic of k_mg_sweep MODE 1), saving that launch.
 __global__ __launch_bounds__(256) void k_mg_restrict(int64_t Pf, int64_t Pc, const double* __restrict__ rf,
                                                      const uint8_t* __restrict__ freef,
-                                                     const uint8_t* __restrict__ freec, double* __restrict__ bc) {
+                                                     const uint8_t* __restrict__ freec, double* __restrict__ bc,
+                                                     const double* __restrict__ binvc, double* __restrict__ xc,
+                                                     double omega) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t I = t >> 4;
   const int lane = (int)(t & 15);
@@ -230,8 +234,20 @@ __global__ __launch_bounds__(256) void k_mg_restrict(int64_t Pf, int64_t Pc, con
 #pragma unroll
     for (int d = 0; d < 3; ++d) acc[d] += __shfl_xor(acc[d], o, 16);
   if (I >= nc || lane != 0) return;
+  double rv[3];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) bc[3 * I + d] = freec[3 * I + d] ? acc[d] : 0.0;
+  for (int d = 0; d < 3; ++d) {
+    rv[d] = freec[3 * I + d] ? acc[d] : 0.0;
+    bc[3 * I + d] = rv[d];
+  }
+  if (xc) {
+    const double* B = binvc + 9 * I;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double z = B[3 * d] * rv[0] + B[3 * d + 1] * rv[1] + B[3 * d + 2] * rv[2];
+      xc[3 * I + d] = 0.0 + omega * z;
+    }
+  }
 }
 
 __global__ __launch_bounds__(256) void k_mg_prolong(int64_t Pf, int64_t Pc, const double* __restrict__ xc,
@@ -1170,7 +1186,9 @@ static void launch_sweep(hipStream_t st, unsigned gs, int mode, const MgLevel& L
     k_mg_sweep<G, 2, VT><<<gs, 256, 0, st>>>(L.nn, L.d_nptr, L.d_nidx, vals, bi, b, xin, out, w);
 }
 
-static void vcycle(Ctx* c, size_t l, const double* b, double* x) {
+// first_done: the restriction into this level already performed its first sweep
+// (result in the level's spare buffer).
+static void vcycle(Ctx* c, size_t l, const double* b, double* x, bool first_done = false) {
   MgHost* g = c->mg;
   MgLevel& L = g->lv[l];
   if (l + 1 == g->lv.size()) {
@@ -1191,7 +1209,7 @@ static void vcycle(Ctx* c, size_t l, const double* b, double* x) {
   // pre-smoothing in the spare buffer makes the nu post-sweeps end in x for any nu.
   double* xa = x;
   double* xb = L.x2.as<double>();
-  sweep(1, nullptr, xb);  // from x = 0
+  if (!first_done) sweep(1, nullptr, xb);  // from x = 0
   double* cur = xb;
   const int nu = l == 0 ? FEDD_MG_NU : FEDD_MG_NU_COARSE;
   for (int s = 1; s < nu; ++s) {
@@ -1207,10 +1225,12 @@ static void vcycle(Ctx* c, size_t l, const double* b, double* x) {
                                                               C.tpos.as<int64_t>(), C.pval.as<double>(),
                                                               L.r.as<double>(), C.free.as<uint8_t>(),
                                                               C.b.as<double>());
-  else
-    k_mg_restrict<<<nbk(C.nn * 16, 256), 256, 0, c->stream>>>(L.P, C.P, L.r.as<double>(), L.free.as<uint8_t>(),
-                                                              C.free.as<uint8_t>(), C.b.as<double>());
-  vcycle(c, l + 1, C.b.as<double>(), C.x.as<double>());
+  const bool fuse = g->mode != 1 && l + 2 < g->lv.size();  // the coarsest level is a dense solve
+  if (g->mode != 1)
+    k_mg_restrict<<<nbk(C.nn * 16, 256), 256, 0, c->stream>>>(
+        L.P, C.P, L.r.as<double>(), L.free.as<uint8_t>(), C.free.as<uint8_t>(), C.b.as<double>(),
+        fuse ? C.binv.as<double>() : nullptr, fuse ? C.x2.as<double>() : nullptr, w);
+  vcycle(c, l + 1, C.b.as<double>(), C.x.as<double>(), fuse);
   if (g->mode == 1)
     k_sa_prolong<<<nbk(L.nn, 256), 256, 0, c->stream>>>(L.nn, C.pptr.as<int64_t>(), C.pidx.as<int32_t>(),
                                                         C.pval.as<double>(), C.x.as<double>(), cur);
