@@ -53,7 +53,7 @@ __global__ __launch_bounds__(256) void k_newton_residual(int64_t n, const double
   red[threadIdx.x] = v * v;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
@@ -70,7 +70,7 @@ __global__ __launch_bounds__(1024) void k_reduce(const double* __restrict__ part
   red[threadIdx.x] = v;
   __syncthreads();
   for (int s = 512; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
   if (threadIdx.x == 0) {
