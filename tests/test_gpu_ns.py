@@ -207,3 +207,75 @@ def test_merged_matrix_and_matrix_market(gpu_ctx, tmp_path):
     bt = scipy.io.mmread(str(tmp_path / "bt.mtx")).tocsr()
     brp, bci = gpu_ctx.block_graph(N.BLOCK_BT)
     assert (bt != sp.csr_matrix((gpu_ctx.block_values(N.BLOCK_BT), bci, brp), shape=(nv, npr))).nnz == 0
+
+
+def _unstructured_ns(M, seed=5):
+    """The lattice with jittered interior vertices (edge nodes kept at the midpoints, so
+    the elements stay affine), permuted P2 and P1 node ids, shuffled elements and a random
+    vertex order inside each element, applied alike to the P2 and the P1 element: half the
+    elements change orientation and no node id follows the lattice."""
+    p2, _, e2 = W.jittered_p2_mesh(M, seed, orient=False)
+    _, _, e1 = N.mesh_structured(M, 1)
+    n1 = int(e1.max()) + 1
+    p1 = np.zeros((n1, 3))
+    p1[e1] = p2[e2[:, :4]]  # P1 vertex k of element e is P2 vertex k of element e
+    assert np.array_equal(p1[e1], p2[e2[:, :4]])
+    rng = np.random.default_rng(seed)
+    edges = [(0, 1), (1, 2), (0, 2), (0, 3), (1, 3), (2, 3)]
+    slot = {}
+    for m, (a, b) in enumerate(edges):
+        slot[(a, b)] = slot[(b, a)] = 4 + m
+    ne = e2.shape[0]
+    v2 = np.empty_like(e2)
+    v1 = np.empty_like(e1)
+    for e in range(ne):
+        pi = rng.permutation(4)
+        v2[e, :4] = e2[e, pi]
+        v1[e] = e1[e, pi]
+        for m, (a, b) in enumerate(edges):
+            v2[e, 4 + m] = e2[e, slot[(pi[a], pi[b])]]
+    order = rng.permutation(ne)
+    v2, v1 = v2[order], v1[order]
+    q2 = rng.permutation(p2.shape[0])  # new id of old node i is inv2[i]
+    q1 = rng.permutation(n1)
+    inv2, inv1 = np.argsort(q2), np.argsort(q1)
+    p2n, p1n = p2[q2], p1[q1]
+    e2n = np.ascontiguousarray(inv2[v2].astype(np.int32))
+    e1n = np.ascontiguousarray(inv1[v1].astype(np.int32))
+    for m, (a, b) in enumerate(edges):
+        assert np.allclose(p2n[e2n[:, 4 + m]], 0.5 * (p2n[e2n[:, a]] + p2n[e2n[:, b]]), atol=1e-15)
+    assert np.array_equal(p1n[e1n], p2n[e2n[:, :4]])
+    J = np.stack([p2n[e2n[:, k]] - p2n[e2n[:, 0]] for k in (1, 2, 3)], axis=1)
+    det = np.linalg.det(J)
+    assert (det > 0).any() and (det < 0).any()
+    u = W.ns_velocity(p2n, seed)
+    p = rng.standard_normal(n1)
+    return p2n, e2n, p1n, e1n, u, p
+
+
+def test_ns_unstructured_mesh(gpu_ctx):
+    """Every mode on a mesh that is not the lattice: the row gather's relabelling and
+    exact tensors hold for any affine P2 element in any vertex order and numbering."""
+    p2, e2, p1, e1, u, p = _unstructured_ns(3)
+    gpu_ctx.mesh_upload(N.MESH_PRIMARY, p2, e2)
+    gpu_ctx.mesh_upload(N.MESH_PRESSURE, p1, e1)
+    gpu_ctx.graph_build(N.GRAPH_NS)
+    gpu_ctx.vector_upload(N.VEC_U, u)
+    gpu_ctx.vector_upload(N.VEC_P, p)
+    ref = O.ns_assembly(p2, e2, e1, p1.shape[0], u, p, NU, RHO, True, 0, False)
+    for blk, name in [(N.BLOCK_A, "A"), (N.BLOCK_BT, "BT"), (N.BLOCK_B, "B")]:
+        assert_graph_equal(gpu_ctx.block_graph(blk), ref[name][:2])
+    for prec, rtol, atol_row in ((N.FP64, 1e-12, 1e-14), (N.FP32_LOCAL, 1e-5, 1e-6)):
+        gpu_ctx.assembly_navier_stokes(NU, RHO, True, N.NS_JACOBIAN, reassemble=False, precision=prec)
+        for blk, name in [(N.BLOCK_A, "A"), (N.BLOCK_BT, "BT"), (N.BLOCK_B, "B")]:
+            v = gpu_ctx.block_values(blk)
+            ratio = parity_ratio(v, ref[name][2], ref[name][0], rtol=rtol, atol_row=atol_row)
+            assert ratio <= 1.0, (prec, name, ratio)
+    fp = O.ns_assembly(p2, e2, e1, p1.shape[0], u, p, NU, RHO, True, 1, True)
+    gpu_ctx.assembly_navier_stokes(NU, RHO, True, N.NS_FIXEDPOINT, reassemble=True, precision=N.FP64)
+    assert parity_ratio(gpu_ctx.block_values(N.BLOCK_A), fp["A"][2], fp["A"][0]) <= 1.0
+    coeff = (0.5, -2.0, 3.0, 1.0)
+    rr = O.ns_assembly(p2, e2, e1, p1.shape[0], u, p, NU, RHO, True, 2, True, coeff)
+    gpu_ctx.assembly_navier_stokes(NU, RHO, True, N.NS_RHS, reassemble=True, coeff=coeff)
+    assert vec_rel_err(gpu_ctx.vector_download(N.VEC_RES_U, 3 * p2.shape[0]), rr["res_u"]) <= 1e-12
+    assert vec_rel_err(gpu_ctx.vector_download(N.VEC_RES_P, p1.shape[0]), rr["res_p"]) <= 1e-12
