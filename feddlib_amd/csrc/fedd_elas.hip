@@ -248,21 +248,19 @@ __device__ __forceinline__ void elas_point(const ElasArgs& a, int32_t e, const i
 #pragma unroll
     for (int A = 0; A < 3; ++A) Jd[ax][A] = jr[A];
   }
-  T Ji[3][3];
-#pragma unroll
-  for (int ax = 0; ax < 3; ++ax)
-#pragma unroll
-    for (int A = 0; A < 3; ++A) Ji[ax][A] = (T)Jd[ax][A];
+  // Physical basis gradients from the FP64 inverse Jacobian, rounded once to T: on
+  // distorted (curved) elements an FP32 inverse Jacobian costs the FP32-local tangent
+  // its 1e-5 bound.
   T G[10][3];
 #pragma unroll
   for (int c = 0; c < 10; ++c)
 #pragma unroll
     for (int A = 0; A < 3; ++A) {
-      T v = 0;
+      double v = 0.0;
 #pragma unroll
       for (int ax = 0; ax < 3; ++ax)
-        if (dphi[c][ax] != 0.0) v += (T)dphi[c][ax] * Ji[ax][A];
-      G[c][A] = v;
+        if (dphi[c][ax] != 0.0) v += dphi[c][ax] * Jd[ax][A];
+      G[c][A] = (T)v;
     }
   T g[3];
 #pragma unroll

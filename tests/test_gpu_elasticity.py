@@ -134,3 +134,80 @@ def test_efield_rejects_bad_field(gpu_ctx):
     bad[3] = -1.0
     with pytest.raises(N.FeddError):
         gpu_ctx.assembly_nonlinear_elasticity_efield(1, bad, NU)
+
+
+_EDGES = [(0, 1), (1, 2), (0, 2), (0, 3), (1, 3), (2, 3)]
+
+
+def _iso_jacobian_dets(pts, el, xi):
+    """det of the isoparametric P2 map x(xi) = sum_k x_k phi_k(xi) at reference point xi."""
+    lam = np.array([1.0 - xi.sum(), xi[0], xi[1], xi[2]])
+    dlam = np.vstack([-np.ones(3), np.eye(3)])  # d lambda_m / d xi
+    grads = [(4 * lam[i] - 1) * dlam[i] for i in range(4)]
+    grads += [4 * (lam[a] * dlam[b] + lam[b] * dlam[a]) for a, b in _EDGES]
+    G = np.array(grads)  # (10, 3)
+    J = np.einsum("ekd,kj->edj", pts[el], G)
+    return np.linalg.det(J)
+
+
+def _curved_shuffled(M, seed=11, amp=0.08):
+    """Isoparametric (curved) P2 elements: the jittered cube with every edge node moved off
+    its midpoint by U(+-amp*h), then node ids permuted, elements shuffled and each
+    element's vertex order permuted (edge slots follow, orientation flips for odd
+    permutations)."""
+    pts, _, el = W.jittered_p2_mesh(M, seed, orient=True)
+    rng = np.random.default_rng(seed)
+    h = 1.0 / M
+    edge_nodes = np.unique(el[:, 4:])
+    pts = pts.copy()
+    pts[edge_nodes] += rng.uniform(-amp * h, amp * h, size=(len(edge_nodes), 3))
+    slot = {}
+    for m, (a, b) in enumerate(_EDGES):
+        slot[(a, b)] = slot[(b, a)] = 4 + m
+    v = np.empty_like(el)
+    for e in range(el.shape[0]):
+        pi = rng.permutation(4)
+        v[e, :4] = el[e, pi]
+        for m, (a, b) in enumerate(_EDGES):
+            v[e, 4 + m] = el[e, slot[(pi[a], pi[b])]]
+    v = v[rng.permutation(el.shape[0])]
+    q = rng.permutation(pts.shape[0])
+    inv = np.argsort(q)
+    pts = pts[q]
+    el = np.ascontiguousarray(inv[v].astype(np.int32))
+    mid = np.stack([0.5 * (pts[el[:, a]] + pts[el[:, b]]) for a, b in _EDGES], axis=1)
+    assert np.max(np.abs(pts[el[:, 4:]] - mid)) > 1e-2 * h  # not affine
+    # the map stays regular with one sign per element at every point of both rules
+    for xi in ([0.25, 0.25, 0.25], [1 / 6, 1 / 6, 1 / 6], [0.5, 1 / 6, 1 / 6], [0.1, 0.2, 0.6], [0, 0, 0]):
+        d = _iso_jacobian_dets(pts, el, np.array(xi))
+        d0 = _iso_jacobian_dets(pts, el, np.array([0.25, 0.25, 0.25]))
+        assert np.all(d * d0 > 0)
+    d0 = _iso_jacobian_dets(pts, el, np.array([0.25, 0.25, 0.25]))
+    assert (d0 > 0).any() and (d0 < 0).any()
+    return pts, el
+
+
+@pytest.mark.parametrize("model", [0, 1, 2])
+def test_elasticity_curved_shuffled_mesh(gpu_ctx, model):
+    """Curved (isoparametric) P2 elements in any numbering, element order and vertex order:
+    the laws integrate with the geometry Jacobian of all 10 nodes at every point
+    (NeoHookQuadraticTets3.f:80-175), so the row gather's relabelling must carry each
+    point's own Jacobian."""
+    pts, el = _curved_shuffled(2)
+    u = W.smoothed_displacement(pts, el, 3, 0.05)
+    gpu_ctx.mesh_upload(N.MESH_PRIMARY, pts, el)
+    gpu_ctx.graph_build(N.GRAPH_VECTOR)
+    gpu_ctx.vector_upload(N.VEC_U, u)
+    ref, rref = O.elasticity(model, EMOD, NU, pts, el, u)
+    assert_graph_equal(gpu_ctx.block_graph(N.BLOCK_A), ref[:2])
+    for prec, rtol, atol_row in ((N.FP64, 1e-12, 1e-14), (N.FP32_LOCAL, 1e-5, 1e-6)):
+        if model == 0:
+            gpu_ctx.assembly_linear_elasticity(EMOD, NU, prec)
+        else:
+            gpu_ctx.assembly_nonlinear_elasticity(model, EMOD, NU, prec)
+        v = gpu_ctx.block_values(N.BLOCK_A)
+        ratio = parity_ratio(v, ref[2], ref[0], rtol=rtol, atol_row=atol_row)
+        assert ratio <= 1.0, (prec, ratio, rel_err_rowfloor(v, ref[2], ref[0]))
+        if model != 0:
+            r = gpu_ctx.vector_download(N.VEC_RES_U, 3 * pts.shape[0])
+            assert vec_rel_err(r, rref) <= 1e-12
