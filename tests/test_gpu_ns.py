@@ -209,12 +209,17 @@ def test_merged_matrix_and_matrix_market(gpu_ctx, tmp_path):
     assert (bt != sp.csr_matrix((gpu_ctx.block_values(N.BLOCK_BT), bci, brp), shape=(nv, npr))).nnz == 0
 
 
-def _unstructured_ns(M, seed=5):
-    """The lattice with jittered interior vertices (edge nodes kept at the midpoints, so
-    the elements stay affine), permuted P2 and P1 node ids, shuffled elements and a random
-    vertex order inside each element, applied alike to the P2 and the P1 element: half the
-    elements change orientation and no node id follows the lattice."""
+def _unstructured_ns(M, seed=5, curved=False):
+    """The lattice with jittered interior vertices, permuted P2 and P1 node ids, shuffled
+    elements and a random vertex order inside each element, applied alike to the P2 and
+    the P1 element: half the elements change orientation and no node id follows the
+    lattice.  Edge nodes stay at the midpoints, or with `curved` move off them by up to
+    0.05 h."""
     p2, _, e2 = W.jittered_p2_mesh(M, seed, orient=False)
+    if curved:
+        p2 = p2.copy()
+        en = np.unique(e2[:, 4:])
+        p2[en] += np.random.default_rng(seed + 1).uniform(-0.05 / M, 0.05 / M, size=(len(en), 3))
     _, _, e1 = N.mesh_structured(M, 1)
     n1 = int(e1.max()) + 1
     p1 = np.zeros((n1, 3))
@@ -242,8 +247,9 @@ def _unstructured_ns(M, seed=5):
     p2n, p1n = p2[q2], p1[q1]
     e2n = np.ascontiguousarray(inv2[v2].astype(np.int32))
     e1n = np.ascontiguousarray(inv1[v1].astype(np.int32))
-    for m, (a, b) in enumerate(edges):
-        assert np.allclose(p2n[e2n[:, 4 + m]], 0.5 * (p2n[e2n[:, a]] + p2n[e2n[:, b]]), atol=1e-15)
+    mid = np.stack([0.5 * (p2n[e2n[:, a]] + p2n[e2n[:, b]]) for a, b in edges], axis=1)
+    off = np.max(np.abs(p2n[e2n[:, 4:]] - mid))
+    assert (off > 1e-3 / M) if curved else (off <= 1e-15)
     assert np.array_equal(p1n[e1n], p2n[e2n[:, :4]])
     J = np.stack([p2n[e2n[:, k]] - p2n[e2n[:, 0]] for k in (1, 2, 3)], axis=1)
     det = np.linalg.det(J)
@@ -253,10 +259,13 @@ def _unstructured_ns(M, seed=5):
     return p2n, e2n, p1n, e1n, u, p
 
 
-def test_ns_unstructured_mesh(gpu_ctx):
+@pytest.mark.parametrize("curved", [False, True])
+def test_ns_unstructured_mesh(gpu_ctx, curved):
     """Every mode on a mesh that is not the lattice: the row gather's relabelling and
-    exact tensors hold for any affine P2 element in any vertex order and numbering."""
-    p2, e2, p1, e1, u, p = _unstructured_ns(3)
+    exact tensors hold in any vertex order and numbering.  FEDDLib maps every element
+    affinely from its four vertices (Helper::buildTransformation, Helper.cpp:196-222), also
+    when the edge nodes are off the midpoints, and so do the row kernels."""
+    p2, e2, p1, e1, u, p = _unstructured_ns(3, curved=curved)
     gpu_ctx.mesh_upload(N.MESH_PRIMARY, p2, e2)
     gpu_ctx.mesh_upload(N.MESH_PRESSURE, p1, e1)
     gpu_ctx.graph_build(N.GRAPH_NS)
@@ -279,3 +288,11 @@ def test_ns_unstructured_mesh(gpu_ctx):
     gpu_ctx.assembly_navier_stokes(NU, RHO, True, N.NS_RHS, reassemble=True, coeff=coeff)
     assert vec_rel_err(gpu_ctx.vector_download(N.VEC_RES_U, 3 * p2.shape[0]), rr["res_u"]) <= 1e-12
     assert vec_rel_err(gpu_ctx.vector_download(N.VEC_RES_P, p1.shape[0]), rr["res_p"]) <= 1e-12
+    # P2 Laplace and load vector on the same mesh
+    gpu_ctx.graph_build(N.GRAPH_SCALAR)
+    lref = O.laplace(2, p2, e2)
+    assert_graph_equal(gpu_ctx.block_graph(N.BLOCK_A), lref[:2])
+    gpu_ctx.assembly_laplace(N.FP64)
+    assert parity_ratio(gpu_ctx.block_values(N.BLOCK_A), lref[2], lref[0]) <= 1.0
+    gpu_ctx.assembly_rhs(1.0)
+    assert vec_rel_err(gpu_ctx.vector_download(N.VEC_RHS, p2.shape[0]), O.rhs(2, p2, e2, 1.0)) <= 1e-12
