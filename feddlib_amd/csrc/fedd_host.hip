@@ -496,6 +496,7 @@ int fedd_graph_build(fedd_ctx* c, int kind) {
   build_adjacency(inc, m.npts, m, g.a);
   int rc;
   const int64_t N = m.npts;
+  c->nbc_rows = -1;  // a rebuilt graph needs a new Dirichlet row list
   if (kind == FEDD_GRAPH_SCALAR) {
     c->blk[0].nrows = N;
     c->blk[0].ncols = N;

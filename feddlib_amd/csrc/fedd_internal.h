@@ -121,6 +121,11 @@ struct Ctx {
   // Dirichlet constraints and Krylov work space (elasticity Newton solve)
   DevBuf d_bcflag;      // uint8 per dof of the primary field
   int64_t nbc = 0;
+  // rows of block A that hold a constrained dof in their row or columns (int32), built
+  // on the device for the current Dirichlet set and graph; nbc_rows < 0: not built
+  DevBuf d_bcrows;
+  int64_t nbc_rows = -1;
+  const void* bcrows_graph = nullptr;  // colind buffer the list was built for
   DevBuf d_kw[5];       // r, z, p, Ap and the block-Jacobi inverse (9 per node)
   DevBuf d_part;        // per-block partial sums
   DevBuf d_scal;        // device scalars of the Krylov loop

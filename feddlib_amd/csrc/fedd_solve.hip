@@ -38,6 +38,46 @@ __global__ __launch_bounds__(256) void k_bc_matrix(int64_t nrows, const int64_t*
   }
 }
 
+// Rows whose own dof or any column dof is constrained, appended in any order (16 lanes
+// per row; the list's order does not change what k_bc_matrix_rows writes).
+__global__ __launch_bounds__(256) void k_bc_rows_list(int64_t nrows, const int64_t* __restrict__ rowptr,
+                                                      const int32_t* __restrict__ colind,
+                                                      const uint8_t* __restrict__ flag, int32_t* __restrict__ rows,
+                                                      int32_t* __restrict__ count) {
+  const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t row = gid >> 4;
+  const int lane = (int)(gid & 15);
+  int hit = 0;
+  if (row < nrows) {
+    hit = flag[row] != 0;
+    for (int64_t k = rowptr[row] + lane; k < rowptr[row + 1] && !hit; k += 16) hit = flag[colind[k]] != 0;
+  }
+#pragma unroll
+  for (int o = 8; o > 0; o >>= 1) hit |= __shfl_xor(hit, o, 16);
+  if (row < nrows && lane == 0 && hit) rows[atomicAdd(count, 1)] = (int32_t)row;
+}
+
+// k_bc_matrix on the listed rows only.
+__global__ __launch_bounds__(256) void k_bc_matrix_rows(int64_t n, const int32_t* __restrict__ rows,
+                                                        const int64_t* __restrict__ rowptr,
+                                                        const int32_t* __restrict__ colind, double* __restrict__ vals,
+                                                        const uint8_t* __restrict__ flag) {
+  const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if ((gid >> 4) >= n) return;
+  const int64_t row = rows[gid >> 4];
+  const int lane = (int)(gid & 15);
+  const bool fr = flag[row] != 0;
+  for (int64_t k = rowptr[row] + lane; k < rowptr[row + 1]; k += 16) {
+    const int32_t col = colind[k];
+    if (fr || flag[col]) vals[k] = (col == row) ? 1.0 : 0.0;
+  }
+}
+
+__global__ void k_fill_i32(int64_t n, int32_t* __restrict__ a, int32_t v) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
 // R = -res - lam fext, zero on constrained dofs (res = assembled element rhs p).
 __global__ __launch_bounds__(256) void k_newton_residual(int64_t n, const double* __restrict__ res,
                                                          const double* __restrict__ fext, double lam,
@@ -305,9 +345,31 @@ static int ensure_vec(Ctx* c, int w, int64_t n) {
 int apply_dirichlet_matrix(Ctx* c) {
   const Block& A = c->blk[0];
   if (c->nbc == 0) return FEDD_OK;
-  k_bc_matrix<<<nb(A.nrows * 16, 256), 256, 0, c->stream>>>(A.nrows, A.d_rowptr.as<int64_t>(),
-                                                             A.d_colind.as<int32_t>(), A.d_vals.as<double>(),
-                                                             c->d_bcflag.as<uint8_t>());
+  const int64_t* rp = A.d_rowptr.as<int64_t>();
+  const int32_t* ci = A.d_colind.as<int32_t>();
+  const uint8_t* fl = c->d_bcflag.as<uint8_t>();
+  if ((c->nbc_rows < 0 || c->bcrows_graph != A.d_colind.p) && !c->capturing) {
+    // once per Dirichlet set and graph: list the rows the constraints touch
+    int rc = ensure(c, c->d_bcrows, (size_t)A.nrows * 4 + 8);
+    if (rc) return rc;
+    int32_t* cnt = c->d_bcrows.as<int32_t>() + A.nrows;
+    k_fill_i32<<<1, 64, 0, c->stream>>>(1, cnt, 0);
+    k_bc_rows_list<<<nb(A.nrows * 16, 256), 256, 0, c->stream>>>(A.nrows, rp, ci, fl, c->d_bcrows.as<int32_t>(), cnt);
+    FEDD_HIP(c, hipGetLastError());
+    int32_t h = 0;
+    FEDD_HIP(c, hipMemcpyAsync(&h, cnt, 4, hipMemcpyDeviceToHost, c->stream));
+    FEDD_HIP(c, hipStreamSynchronize(c->stream));
+    c->nbc_rows = h;
+    c->bcrows_graph = A.d_colind.p;
+  }
+  if (c->nbc_rows >= 0 && c->bcrows_graph == A.d_colind.p) {
+    if (c->nbc_rows > 0)
+      k_bc_matrix_rows<<<nb(c->nbc_rows * 16, 256), 256, 0, c->stream>>>(c->nbc_rows, c->d_bcrows.as<int32_t>(), rp,
+                                                                          ci, A.d_vals.as<double>(), fl);
+  } else {
+    // a capture is open and the list is not built: every row
+    k_bc_matrix<<<nb(A.nrows * 16, 256), 256, 0, c->stream>>>(A.nrows, rp, ci, A.d_vals.as<double>(), fl);
+  }
   FEDD_HIP(c, hipGetLastError());
   return FEDD_OK;
 }
@@ -558,6 +620,7 @@ int fedd_dirichlet_set(fedd_ctx* cc, const int32_t* dofs, int64_t n) {
   int rc = upload(c, c->d_bcflag, flag.data(), nd);
   if (rc) return rc;
   c->nbc = n;
+  c->nbc_rows = -1;  // the row list follows the Dirichlet set
   c->bcflag_host = flag;
   mg_free(c);  // the multigrid masks follow the Dirichlet set
   FEDD_HIP(c, hipStreamSynchronize(c->stream));

@@ -108,3 +108,35 @@ def test_newton_matches_cpu_newton(gpu_ctx):
     assert np.max(np.abs(uref)) > 0.01  # a genuinely nonlinear load
     # quadratic convergence at the end of each load step
     assert st["newton_iters"] <= 3 * 8
+
+
+def test_dirichlet_rows_and_columns(gpu_ctx):
+    """fedd_apply_dirichlet: constrained rows and columns become identity rows / zero
+    columns (BCBuilder::setDirichletBC, BCBuilder_def.hpp:590-618, with the column
+    elimination that keeps the system symmetric for CG), every other entry untouched.
+    Two different constraint sets on one graph, then a rebuilt graph: each must use its
+    own list of touched rows."""
+    pts, el, _ = _problem(2)
+    n = 3 * pts.shape[0]
+    u = W.smoothed_displacement(pts, el, 5, 0.05)
+    gpu_ctx.mesh_upload(N.MESH_PRIMARY, pts, el)
+    gpu_ctx.graph_build(N.GRAPH_VECTOR)
+    rng = np.random.default_rng(4)
+    for trial in range(3):
+        if trial == 2:
+            gpu_ctx.graph_build(N.GRAPH_VECTOR)
+        gpu_ctx.vector_upload(N.VEC_U, u)
+        gpu_ctx.assembly_nonlinear_elasticity(1, EMOD, NU, N.FP64)
+        rp, ci = gpu_ctx.block_graph(N.BLOCK_A)
+        before = gpu_ctx.block_values(N.BLOCK_A)
+        dofs = np.sort(rng.choice(n, size=n // (5 + trial), replace=False)).astype(np.int32)
+        gpu_ctx.dirichlet_set(dofs)
+        gpu_ctx.apply_dirichlet()
+        after = gpu_ctx.block_values(N.BLOCK_A)
+        flag = np.zeros(n, bool)
+        flag[dofs] = True
+        rows = np.repeat(np.arange(n), np.diff(rp))
+        hit = flag[rows] | flag[ci]
+        expect = np.where(hit, (rows == ci).astype(np.float64), before)
+        assert np.array_equal(after, expect), trial
+    gpu_ctx.dirichlet_set(np.zeros(0, np.int32))
