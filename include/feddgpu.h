@@ -208,7 +208,10 @@ int fedd_assembly_nonlinear_elasticity_efield(fedd_ctx* ctx, int neo_hooke_model
 /* ------------------------------------------------------- nonlinear solve */
 /* Dirichlet dofs of the primary field (BCBuilder::addBC / setSystem,
  * BCBuilder_def.hpp:590-618): constrained rows and columns of the assembled tangent
- * become identity rows/columns, their residual entries zero. */
+ * become identity rows/columns, their residual entries zero.  The first
+ * fedd_apply_dirichlet after a fedd_dirichlet_set or a graph build lists the rows the
+ * constraints touch (one stream synchronisation); later calls rewrite only those rows.
+ * Inside an open capture with no list built, every row is scanned instead. */
 int fedd_dirichlet_set(fedd_ctx* ctx, const int32_t* dofs, int64_t n);
 int fedd_apply_dirichlet(fedd_ctx* ctx);
 /* Uniform traction t on the face {x_axis == value} of the P2 mesh into VEC_FEXT
