@@ -18,8 +18,28 @@ int fail(Ctx* c, int code, const std::string& msg) {
   return code;
 }
 
+void sync_streams(Ctx* c) {
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipStreamSynchronize(c->side);
+}
+
+void drop_user_graph(Ctx* c, const char* why) {
+  if (!c->gexec && !c->graph) return;
+  sync_streams(c);  // a replay may still be running
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+  c->graph_dropped = why;
+}
+
 int alloc(Ctx* c, DevBuf& b, size_t bytes) {
   if (c->capturing) return fail(c, FEDD_ERR_STATE, "device allocation while a capture is open: run the call once before capturing");
+  if (b.p) {
+    // kernels queued on the streams or recorded in the captured graph may read b
+    drop_user_graph(c, "a device buffer the captured graph uses was reallocated (resized vector, mesh or graph): capture again");
+    sync_streams(c);
+  }
   b.release();
   if (bytes == 0) return FEDD_OK;
   FEDD_HIP(c, hipMalloc(&b.p, bytes));
@@ -431,8 +451,10 @@ int fedd_synchronize(fedd_ctx* c) {
 
 int fedd_mesh_upload(fedd_ctx* c, int slot, const double* pts, int64_t npts,
                      const int32_t* elems, int64_t nelem, int npe) {
-  if (c) mg_free(c);
   if (!c) return FEDD_ERR_ARG;
+  if (c->capturing) return fail(c, FEDD_ERR_STATE, "a mesh upload cannot be captured");
+  drop_user_graph(c, "the mesh was replaced after the capture: capture again");
+  mg_free(c);
   if (slot < 0 || slot > 1) return fail(c, FEDD_ERR_ARG, "mesh slot must be 0 or 1");
   if (!pts || !elems || npts <= 0 || nelem <= 0)
     return fail(c, FEDD_ERR_ARG, "empty mesh");
@@ -470,8 +492,10 @@ int fedd_mesh_upload(fedd_ctx* c, int slot, const double* pts, int64_t npts,
 }
 
 int fedd_graph_build(fedd_ctx* c, int kind) {
-  if (c) mg_free(c);
   if (!c) return FEDD_ERR_ARG;
+  if (c->capturing) return fail(c, FEDD_ERR_STATE, "a graph build cannot be captured");
+  drop_user_graph(c, "the CRS graph was rebuilt after the capture: capture again");
+  mg_free(c);
   if (kind < FEDD_GRAPH_SCALAR || kind > FEDD_GRAPH_NS) return fail(c, FEDD_ERR_ARG, "unknown graph kind");
   Mesh& m = c->mesh[0];
   if (!m.valid()) return fail(c, FEDD_ERR_STATE, "no primary mesh");
@@ -484,6 +508,7 @@ int fedd_graph_build(fedd_ctx* c, int kind) {
   }
   (void)hipSetDevice(c->device);
   c->graph_kind = GK_NONE;
+  sync_streams(c);  // queued kernels may still read the old blocks
   for (auto& b : c->blk) {
     b.nrows = b.ncols = b.nnz = 0;
     b.d_rowptr.release();
@@ -832,10 +857,14 @@ int fedd_capture_end(fedd_ctx* c, int64_t* nodes) {
     if (g) (void)hipGraphDestroy(g);
     return fail(c, FEDD_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
   }
-  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-  if (c->graph) (void)hipGraphDestroy(c->graph);
+  if (c->gexec || c->graph) {
+    sync_streams(c);  // the previous graph may still be replaying
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+  }
   c->gexec = nullptr;
   c->graph = g;
+  c->graph_dropped.clear();
   FEDD_HIP(c, hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
   if (nodes) {
     size_t cnt = 0;
@@ -847,7 +876,8 @@ int fedd_capture_end(fedd_ctx* c, int64_t* nodes) {
 
 int fedd_graph_launch(fedd_ctx* c) {
   if (!c) return FEDD_ERR_ARG;
-  if (!c->gexec) return fail(c, FEDD_ERR_STATE, "no captured graph");
+  if (!c->gexec)
+    return fail(c, FEDD_ERR_STATE, c->graph_dropped.empty() ? std::string("no captured graph") : c->graph_dropped);
   if (c->capturing) return fail(c, FEDD_ERR_STATE, "a capture is open");
   (void)hipSetDevice(c->device);
   FEDD_HIP(c, hipGraphLaunch(c->gexec, c->stream));

@@ -143,6 +143,8 @@ struct Ctx {
   bool capturing = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
+  // why the captured graph was destroyed (a buffer it refers to was freed or moved)
+  std::string graph_dropped;
   // timing
   hipEvent_t ev[16] = {};
   bool timing = false;
@@ -168,7 +170,15 @@ std::string& free_error();
                           std::string(#call) + ": " + hipGetErrorString(_e));    \
   } while (0)
 
+// Frees b (after the context's streams are idle) and allocates `bytes`.  Freeing a
+// buffer destroys the caller's captured graph first (drop_user_graph).
 int alloc(Ctx* c, DevBuf& b, size_t bytes);
+// Waits until the context's main and side streams are idle.
+void sync_streams(Ctx* c);
+// Waits for the streams, then destroys the captured graph of fedd_capture_begin/end:
+// its nodes hold the addresses of buffers that are about to be freed or moved, so a
+// later fedd_graph_launch must fail (FEDD_ERR_STATE with `why`) instead of replaying.
+void drop_user_graph(Ctx* c, const char* why);
 int upload(Ctx* c, DevBuf& b, const void* host, size_t bytes);
 
 // Kernel-timing bracket: events recorded on the context stream around a launch

@@ -92,7 +92,11 @@ struct MgHost {
   }
 };
 
+// The replayed graphs and the level buffers may still be in use by work queued on the
+// context's streams: wait for them before destroying the executables and freeing.
 void mg_free(Ctx* c) {
+  if (!c->mg) return;
+  sync_streams(c);
   delete c->mg;
   c->mg = nullptr;
 }
@@ -1303,7 +1307,10 @@ int mg_vcycle(Ctx* c, const double* b, double* x) {
     return FEDD_OK;
   }
   if (!g->vexec || g->vb != b || g->vx != x) {
-    if (g->vexec) (void)hipGraphExecDestroy(g->vexec);
+    if (g->vexec) {
+      sync_streams(c);  // an earlier replay of this executable may still be running
+      (void)hipGraphExecDestroy(g->vexec);
+    }
     g->vexec = nullptr;
     int rc = capture(c, &g->vexec, [&]() -> int {
       vcycle(c, 0, b, x);

@@ -616,13 +616,15 @@ int fedd_dirichlet_set(fedd_ctx* cc, const int32_t* dofs, int64_t n) {
     if (dofs[k] < 0 || dofs[k] >= nd) return fail(c, FEDD_ERR_ARG, "Dirichlet dof out of range");
     flag[dofs[k]] = 1;
   }
+  if (c->capturing) return fail(c, FEDD_ERR_STATE, "host transfers cannot be captured");
   (void)hipSetDevice(c->device);
+  mg_free(c);  // the multigrid masks follow the Dirichlet set (waits for queued work first)
+  sync_streams(c);  // queued kernels may still read the previous flags
   int rc = upload(c, c->d_bcflag, flag.data(), nd);
   if (rc) return rc;
   c->nbc = n;
   c->nbc_rows = -1;  // the row list follows the Dirichlet set
   c->bcflag_host = flag;
-  mg_free(c);  // the multigrid masks follow the Dirichlet set
   FEDD_HIP(c, hipStreamSynchronize(c->stream));
   return FEDD_OK;
 }

@@ -134,3 +134,51 @@ def test_neighbour_limit(ctx):
     assert _code(e) == ERR_LIMIT
     with pytest.raises(N.FeddError, match="needs a scalar graph"):
         ctx.assembly_laplace(N.FP64)
+
+
+def test_captured_graph_dropped_when_buffers_move(ctx):
+    """A captured graph holds the device addresses of the mesh, graph and vector buffers.
+    Rebuilding the graph, replacing the mesh or resizing a vector frees them, so the
+    library destroys the captured graph and fedd_graph_launch refuses with
+    FEDD_ERR_STATE instead of replaying kernels on freed memory."""
+    from feddlib_amd import workloads as W
+    p2, e2, p1, e1, u, p = W.ns_problem(2, 0)
+
+    def setup():
+        ctx.mesh_upload(N.MESH_PRIMARY, p2, e2)
+        ctx.mesh_upload(N.MESH_PRESSURE, p1, e1)
+        ctx.graph_build(N.GRAPH_NS)
+        ctx.vector_upload(N.VEC_U, u)
+        ctx.vector_upload(N.VEC_P, p)
+        ctx.assembly_navier_stokes(0.01, 1.0, True, N.NS_JACOBIAN, reassemble=False)
+
+    def capture():
+        ctx.capture_begin()
+        ctx.assembly_navier_stokes(0.01, 1.0, True, N.NS_JACOBIAN, reassemble=False)
+        ctx.capture_end()
+        ctx.graph_launch()
+        ctx.synchronize()
+
+    setup()
+    capture()
+    ref = ctx.block_values(N.BLOCK_A)
+    ctx.graph_build(N.GRAPH_NS)  # frees every block the graph writes
+    with pytest.raises(N.FeddError, match="rebuilt after the capture") as e:
+        ctx.graph_launch()
+    assert _code(e) == ERR_STATE
+    ctx.vector_upload(N.VEC_U, u)
+    ctx.vector_upload(N.VEC_P, p)
+    capture()
+    ctx.mesh_upload(N.MESH_PRESSURE, p1, e1)  # a new mesh drops the graph too
+    with pytest.raises(N.FeddError, match="mesh was replaced") as e:
+        ctx.graph_launch()
+    assert _code(e) == ERR_STATE
+    setup()
+    capture()
+    rp, _ = ctx.block_graph(N.BLOCK_A)
+    assert parity_ratio(ctx.block_values(N.BLOCK_A), ref, rp) <= 1.0
+    # a same-length upload keeps the buffers (and the graph); a replay reads the new iterate
+    ctx.vector_upload(N.VEC_U, 2.0 * u)
+    ctx.graph_launch()
+    ctx.synchronize()
+    assert parity_ratio(ctx.block_values(N.BLOCK_A), ref, rp) > 1e3
