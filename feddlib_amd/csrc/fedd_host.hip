@@ -406,6 +406,7 @@ int fedd_ctx_create(int device, fedd_ctx** out) {
     return FEDD_ERR_HIP;
   }
   for (auto& e : c->ev) (void)hipEventCreate(&e);
+  noalloc_init(c);
   *out = c;
   return FEDD_OK;
 }

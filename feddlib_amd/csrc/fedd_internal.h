@@ -217,6 +217,8 @@ int mg_refresh(Ctx* c);
 int mg_refresh_fine(Ctx* c);
 int mg_vcycle(Ctx* c, const double* b, double* x);
 void mg_free(Ctx* c);
+// Looks up this device's copy of the no-allocation solve's work array (fedd_solve.hip).
+void noalloc_init(Ctx* c);
 
 // Host worker threads (at most 16).
 inline unsigned nthreads() {

@@ -10,6 +10,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 
 #include "fedd_internal.h"
 
@@ -374,23 +376,26 @@ int apply_dirichlet_matrix(Ctx* c) {
   return FEDD_OK;
 }
 
-// Preconditioned CG on block A with b = VEC_B, x = VEC_X (starting from 0).
-int pcg_solve(Ctx* c, double rtol, double atol, int maxit, int* iters, double* relres) {
+// Work space of the block-Jacobi CG: four vectors of n, the 3x3 block inverses (9 per
+// node = 3n) and the PCG_SCAL reduction slots.
+struct PcgWork {
+  double *r, *z, *p, *Ap, *binv, *scal;
+};
+
+// Block-Jacobi preconditioned CG on block A: A x = b, x starting from 0.  Every work
+// array is written before it is read (r = b, scal and x cleared by kernels), so the
+// work space may hold anything on entry.  timed: kernel-timing events around the SpMV.
+static int pcg_core(Ctx* c, const double* b, double* x, const PcgWork& w, double rtol, double atol, int maxit,
+                    int* iters, double* relres, bool timed) {
   const Block& A = c->blk[0];
   const int64_t n = A.nrows, nnode = n / 3;
   int rc;
-  for (int k = 0; k < 4; ++k)
-    if ((rc = ensure(c, c->d_kw[k], n * 8))) return rc;
-  if ((rc = ensure(c, c->d_kw[4], nnode * 9 * 8))) return rc;
-  if ((rc = ensure(c, c->d_scal, PCG_SCAL * 8))) return rc;
-  if ((rc = ensure_vec(c, FEDD_VEC_X, n))) return rc;
-  double* r = c->d_kw[0].as<double>();
-  double* z = c->d_kw[1].as<double>();
-  double* p = c->d_kw[2].as<double>();
-  double* Ap = c->d_kw[3].as<double>();
-  double* binv = c->d_kw[4].as<double>();
-  double* x = c->vec[FEDD_VEC_X].as<double>();
-  double* scal = c->d_scal.as<double>();
+  double* r = w.r;
+  double* z = w.z;
+  double* p = w.p;
+  double* Ap = w.Ap;
+  double* binv = w.binv;
+  double* scal = w.scal;
   const int64_t* rp = A.d_rowptr.as<int64_t>();
   const int32_t* ci = A.d_colind.as<int32_t>();
   const double* va = A.d_vals.as<double>();
@@ -401,7 +406,7 @@ int pcg_solve(Ctx* c, double rtol, double atol, int maxit, int* iters, double* r
   const unsigned g_nodes = std::min(nb(nnode, 256), gmax);
   const unsigned g_spmv = std::min(nb(nnode * FEDD_SPMV_LANES, 256), gmax), g_dofs = nb(n, 256);
   k_block_diag<<<g_nodes, 256, 0, c->stream>>>(nnode, rp, ci, va, binv);
-  FEDD_HIP(c, hipMemcpyAsync(r, c->vec[FEDD_VEC_B].p, n * 8, hipMemcpyDeviceToDevice, c->stream));
+  FEDD_HIP(c, hipMemcpyAsync(r, b, n * 8, hipMemcpyDeviceToDevice, c->stream));
   k_fill<<<g_dofs, 256, 0, c->stream>>>(n, x, 0.0);
   k_fill<<<1, 256, 0, c->stream>>>(PCG_SCAL, scal, 0.0);
   k_pcg_update<<<g_nodes, 256, 0, c->stream>>>(nnode, x, r, p, Ap, z, binv, scal, -1, 1);
@@ -426,9 +431,9 @@ int pcg_solve(Ctx* c, double rtol, double atol, int maxit, int* iters, double* r
   if (bnorm > 0.0 && rnorm > target) {
     while (it < maxit) {
       const int t = it;
-      ktimer_begin(c);
+      if (timed) ktimer_begin(c);
       k_bspmv_dot<FEDD_SPMV_LANES><<<g_spmv, 256, 0, c->stream>>>(nnode, nptr, nidx, va, p, Ap, scal + PCG_PAP(t));
-      ktimer_end(c, "bspmv");
+      if (timed) ktimer_end(c, "bspmv");
       k_pcg_update<<<g_nodes, 256, 0, c->stream>>>(nnode, x, r, p, Ap, z, binv, scal, t, 0);
       k_pcg_p<<<g_dofs, 256, 0, c->stream>>>(n, p, z, scal, t, 0);
       ++it;
@@ -442,9 +447,61 @@ int pcg_solve(Ctx* c, double rtol, double atol, int maxit, int* iters, double* r
     }
   }
   FEDD_HIP(c, hipGetLastError());
+  FEDD_HIP(c, hipStreamSynchronize(c->stream));
   if (iters) *iters = it;
   if (relres) *relres = bnorm > 0 ? rnorm / bnorm : 0.0;
   return FEDD_OK;
+}
+
+// Preconditioned CG on block A with b = VEC_B, x = VEC_X (starting from 0), work space
+// allocated in the context.
+int pcg_solve(Ctx* c, double rtol, double atol, int maxit, int* iters, double* relres) {
+  const int64_t n = c->blk[0].nrows, nnode = n / 3;
+  int rc;
+  for (int k = 0; k < 4; ++k)
+    if ((rc = ensure(c, c->d_kw[k], n * 8))) return rc;
+  if ((rc = ensure(c, c->d_kw[4], nnode * 9 * 8))) return rc;
+  if ((rc = ensure(c, c->d_scal, PCG_SCAL * 8))) return rc;
+  if ((rc = ensure_vec(c, FEDD_VEC_X, n))) return rc;
+  const PcgWork w{c->d_kw[0].as<double>(), c->d_kw[1].as<double>(), c->d_kw[2].as<double>(),
+                  c->d_kw[3].as<double>(), c->d_kw[4].as<double>(), c->d_scal.as<double>()};
+  return pcg_core(c, c->vec[FEDD_VEC_B].as<double>(), c->vec[FEDD_VEC_X].as<double>(), w, rtol, atol, maxit,
+                  iters, relres, true);
+}
+
+// ------------------------------------------------------ no-allocation solve
+// Work space of fedd_linear_solve_noalloc: one file-scope device array per GPU, held
+// until the process ends.  Host code never uses the array's name as an address (on the
+// host it is only a stand-in for the symbol); noalloc_init looks up each device's copy
+// with hipGetSymbolAddress and every launch takes that pointer.
+constexpr int64_t kNoallocDoubles = FEDD_NOALLOC_SCRATCH_BYTES / 8;
+static __device__ __align__(256) double scratch[kNoallocDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};  // device address of `scratch` per device (null: not looked up)
+std::string g_scratch_err[kMaxDevices];
+
+// Segment sizes are rounded to 256 bytes (32 doubles).
+inline int64_t pad32(int64_t v) { return (v + 31) & ~int64_t(31); }
+constexpr int64_t kScalPad = (PCG_SCAL + 31) & ~31;
+inline int64_t noalloc_doubles(int64_t n) { return kScalPad + 4 * pad32(n) + pad32(3 * n); }
+}  // namespace
+
+// Called by fedd_ctx_create with the context's device current: looks up this device's
+// copy of `scratch` once per device.  A failed lookup is kept and reported by
+// fedd_linear_solve_noalloc; the other entry points do not need the array.
+void noalloc_init(Ctx* c) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (c->device < 0 || c->device >= kMaxDevices || g_scratch[c->device]) return;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    g_scratch_err[c->device] = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    return;
+  }
+  g_scratch[c->device] = ptr;
 }
 
 // CG preconditioned with one geometric-multigrid V-cycle per iteration (structured
@@ -645,6 +702,37 @@ int fedd_linear_solve(fedd_ctx* cc, double rtol, int maxit, int* iters, double* 
   (void)hipSetDevice(c->device);
   if (c->precond >= 1) return pcg_mg_solve(c, rtol, 0.0, maxit, iters, relres, 1, 0.0, c->precond - 1);
   return pcg_solve(c, rtol, 0.0, maxit, iters, relres);
+}
+
+int fedd_linear_solve_noalloc(fedd_ctx* cc, const double* d_b, double* d_x, int64_t n, double rtol, int maxit,
+                              int* iters, double* relres) {
+  Ctx* c = cc;
+  if (!c || !d_b || !d_x || !(rtol > 0) || maxit < 1) return FEDD_ERR_ARG;
+  if (c->graph_kind != FEDD_GRAPH_VECTOR) return fail(c, FEDD_ERR_STATE, "needs a vector graph");
+  if (c->capturing) return fail(c, FEDD_ERR_STATE, "the solve synchronises: it cannot be captured");
+  if (n != c->blk[0].nrows) return fail(c, FEDD_ERR_ARG, "n must equal the rows of block A");
+  if (n > FEDD_NOALLOC_MAX_DOFS || noalloc_doubles(n) > kNoallocDoubles)
+    return fail(c, FEDD_ERR_LIMIT, "the no-allocation solve holds at most FEDD_NOALLOC_MAX_DOFS (599040) unknowns");
+  double* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (c->device >= 0 && c->device < kMaxDevices) base = static_cast<double*>(g_scratch[c->device]);
+    if (!base)
+      return fail(c, FEDD_ERR_HIP,
+                  c->device >= 0 && c->device < kMaxDevices && !g_scratch_err[c->device].empty()
+                      ? g_scratch_err[c->device]
+                      : std::string("no scratch address for this device"));
+  }
+  (void)hipSetDevice(c->device);
+  const int64_t np = pad32(n);
+  PcgWork w;
+  w.scal = base;
+  w.r = base + kScalPad;
+  w.z = w.r + np;
+  w.p = w.z + np;
+  w.Ap = w.p + np;
+  w.binv = w.Ap + np;
+  return pcg_core(c, d_b, d_x, w, rtol, 0.0, maxit, iters, relres, false);
 }
 
 int fedd_set_preconditioner(fedd_ctx* cc, int kind) {

@@ -102,6 +102,8 @@ _SIGS = {
     "fedd_traction_load": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, _dp]),
     "fedd_set_preconditioner": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fedd_linear_solve": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _ip, _dp]),
+    "fedd_linear_solve_noalloc": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, ctypes.c_double, ctypes.c_int,
+                                                 _ip, _dp]),
     "fedd_newton_elasticity": (ctypes.c_int, [_vp, ctypes.POINTER(NewtonParams), ctypes.POINTER(NewtonStats),
                                               _dp, ctypes.c_int]),
     "fedd_mlp_init": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_uint64]),
@@ -289,6 +291,12 @@ class Context:
         a = np.ascontiguousarray(a, np.float64).ravel()
         self._chk(self._L.fedd_vector_upload(self.h, which, _p(a, _dp), a.size))
 
+    def vector_device_ptr(self, which):
+        """(device address as int, length) of a context vector."""
+        d, n = _dp(), ctypes.c_int64()
+        self._chk(self._L.fedd_vector_device_ptr(self.h, which, ctypes.byref(d), ctypes.byref(n)))
+        return ctypes.cast(d, ctypes.c_void_p).value or 0, n.value
+
     def vector_download(self, which, n):
         a = np.zeros(n)
         self._chk(self._L.fedd_vector_download(self.h, which, _p(a, _dp), n))
@@ -353,6 +361,13 @@ class Context:
     def linear_solve(self, rtol, maxit):
         it, rr = ctypes.c_int(), ctypes.c_double()
         self._chk(self._L.fedd_linear_solve(self.h, rtol, maxit, ctypes.byref(it), ctypes.byref(rr)))
+        return it.value, rr.value
+
+    def linear_solve_noalloc(self, d_b, d_x, n, rtol, maxit):
+        """fedd_linear_solve_noalloc on caller-owned device vectors (integer addresses)."""
+        it, rr = ctypes.c_int(), ctypes.c_double()
+        self._chk(self._L.fedd_linear_solve_noalloc(self.h, ctypes.c_void_p(d_b), ctypes.c_void_p(d_x), n, rtol,
+                                                    maxit, ctypes.byref(it), ctypes.byref(rr)))
         return it.value, rr.value
 
     def newton_elasticity(self, model=1, E=1000.0, nu=0.3, load_steps=5, abs_tol=1e-8, max_newton=30,

@@ -227,6 +227,21 @@ int fedd_linear_solve(fedd_ctx* ctx, double rtol, int maxit, int* iters, double*
  * dof component, Galerkin coarse operators, the same smoother and coarsest solve).  1 and 2 stand in for the reference's
  * FROSch/MueLu preconditioners (Preconditioner_def.hpp:206-291). */
 int fedd_set_preconditioner(fedd_ctx* ctx, int kind);
+/* No counterpart in the reference.  For callers that cannot allocate device memory
+ * (code under stream capture, tight serving loops, harnesses that forbid hipMalloc):
+ * the block-Jacobi CG of fedd_linear_solve (preconditioner 0, whatever
+ * fedd_set_preconditioner chose) on block A, A x = b from x = 0, with caller-owned
+ * device vectors d_b (read) and d_x (written), n = rows of A.  Its work space (four
+ * Krylov vectors, the 3x3 block inverses, the reduction slots) is one
+ * FEDD_NOALLOC_SCRATCH_BYTES device array per GPU, held until the process ends, so
+ * n <= FEDD_NOALLOC_MAX_DOFS (FEDD_ERR_LIMIT above).  It calls no hipMalloc / hipFree.
+ * Every call on one GPU shares that array: calls that overlap on one GPU (two host
+ * threads, or two contexts on the same device) race.  It synchronises the context
+ * stream, so it cannot be captured. */
+#define FEDD_NOALLOC_SCRATCH_BYTES (32ll << 20)
+#define FEDD_NOALLOC_MAX_DOFS 599040
+int fedd_linear_solve_noalloc(fedd_ctx* ctx, const double* d_b, double* d_x, int64_t n, double rtol, int maxit,
+                              int* iters, double* relres);
 
 typedef struct {
   int neo_hooke_model;  /* Parameter/"Neo-Hooke Modell": 1 -> skr2 law, otherwise skr3 */
