@@ -489,6 +489,12 @@ int fedd_mesh_upload(fedd_ctx* c, int slot, const double* pts, int64_t npts,
   c->colour_ptr.clear();
   c->colour_perm.clear();
   c->newton_step = 0;
+  if (slot == 0) {  // the Dirichlet set belongs to the primary mesh
+    c->nbc = 0;
+    c->nbc_dofs = 0;
+    c->nbc_rows = -1;
+    c->bcflag_host.clear();
+  }
   return FEDD_OK;
 }
 

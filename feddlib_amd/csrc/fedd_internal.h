@@ -121,6 +121,10 @@ struct Ctx {
   // Dirichlet constraints and Krylov work space (elasticity Newton solve)
   DevBuf d_bcflag;      // uint8 per dof of the primary field
   int64_t nbc = 0;
+  DevBuf d_bcdofs;      // int32: the constrained dofs, ascending (nbc_dofs of them)
+  int64_t nbc_dofs = 0;
+  int bc_per_node = 3;  // dofs per node the Dirichlet set was made for (1: scalar graph)
+  DevBuf d_bcval;       // double per dof of the primary field: Dirichlet values (absent: 0)
   // rows of block A that hold a constrained dof in their row or columns (int32), built
   // on the device for the current Dirichlet set and graph; nbc_rows < 0: not built
   DevBuf d_bcrows;

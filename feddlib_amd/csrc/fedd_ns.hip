@@ -220,6 +220,63 @@ struct NsTensors<1> {
   static constexpr int row = 4;
 };
 
+// Convection row N_c = int phi_r (uhat . grad_hat phi_c) (without |det|), uhat_ax =
+// sum_d Bi[ax][d] u_d.  grad_hat phi_c is linear, so it equals its vertex interpolant
+// sum_m DV[ax][c][m] lambda_m, and
+//   N_c = sum_(ax,m) DV[ax][c][m] Y[ax][m],  Y[ax][m] = sum_d Bi[ax][d] Z[d][m],
+//   Z[d][m] = sum_k u_kd int lambda_m phi_r phi_k = sum_k u_kd L[m][k]
+// (L is the same table as the Newton term's): 171 FMA per row instead of the 276 of
+// the direct contraction with T[ax][k][c] = int phi_r phi_k d_ax phi_c.
+#ifndef FEDD_NS_N_VIA_L
+#define FEDD_NS_N_VIA_L 1
+#endif
+template <typename T, int CLS>
+__device__ __forceinline__ void convection_row(const T Bi[3][3], const T U[10][3], T N[10]) {
+  using NT = NsTensors<CLS>;
+#if FEDD_NS_N_VIA_L
+  T Z[3][4];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int mm = 0; mm < 4; ++mm) {
+      T z = 0;
+#pragma unroll
+      for (int k = 0; k < 10; ++k)
+        if (NT::L[mm][k] != 0.0) z += U[k][d] * (T)NT::L[mm][k];
+      Z[d][mm] = z;
+    }
+  T Y[3][4];
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+    for (int mm = 0; mm < 4; ++mm) Y[ax][mm] = Bi[ax][0] * Z[0][mm] + Bi[ax][1] * Z[1][mm] + Bi[ax][2] * Z[2][mm];
+#pragma unroll
+  for (int c = 0; c < 10; ++c) {
+    T n = 0;
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+      for (int mm = 0; mm < 4; ++mm)
+        if (ft::DV[ax][c][mm] != 0.0) n += Y[ax][mm] * (T)ft::DV[ax][c][mm];
+    N[c] = n;
+  }
+#else
+#pragma unroll
+  for (int c = 0; c < 10; ++c) N[c] = 0;
+#pragma unroll
+  for (int k = 0; k < 10; ++k) {
+    T ut[3];  // ut_ax = sum_d Bi[ax][d] u_kd
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) ut[ax] = Bi[ax][0] * U[k][0] + Bi[ax][1] * U[k][1] + Bi[ax][2] * U[k][2];
+#pragma unroll
+    for (int c = 0; c < 10; ++c)
+#pragma unroll
+      for (int ax = 0; ax < 3; ++ax)
+        if (NT::TT[ax][k][c] != 0.0) N[c] += ut[ax] * (T)NT::TT[ax][k][c];
+  }
+#endif
+}
+
 // ---------------------------------------------------------------------------
 // Navier-Stokes velocity rows
 // ---------------------------------------------------------------------------
@@ -302,19 +359,7 @@ struct VelRow {
     }
     {
       T Nacc[10];
-#pragma unroll
-      for (int c = 0; c < 10; ++c) Nacc[c] = 0;
-#pragma unroll
-      for (int k = 0; k < 10; ++k) {
-        T ut[3];  // ut_ax = sum_d Bi[ax][d] u_kd
-#pragma unroll
-        for (int ax = 0; ax < 3; ++ax) ut[ax] = Bi[ax][0] * U[k][0] + Bi[ax][1] * U[k][1] + Bi[ax][2] * U[k][2];
-#pragma unroll
-        for (int c = 0; c < 10; ++c)
-#pragma unroll
-          for (int ax = 0; ax < 3; ++ax)
-            if (NT::TT[ax][k][c] != 0.0) Nacc[c] += ut[ax] * (T)NT::TT[ax][k][c];
-      }
+      convection_row<T, CLS>(Bi, U, Nacc);
 #pragma unroll
       for (int c = 0; c < 10; ++c) S[c] = (T)((double)(S[c] + (T)a.rho * Nacc[c]) * adet);
     }
@@ -539,16 +584,11 @@ __device__ __forceinline__ void res_row(const NsArgs& a, int32_t e, int li, doub
   for (int k = 0; k < 10; ++k)
 #pragma unroll
     for (int d = 0; d < 3; ++d) uk[k][d] = (T)a.u[3 * (int64_t)nodes[k] + d];
+  {
+    T Nr[10];
+    convection_row<T, CLS>(Bi, uk, Nr);
 #pragma unroll
-  for (int k = 0; k < 10; ++k) {
-    T ut[3];
-#pragma unroll
-    for (int ax = 0; ax < 3; ++ax) ut[ax] = Bi[ax][0] * uk[k][0] + Bi[ax][1] * uk[k][1] + Bi[ax][2] * uk[k][2];
-#pragma unroll
-    for (int c = 0; c < 10; ++c)
-#pragma unroll
-      for (int ax = 0; ax < 3; ++ax)
-        if (NT::TT[ax][k][c] != 0.0) S[c] += (T)a.rho * ut[ax] * (T)NT::TT[ax][k][c];
+    for (int c = 0; c < 10; ++c) S[c] += (T)a.rho * Nr[c];
   }
   T ru[3] = {0, 0, 0};
 #pragma unroll

@@ -75,6 +75,35 @@ __global__ __launch_bounds__(256) void k_bc_matrix_rows(int64_t n, const int32_t
   }
 }
 
+// BCBuilder::setLocalRowOne / setLocalRowZero (BCBuilder_def.hpp:657-704) on the listed
+// rows of one CRS block: every stored entry of a constrained row becomes 0, except, in the
+// diagonal block (diag = 1), the entry in the row's own column, which becomes 1.  The
+// columns of other rows and the graph stay as they are.  16 lanes per row.
+__global__ __launch_bounds__(256) void k_bc_set_rows(int64_t n, const int32_t* __restrict__ rows,
+                                                     const int64_t* __restrict__ rowptr,
+                                                     const int32_t* __restrict__ colind, double* __restrict__ vals,
+                                                     int diag) {
+  const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if ((gid >> 4) >= n) return;
+  const int32_t row = rows[gid >> 4];
+  const int lane = (int)(gid & 15);
+  for (int64_t k = rowptr[row] + lane; k < rowptr[row + 1]; k += 16)
+    vals[k] = (diag && colind[k] == row) ? 1.0 : 0.0;
+}
+
+// Dirichlet values into a vector at the constrained dofs (BCBuilder_def.hpp:93-160, 234-300,
+// 350-420): mode 0 out = g (setRHS), 1 out = g - sub (setBCMinusVector), 2 out = sub - g
+// (setVectorMinusBC).
+__global__ __launch_bounds__(256) void k_bc_vector(int64_t n, const int32_t* __restrict__ dofs,
+                                                   const double* __restrict__ g, const double* __restrict__ sub,
+                                                   double* __restrict__ out, int mode) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t d = dofs[i];
+  const double gv = g[d];
+  out[d] = mode == 0 ? gv : mode == 1 ? gv - sub[d] : sub[d] - gv;
+}
+
 __global__ void k_fill_i32(int64_t n, int32_t* __restrict__ a, int32_t v) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) a[i] = v;
@@ -344,9 +373,19 @@ static int ensure_vec(Ctx* c, int w, int64_t n) {
   return FEDD_OK;
 }
 
+// The Dirichlet set must have been made for the current graph kind (dofs per node) and mesh.
+static int bc_matches_graph(Ctx* c) {
+  if (c->graph_kind == GK_NONE) return fail(c, FEDD_ERR_STATE, "no graph");
+  const int per_node = c->graph_kind == FEDD_GRAPH_SCALAR ? 1 : 3;
+  if (c->nbc_dofs > 0 && (c->bc_per_node != per_node || (int64_t)c->bcflag_host.size() != per_node * c->mesh[0].npts))
+    return fail(c, FEDD_ERR_STATE, "the Dirichlet set was made for another graph kind or mesh: set it again");
+  return FEDD_OK;
+}
+
 int apply_dirichlet_matrix(Ctx* c) {
   const Block& A = c->blk[0];
   if (c->nbc == 0) return FEDD_OK;
+  if (int rc = bc_matches_graph(c)) return rc;
   const int64_t* rp = A.d_rowptr.as<int64_t>();
   const int32_t* ci = A.d_colind.as<int32_t>();
   const uint8_t* fl = c->d_bcflag.as<uint8_t>();
@@ -666,7 +705,8 @@ extern "C" {
 int fedd_dirichlet_set(fedd_ctx* cc, const int32_t* dofs, int64_t n) {
   Ctx* c = cc;
   if (!c || (n > 0 && !dofs)) return FEDD_ERR_ARG;
-  const int64_t nd = 3 * c->mesh[0].npts;
+  const int per_node = c->graph_kind == FEDD_GRAPH_SCALAR ? 1 : 3;
+  const int64_t nd = per_node * c->mesh[0].npts;
   if (nd == 0) return fail(c, FEDD_ERR_STATE, "no primary mesh");
   std::vector<uint8_t> flag(nd, 0);
   for (int64_t k = 0; k < n; ++k) {
@@ -674,15 +714,80 @@ int fedd_dirichlet_set(fedd_ctx* cc, const int32_t* dofs, int64_t n) {
     flag[dofs[k]] = 1;
   }
   if (c->capturing) return fail(c, FEDD_ERR_STATE, "host transfers cannot be captured");
+  std::vector<int32_t> list;
+  for (int64_t d = 0; d < nd; ++d)
+    if (flag[d]) list.push_back((int32_t)d);
   (void)hipSetDevice(c->device);
   mg_free(c);  // the multigrid masks follow the Dirichlet set (waits for queued work first)
   sync_streams(c);  // queued kernels may still read the previous flags
   int rc = upload(c, c->d_bcflag, flag.data(), nd);
   if (rc) return rc;
+  if ((rc = upload(c, c->d_bcdofs, list.data(), list.size() * 4))) return rc;
+  // values follow the set: homogeneous until fedd_dirichlet_values
+  if (c->d_bcval.bytes != (size_t)nd * 8 && (rc = alloc(c, c->d_bcval, nd * 8))) return rc;
+  FEDD_HIP(c, hipMemsetAsync(c->d_bcval.p, 0, nd * 8, c->stream));
   c->nbc = n;
+  c->nbc_dofs = (int64_t)list.size();
+  c->bc_per_node = per_node;
   c->nbc_rows = -1;  // the row list follows the Dirichlet set
   c->bcflag_host = flag;
   FEDD_HIP(c, hipStreamSynchronize(c->stream));
+  return FEDD_OK;
+}
+
+int fedd_dirichlet_values(fedd_ctx* cc, const double* g, int64_t n) {
+  Ctx* c = cc;
+  if (!c || !g) return FEDD_ERR_ARG;
+  int rc = bc_matches_graph(c);
+  if (rc) return rc;
+  if (c->bcflag_host.empty()) return fail(c, FEDD_ERR_STATE, "no Dirichlet set (fedd_dirichlet_set)");
+  if (n != (int64_t)c->bcflag_host.size()) return fail(c, FEDD_ERR_ARG, "Dirichlet values: one per dof of the primary field");
+  if (c->capturing) return fail(c, FEDD_ERR_STATE, "host transfers cannot be captured");
+  (void)hipSetDevice(c->device);
+  if ((rc = upload(c, c->d_bcval, g, n * 8))) return rc;
+  FEDD_HIP(c, hipStreamSynchronize(c->stream));
+  return FEDD_OK;
+}
+
+int fedd_dirichlet_set_system(fedd_ctx* cc) {
+  Ctx* c = cc;
+  if (!c) return FEDD_ERR_ARG;
+  int rc = bc_matches_graph(c);
+  if (rc) return rc;
+  if (c->nbc_dofs == 0) return FEDD_OK;
+  (void)hipSetDevice(c->device);
+  const int32_t* rows = c->d_bcdofs.as<int32_t>();
+  const unsigned g = nb(c->nbc_dofs * 16, 256);
+  // diagonal block of the constrained field: identity rows (setLocalRowOne)
+  const Block& A = c->blk[0];
+  k_bc_set_rows<<<g, 256, 0, c->stream>>>(c->nbc_dofs, rows, A.d_rowptr.as<int64_t>(), A.d_colind.as<int32_t>(),
+                                          A.d_vals.as<double>(), 1);
+  if (c->graph_kind == FEDD_GRAPH_NS) {
+    // off-diagonal block of the same block row: zero rows (setLocalRowZero)
+    const Block& BT = c->blk[1];
+    k_bc_set_rows<<<g, 256, 0, c->stream>>>(c->nbc_dofs, rows, BT.d_rowptr.as<int64_t>(),
+                                            BT.d_colind.as<int32_t>(), BT.d_vals.as<double>(), 0);
+  }
+  FEDD_HIP(c, hipGetLastError());
+  return FEDD_OK;
+}
+
+int fedd_dirichlet_vector(fedd_ctx* cc, int mode, int out, int sub) {
+  Ctx* c = cc;
+  if (!c || mode < FEDD_BC_RHS || mode > FEDD_BC_VECTOR_MINUS_BC || out < 0 || out >= FEDD_NUM_VECTORS)
+    return FEDD_ERR_ARG;
+  if (mode != FEDD_BC_RHS && (sub < 0 || sub >= FEDD_NUM_VECTORS)) return FEDD_ERR_ARG;
+  int rc = bc_matches_graph(c);
+  if (rc) return rc;
+  const int64_t nd = (c->graph_kind == FEDD_GRAPH_SCALAR ? 1 : 3) * c->mesh[0].npts;
+  if (c->vec_n[out] != nd || (mode != FEDD_BC_RHS && c->vec_n[sub] != nd))
+    return fail(c, FEDD_ERR_STATE, "vector absent or not one entry per dof of the primary field");
+  if (c->nbc_dofs == 0) return FEDD_OK;
+  (void)hipSetDevice(c->device);
+  k_bc_vector<<<nb(c->nbc_dofs, 256), 256, 0, c->stream>>>(
+      c->nbc_dofs, c->d_bcdofs.as<int32_t>(), c->d_bcval.as<double>(),
+      mode == FEDD_BC_RHS ? nullptr : c->vec[sub].as<double>(), c->vec[out].as<double>(), mode);
+  FEDD_HIP(c, hipGetLastError());
   return FEDD_OK;
 }
 
@@ -742,7 +847,7 @@ int fedd_set_preconditioner(fedd_ctx* cc, int kind) {
   return FEDD_OK;
 }
 
-int fedd_apply_dirichlet(fedd_ctx* cc) {
+int fedd_apply_dirichlet_symmetric(fedd_ctx* cc) {
   Ctx* c = cc;
   if (!c) return FEDD_ERR_ARG;
   if (c->graph_kind != FEDD_GRAPH_VECTOR) return fail(c, FEDD_ERR_STATE, "needs a vector graph");
@@ -767,6 +872,7 @@ int fedd_newton_elasticity(fedd_ctx* cc, const fedd_newton_params* P, fedd_newto
   (void)hipSetDevice(c->device);
   const int64_t n = 3 * c->mesh[0].npts;
   int rc;
+  if ((rc = bc_matches_graph(c))) return rc;
   if ((rc = ensure_vec(c, FEDD_VEC_U, n))) return rc;
   if ((rc = ensure_vec(c, FEDD_VEC_RES_U, n))) return rc;
   std::memset(st, 0, sizeof(*st));

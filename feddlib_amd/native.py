@@ -17,6 +17,7 @@ GRAPH_SCALAR, GRAPH_VECTOR, GRAPH_NS = 0, 1, 2
 BLOCK_A, BLOCK_BT, BLOCK_B, BLOCK_C = 0, 1, 2, 3
 VEC_U, VEC_P, VEC_RES_U, VEC_RES_P, VEC_RHS, VEC_FEXT, VEC_X, VEC_B = 0, 1, 2, 3, 4, 5, 6, 7
 FP64, FP32_LOCAL = 0, 1
+BC_RHS, BC_MINUS_VECTOR, BC_VECTOR_MINUS_BC = 0, 1, 2
 NS_JACOBIAN, NS_FIXEDPOINT, NS_RHS = 0, 1, 2
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -98,7 +99,10 @@ _SIGS = {
     "fedd_assembly_nonlinear_elasticity_efield": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.c_int64,
                                                                  ctypes.c_double, ctypes.c_int]),
     "fedd_dirichlet_set": (ctypes.c_int, [_vp, _ip, ctypes.c_int64]),
-    "fedd_apply_dirichlet": (ctypes.c_int, [_vp]),
+    "fedd_apply_dirichlet_symmetric": (ctypes.c_int, [_vp]),
+    "fedd_dirichlet_values": (ctypes.c_int, [_vp, _dp, ctypes.c_int64]),
+    "fedd_dirichlet_set_system": (ctypes.c_int, [_vp]),
+    "fedd_dirichlet_vector": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "fedd_traction_load": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, _dp]),
     "fedd_set_preconditioner": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fedd_linear_solve": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _ip, _dp]),
@@ -348,8 +352,18 @@ class Context:
         d = np.ascontiguousarray(dofs, np.int32)
         self._chk(self._L.fedd_dirichlet_set(self.h, _p(d, _ip), d.size))
 
-    def apply_dirichlet(self):
-        self._chk(self._L.fedd_apply_dirichlet(self.h))
+    def apply_dirichlet_symmetric(self):
+        self._chk(self._L.fedd_apply_dirichlet_symmetric(self.h))
+
+    def dirichlet_values(self, g):
+        g = np.ascontiguousarray(g, np.float64).ravel()
+        self._chk(self._L.fedd_dirichlet_values(self.h, _p(g, _dp), g.size))
+
+    def dirichlet_set_system(self):
+        self._chk(self._L.fedd_dirichlet_set_system(self.h))
+
+    def dirichlet_vector(self, mode, out, sub=-1):
+        self._chk(self._L.fedd_dirichlet_vector(self.h, mode, out, sub))
 
     def traction_load(self, axis, value, t):
         t = np.ascontiguousarray(t, np.float64)

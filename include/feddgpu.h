@@ -206,14 +206,36 @@ int fedd_assembly_nonlinear_elasticity(fedd_ctx* ctx, int neo_hooke_model, doubl
 int fedd_assembly_nonlinear_elasticity_efield(fedd_ctx* ctx, int neo_hooke_model, const double* E_elem,
                                               int64_t nelem, double nu, int precision);
 /* ------------------------------------------------------- nonlinear solve */
-/* Dirichlet dofs of the primary field (BCBuilder::addBC / setSystem,
- * BCBuilder_def.hpp:590-618): constrained rows and columns of the assembled tangent
- * become identity rows/columns, their residual entries zero.  The first
- * fedd_apply_dirichlet after a fedd_dirichlet_set or a graph build lists the rows the
- * constraints touch (one stream synchronisation); later calls rewrite only those rows.
- * Inside an open capture with no list built, every row is scanned instead. */
+/* Dirichlet dofs of the primary field (the dofs BCBuilder::addBC flags,
+ * BCBuilder_def.hpp:590-700): one entry per constrained dof, dof = node for a scalar graph,
+ * 3 node + d otherwise (the graph built when the set is made decides).  Values are zero
+ * until fedd_dirichlet_values. */
 int fedd_dirichlet_set(fedd_ctx* ctx, const int32_t* dofs, int64_t n);
-int fedd_apply_dirichlet(fedd_ctx* ctx);
+/* Dirichlet values g, one per dof of the primary field (only the constrained ones are read). */
+int fedd_dirichlet_values(fedd_ctx* ctx, const double* g, int64_t n);
+/* BCBuilder::setSystem (BCBuilder_def.hpp:590-618) with setDirichletBC /
+ * setLocalRowOne / setLocalRowZero (:621-704) on the block row of the primary field:
+ * every constrained row of the diagonal block (A) becomes an identity row (1 in its own
+ * column, every other stored entry 0) and the same row of the off-diagonal block (NS B^T)
+ * a zero row.  Columns, the pressure rows (B, C) and the graph are left as they are. */
+int fedd_dirichlet_set_system(fedd_ctx* ctx);
+enum fedd_bc_vector_mode {
+  FEDD_BC_RHS = 0,            /* out = g        BCBuilder::setRHS (BCBuilder_def.hpp:93-160) */
+  FEDD_BC_MINUS_VECTOR = 1,   /* out = g - sub  BCBuilder::setBCMinusVector (:234-300) */
+  FEDD_BC_VECTOR_MINUS_BC = 2 /* out = sub - g  BCBuilder::setVectorMinusBC (:350-420) */
+};
+/* The Dirichlet values into vector `out` (a FEDD_VEC_* of one entry per primary-field dof)
+ * at the constrained dofs; `sub` is the subtracted vector of modes 1 and 2 (ignored for 0).
+ * The other entries of `out` are left as they are. */
+int fedd_dirichlet_vector(fedd_ctx* ctx, int mode, int out, int sub);
+/* Symmetric elimination for CG (no counterpart in the reference, whose setSystem above
+ * rewrites rows only): constrained rows AND columns of block A of a vector graph become
+ * identity rows/columns, which keeps an SPD tangent SPD; valid for homogeneous increments
+ * (the Newton solver's constrained increments are 0).  The first call after a
+ * fedd_dirichlet_set or a graph build lists the rows the constraints touch (one stream
+ * synchronisation); later calls rewrite only those rows.  Inside an open capture with no
+ * list built, every row is scanned instead. */
+int fedd_apply_dirichlet_symmetric(fedd_ctx* ctx);
 /* Uniform traction t on the face {x_axis == value} of the P2 mesh into VEC_FEXT
  * (surface term of the load-stepped source, NonLinElasAssFE_def.hpp:262-325). */
 int fedd_traction_load(fedd_ctx* ctx, int axis, double value, const double* traction);

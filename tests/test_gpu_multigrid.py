@@ -37,7 +37,7 @@ def test_mg_linear_solve(gpu_ctx, M):
     gpu_ctx.vector_upload(N.VEC_U, np.zeros(n))
     gpu_ctx.assembly_nonlinear_elasticity(1, EMOD, NU)
     gpu_ctx.dirichlet_set(dofs)
-    gpu_ctx.apply_dirichlet()
+    gpu_ctx.apply_dirichlet_symmetric()
     b = np.random.default_rng(M).standard_normal(n)
     b[dofs] = 0
     rp, ci = gpu_ctx.block_graph(N.BLOCK_A)
@@ -154,7 +154,7 @@ def test_amg_linear_solve(gpu_ctx, M, relabel):
     gpu_ctx.vector_upload(N.VEC_U, np.zeros(n))
     gpu_ctx.assembly_nonlinear_elasticity(1, EMOD, NU)
     gpu_ctx.dirichlet_set(dofs)
-    gpu_ctx.apply_dirichlet()
+    gpu_ctx.apply_dirichlet_symmetric()
     b = np.random.default_rng(M).standard_normal(n)
     b[dofs] = 0
     rp, ci = gpu_ctx.block_graph(N.BLOCK_A)

@@ -77,7 +77,7 @@ def test_pcg_solves_spd_system(gpu_ctx):
     gpu_ctx.vector_upload(N.VEC_U, np.zeros(3 * pts.shape[0]))
     gpu_ctx.assembly_nonlinear_elasticity(1, EMOD, NU)
     gpu_ctx.dirichlet_set(dofs)
-    gpu_ctx.apply_dirichlet()
+    gpu_ctx.apply_dirichlet_symmetric()
     rng = np.random.default_rng(0)
     b = rng.normal(size=3 * pts.shape[0])
     b[dofs] = 0
@@ -111,9 +111,9 @@ def test_newton_matches_cpu_newton(gpu_ctx):
 
 
 def test_dirichlet_rows_and_columns(gpu_ctx):
-    """fedd_apply_dirichlet: constrained rows and columns become identity rows / zero
-    columns (BCBuilder::setDirichletBC, BCBuilder_def.hpp:590-618, with the column
-    elimination that keeps the system symmetric for CG), every other entry untouched.
+    """fedd_apply_dirichlet_symmetric: constrained rows and columns become identity rows / zero
+    columns (the symmetric elimination CG needs; the reference's row-only setSystem is
+    fedd_dirichlet_set_system, tested in test_gpu_dirichlet.py), every other entry untouched.
     Two different constraint sets on one graph, then a rebuilt graph: each must use its
     own list of touched rows."""
     pts, el, _ = _problem(2)
@@ -131,7 +131,7 @@ def test_dirichlet_rows_and_columns(gpu_ctx):
         before = gpu_ctx.block_values(N.BLOCK_A)
         dofs = np.sort(rng.choice(n, size=n // (5 + trial), replace=False)).astype(np.int32)
         gpu_ctx.dirichlet_set(dofs)
-        gpu_ctx.apply_dirichlet()
+        gpu_ctx.apply_dirichlet_symmetric()
         after = gpu_ctx.block_values(N.BLOCK_A)
         flag = np.zeros(n, bool)
         flag[dofs] = True

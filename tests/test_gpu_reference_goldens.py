@@ -58,7 +58,7 @@ def test_gpu_linelas_reproduces_reference_solution(gpu_ctx, model, pc):
         gpu_ctx.assembly_nonlinear_elasticity(model, RP.LINELAS_E, RP.LINELAS_NU)
     rows = RP.node_dofs(np.where(RP.flags(pts) == 1)[0]).astype(np.int32)
     gpu_ctx.dirichlet_set(rows)
-    gpu_ctx.apply_dirichlet()
+    gpu_ctx.apply_dirichlet_symmetric()
     b = np.zeros(n)
     b[0::3] = RP.LINELAS_FORCE * bs
     b[rows] = 0.0
@@ -136,7 +136,7 @@ def test_gpu_linelas_xdim_reproduces_reference_solution(gpu_ctx, fe, pc):
     gpu_ctx.assembly_linear_elasticity_xdim(RP.LINELAS_LAMBDA, RP.LINELAS_MU)
     rows = RP.node_dofs(np.where(RP.flags(pts) == 1)[0]).astype(np.int32)
     gpu_ctx.dirichlet_set(rows)
-    gpu_ctx.apply_dirichlet()
+    gpu_ctx.apply_dirichlet_symmetric()
     b = np.zeros(n)
     b[0::3] = RP.LINELAS_FORCE * bs
     b[rows] = 0.0

@@ -29,7 +29,7 @@ def _elasticity_system(ctx, M, seed):
     base = np.nonzero(np.abs(pts[:, 2]) < 1e-12)[0]
     dofs = np.sort(np.concatenate([3 * base + d for d in range(3)])).astype(np.int32)
     ctx.dirichlet_set(dofs)
-    ctx.apply_dirichlet()
+    ctx.apply_dirichlet_symmetric()
     n = 3 * pts.shape[0]
     b = np.random.default_rng(seed).normal(size=n)
     b[dofs] = 0.0
