@@ -973,12 +973,12 @@ int fedd_assembly_navier_stokes(fedd_ctx* c, const fedd_ns_params* p, int mode, 
                                 int precision) {
   if (!c || !p) return FEDD_ERR_ARG;
   if (c->graph_kind != FEDD_GRAPH_NS) return fail(c, FEDD_ERR_STATE, "needs a Navier-Stokes graph");
-  if (mode < FEDD_NS_JACOBIAN || mode > FEDD_NS_RHS) return fail(c, FEDD_ERR_ARG, "unknown assembleMode");
+  if (mode < FEDD_NS_JACOBIAN || mode > FEDD_NS_FIXEDPOINT_RHS) return fail(c, FEDD_ERR_ARG, "unknown assembleMode");
   if (precision != FEDD_FP64 && precision != FEDD_FP32_LOCAL) return fail(c, FEDD_ERR_ARG, "bad precision");
   if (c->vec_n[FEDD_VEC_U] != 3 * c->mesh[0].npts) return fail(c, FEDD_ERR_STATE, "velocity iterate not uploaded");
   if (c->vec_n[FEDD_VEC_P] != c->mesh[1].npts) return fail(c, FEDD_ERR_STATE, "pressure iterate not uploaded");
   (void)hipSetDevice(c->device);
-  if (mode == FEDD_NS_RHS) {
+  if (mode == FEDD_NS_RHS || mode == FEDD_NS_JACOBIAN_RHS || mode == FEDD_NS_FIXEDPOINT_RHS) {
     for (int w : {FEDD_VEC_RES_U, FEDD_VEC_RES_P}) {
       int64_t n = vec_len(c, w);
       if (c->vec_n[w] != n) {

@@ -22,6 +22,7 @@
 #include "fedd_tensors.h"
 
 #include <algorithm>
+#include <type_traits>
 
 #ifndef FEDD_ROW_BLOCK
 #define FEDD_ROW_BLOCK 256
@@ -313,7 +314,11 @@ struct NsArgs {
   int with_bt;
 };
 
-template <typename T, int CLS>
+// RES = 1 (the fused Jacobian + residual modes): the pair also adds its row of the
+// residual r_(r,d) = c00 sum_c S_rc u_c,d + c01 sum_m BT_(r,d),m p_m (S = (nu rho A + rho N)
+// |det|, no W; AssembleFENavierStokes_def.hpp:200-224) into rowRes[d].  The residual stays
+// FP64: with T = float the S row is then computed in FP64 too and shared with the matrix.
+template <typename T, int CLS, int RES>
 struct VelRow {
   // Computes the canonical row of one pair and adds it into LDS.
   //   e      element, m = pair meta (loc, node-in-chunk, 10 column slots, 4 pressure slots)
@@ -321,8 +326,9 @@ struct VelRow {
   //   u      the ten canonical nodes' velocities
   static __device__ __forceinline__ void run(const NsArgs& a, int32_t e, const uint8_t* m,
                                              double* __restrict__ rowA, int deg,
-                                             double* __restrict__ rowBT, int pdeg) {
+                                             double* __restrict__ rowBT, int pdeg, double* __restrict__ rowRes) {
     using NT = NsTensors<CLS>;
+    using TS = typename std::conditional<RES != 0, double, T>::type;  // type of the S row
     const uint64_t code = c_perm_code[m[0]];
     int32_t nodes[10];
 #pragma unroll
@@ -330,38 +336,72 @@ struct VelRow {
     Geo g;
     geometry(a.xyz, a.npts, nodes, g);  // of the relabelled element
     T Bi[3][3];
+    TS BiS[3][3];
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax)
 #pragma unroll
-      for (int d = 0; d < 3; ++d) Bi[ax][d] = (T)g.Bi[ax][d];
+      for (int d = 0; d < 3; ++d) {
+        Bi[ax][d] = (T)g.Bi[ax][d];
+        BiS[ax][d] = (TS)g.Bi[ax][d];
+      }
     const double adet = g.adet;
-    T U[10][3];
+    TS U[10][3];
 #pragma unroll
     for (int k = 0; k < 10; ++k) {
       const double* uk = a.u + 3 * (int64_t)nodes[k];
-      U[k][0] = (T)uk[0];
-      U[k][1] = (T)uk[1];
-      U[k][2] = (T)uk[2];
+      U[k][0] = (TS)uk[0];
+      U[k][1] = (TS)uk[1];
+      U[k][2] = (TS)uk[2];
     }
-    // S_c = nu rho A_rc + rho N_rc   (without |det|)
-    T S[10];
+    // S_c = (nu rho A_rc + rho N_rc) |det|
+    TS S[10];
     {
-      T Gs[6];
-      metric<T>(Bi, Gs);
+      TS Gs[6];
+      metric<TS>(BiS, Gs);
 #pragma unroll
       for (int c = 0; c < 10; ++c) {
-        T acc = 0;
+        TS acc = 0;
 #pragma unroll
         for (int s = 0; s < 6; ++s)
-          if (NT::KS[s][c] != 0.0) acc += Gs[s] * (T)NT::KS[s][c];
-        S[c] = (T)a.nurho * acc;
+          if (NT::KS[s][c] != 0.0) acc += Gs[s] * (TS)NT::KS[s][c];
+        S[c] = (TS)a.nurho * acc;
       }
     }
     {
-      T Nacc[10];
-      convection_row<T, CLS>(Bi, U, Nacc);
+      TS Nacc[10];
+      convection_row<TS, CLS>(BiS, U, Nacc);
 #pragma unroll
-      for (int c = 0; c < 10; ++c) S[c] = (T)((double)(S[c] + (T)a.rho * Nacc[c]) * adet);
+      for (int c = 0; c < 10; ++c) S[c] = (TS)((double)(S[c] + (TS)a.rho * Nacc[c]) * adet);
+    }
+    if (rowBT || RES) {
+      // B^T entries of row (r, d1): -|det| sum_ax Bi[ax][d1] RV[ax][m]
+      TS pm[4] = {0, 0, 0, 0};
+      if (RES) {
+#pragma unroll
+        for (int mm = 0; mm < 4; ++mm) pm[mm] = (TS)a.p[a.pconn[(int64_t)((code >> (4 * mm)) & 15) * a.nelem + e]];
+      }
+      TS rp[3] = {0, 0, 0};
+#pragma unroll
+      for (int mm = 0; mm < 4; ++mm)
+#pragma unroll
+        for (int d1 = 0; d1 < 3; ++d1) {
+          TS v = 0;
+#pragma unroll
+          for (int ax = 0; ax < 3; ++ax)
+            if (NT::RV[ax][mm] != 0.0) v += BiS[ax][d1] * (TS)NT::RV[ax][mm];
+          if (rowBT) atomicAdd(rowBT + d1 * pdeg + m[12 + mm], -(double)(T)v * adet);
+          if (RES) rp[d1] -= v * pm[mm];
+        }
+      if (RES) {
+        // residual row, FP64: c00 sum_c S_c u_c + c01 B^T p
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          double ru = 0.0;
+#pragma unroll
+          for (int c = 0; c < 10; ++c) ru += (double)S[c] * (double)U[c][d];
+          atomicAdd(rowRes + d, a.coeff[0] * ru + a.coeff[1] * ((double)rp[d] * adet));
+        }
+      }
     }
     if (a.newton) {
       // grad u is P1: du_d1/dxhat_ax = sum_m lambda_m gh[m][ax], gh[m][ax] = sum_k u_k,d1 DV[ax][k][m]
@@ -379,7 +419,7 @@ struct VelRow {
           for (int mm = 0; mm < 4; ++mm)
 #pragma unroll
             for (int ax = 0; ax < 3; ++ax)
-              if (ft::DV[ax][k][mm] != 0.0) gh[mm][ax] += U[k][d1] * (T)ft::DV[ax][k][mm];
+              if (ft::DV[ax][k][mm] != 0.0) gh[mm][ax] += (T)U[k][d1] * (T)ft::DV[ax][k][mm];
 #pragma unroll
         for (int d2 = 0; d2 < 3; ++d2) {
           T gw[4];  // rho |det| du_d1/dx_d2 at vertex m
@@ -392,33 +432,21 @@ struct VelRow {
 #pragma unroll
             for (int mm = 0; mm < 4; ++mm)
               if (NT::L[mm][c] != 0.0) w += gw[mm] * (T)NT::L[mm][c];
-            if (d1 == d2) w += S[c];
-            atomicAdd(rowA + d1 * 3 * deg + 3 * m[2 + c] + d2, (double)w);
+            double wd = (double)w;
+            if (d1 == d2) wd = (double)((T)wd + (T)S[c]);
+            atomicAdd(rowA + d1 * 3 * deg + 3 * m[2 + c] + d2, wd);
           }
         }
       }
     } else {
 #pragma unroll
       for (int c = 0; c < 10; ++c) {
-        const double v = (double)S[c];
+        const double v = (double)(T)S[c];
         double* r = rowA + 3 * m[2 + c];
         atomicAdd(r + 0 * 3 * deg + 0, v);
         atomicAdd(r + 1 * 3 * deg + 1, v);
         atomicAdd(r + 2 * 3 * deg + 2, v);
       }
-    }
-    if (rowBT) {
-      // B^T entries of row (r, d1): -|det| sum_ax Bi[ax][d1] RV[ax][m]
-#pragma unroll
-      for (int mm = 0; mm < 4; ++mm)
-#pragma unroll
-        for (int d1 = 0; d1 < 3; ++d1) {
-          T v = 0;
-#pragma unroll
-          for (int ax = 0; ax < 3; ++ax)
-            if (NT::RV[ax][mm] != 0.0) v += Bi[ax][d1] * (T)NT::RV[ax][mm];
-          atomicAdd(rowBT + d1 * pdeg + m[12 + mm], -(double)v * adet);
-        }
     }
   }
 };
@@ -451,8 +479,11 @@ __device__ __forceinline__ void block_chunks(int64_t nchunks, int64_t& c0, int64
 // gathered.  Inside a chunk, vertex-class pairs occupy whole waves before the
 // edge-class pairs start (nv is padded to a multiple of 64), so each wave runs one
 // specialised row kernel.
-template <typename T>
-__global__ __launch_bounds__(FEDD_ROW_BLOCK) __attribute__((amdgpu_waves_per_eu(FEDD_ROW_WAVES))) void k_ns_vel_rows(NsArgs a) {
+#ifndef FEDD_ROW_WAVES_RES
+#define FEDD_ROW_WAVES_RES 3
+#endif
+template <typename T, int RES>
+__global__ __launch_bounds__(FEDD_ROW_BLOCK) __attribute__((amdgpu_waves_per_eu(RES ? FEDD_ROW_WAVES_RES : FEDD_ROW_WAVES))) void k_ns_vel_rows(NsArgs a) {
   extern __shared__ double lds[];
   __shared__ int4 tab[256];  // per chunk node: A-row offset, deg, B^T-row offset, pdeg
   int64_t c0, c1;
@@ -483,7 +514,8 @@ __global__ __launch_bounds__(FEDD_ROW_BLOCK) __attribute__((amdgpu_waves_per_eu(
       }
       tab[t] = v;
     }
-    for (int k = threadIdx.x; k < aLen + btLen; k += blockDim.x) lds[k] = 0.0;
+    const int resLen = RES ? 3 * (n1 - n0) : 0;  // residual rows after A and B^T
+    for (int k = threadIdx.x; k < aLen + btLen + resLen; k += blockDim.x) lds[k] = 0.0;
     lds_sync();
     const int nv = qs - q0, ne = q1 - qs;
     const int nvpad = (nv + 63) & ~63;
@@ -497,10 +529,11 @@ __global__ __launch_bounds__(FEDD_ROW_BLOCK) __attribute__((amdgpu_waves_per_eu(
       const int4 nd = tab[m[1]];
       double* rowA = lds + nd.x;
       double* rowBT = a.with_bt ? lds + nd.z : nullptr;
+      double* rowRes = lds + aLen + btLen + 3 * m[1];
       if (vert)
-        VelRow<T, 0>::run(a, e, m, rowA, nd.y, rowBT, nd.w);
+        VelRow<T, 0, RES>::run(a, e, m, rowA, nd.y, rowBT, nd.w, rowRes);
       else
-        VelRow<T, 1>::run(a, e, m, rowA, nd.y, rowBT, nd.w);
+        VelRow<T, 1, RES>::run(a, e, m, rowA, nd.y, rowBT, nd.w, rowRes);
     }
     lds_sync();
     double* outA = a.valsA + 9 * na0;
@@ -510,20 +543,27 @@ __global__ __launch_bounds__(FEDD_ROW_BLOCK) __attribute__((amdgpu_waves_per_eu(
       for (int k = threadIdx.x; k < btLen; k += blockDim.x)
         __builtin_nontemporal_store(lds[aLen + k], outB + k);
     }
+    if (RES) {
+      double* outR = a.res_u + 3 * (int64_t)n0;
+      for (int k = threadIdx.x; k < resLen; k += blockDim.x) outR[k] = lds[aLen + btLen + k];
+    }
     lds_sync();  // the rows are read out before the next chunk zeroes them
   }
 }
 
 // Pressure rows of B (1,0): row p of element matrix rows 30+m (already scaled by -1).
-template <typename T>
+// RES = 1: also the pressure residual r_m = c10 sum_(j,d) B_m,(j,d) u_j,d (FP64) into
+// res_p; WRITEB = 0: the residual only (reAssemble, where B is not rewritten).
+template <typename T, int RES, int WRITEB>
 __global__ __launch_bounds__(256) void k_ns_pres_rows(NsArgs a) {
   extern __shared__ double lds[];
   const int ch = (int)xcd_chunk(blockIdx.x, gridDim.x);
   const int n0 = a.chunk_node[ch], n1 = a.chunk_node[ch + 1];
   const int q0 = a.chunk_pair[ch], q1 = a.chunk_pair[ch + 1];
   const int64_t nb0 = a.bptr[n0];
-  const int len = (int)(3 * (a.bptr[n1] - nb0));
-  for (int k = threadIdx.x; k < len; k += blockDim.x) lds[k] = 0.0;
+  const int len = WRITEB ? (int)(3 * (a.bptr[n1] - nb0)) : 0;
+  const int resLen = RES ? n1 - n0 : 0;
+  for (int k = threadIdx.x; k < len + resLen; k += blockDim.x) lds[k] = 0.0;
   __syncthreads();
   for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
     const int32_t e = a.pair_elem[q];
@@ -537,18 +577,34 @@ __global__ __launch_bounds__(256) void k_ns_pres_rows(NsArgs a) {
     Geo g;
     geometry(a.xyz, a.npts, nodes, g);
     double* row = lds + 3 * (a.bptr[pn] - nb0);
+    double acc = 0.0;
 #pragma unroll
-    for (int c = 0; c < 10; ++c)
+    for (int c = 0; c < 10; ++c) {
+      const double* uc = RES ? a.u + 3 * (int64_t)a.conn[(int64_t)c_perm2[li][c] * a.nelem + e] : nullptr;
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        T v = 0;
+        if (WRITEB) {
+          T v = 0;
 #pragma unroll
-        for (int ax = 0; ax < 3; ++ax)
-          if (ft::RP[ax][c] != 0.0) v += (T)g.Bi[ax][d] * (T)ft::RP[ax][c];
-        atomicAdd(row + 3 * m[2 + c] + d, -(double)v * g.adet);
+          for (int ax = 0; ax < 3; ++ax)
+            if (ft::RP[ax][c] != 0.0) v += (T)g.Bi[ax][d] * (T)ft::RP[ax][c];
+          atomicAdd(row + 3 * m[2 + c] + d, -(double)v * g.adet);
+        }
+        if (RES) {
+          double v = 0.0;
+#pragma unroll
+          for (int ax = 0; ax < 3; ++ax)
+            if (ft::RP[ax][c] != 0.0) v += g.Bi[ax][d] * ft::RP[ax][c];
+          acc += v * uc[d];
+        }
       }
+    }
+    if (RES) atomicAdd(lds + len + m[1], -a.coeff[2] * acc * g.adet);
   }
   __syncthreads();
+  if (RES)
+    for (int k = threadIdx.x; k < resLen; k += blockDim.x) a.res_p[n0 + k] = lds[len + k];
+  if (!WRITEB) return;
   double* out = a.valsB + 3 * nb0;
   for (int k = threadIdx.x; k < len; k += blockDim.x) __builtin_nontemporal_store(lds[k], out + k);
 }
@@ -1047,39 +1103,59 @@ static NsArgs ns_args(Ctx* c, const fedd_ns_params* p, const GatherPlan& pl) {
 
 int launch_ns_assembly(Ctx* c, const fedd_ns_params* p, int mode, int reassemble, int precision) {
   const bool f32 = precision == FEDD_FP32_LOCAL;
-  if (mode == FEDD_NS_JACOBIAN || mode == FEDD_NS_FIXEDPOINT) {
+  const bool fused = mode == FEDD_NS_JACOBIAN_RHS || mode == FEDD_NS_FIXEDPOINT_RHS;
+  if (mode != FEDD_NS_RHS) {
     fedd_ns_params q = *p;
-    if (mode == FEDD_NS_FIXEDPOINT) q.newton = 0;  // the fixed-point matrix never holds W (:115-146)
+    // the fixed-point matrix never holds W (:115-146)
+    if (mode == FEDD_NS_FIXEDPOINT || mode == FEDD_NS_FIXEDPOINT_RHS) q.newton = 0;
     NsArgs a = ns_args(c, &q, c->vplan);
     a.with_bt = reassemble ? 0 : 1;
-    const size_t lds = (size_t)c->vplan.lds_doubles * 8;
+    // fused: 3 residual slots per chunk node after the rows (at most 256 nodes per chunk)
+    const size_t lds = (size_t)c->vplan.lds_doubles * 8 + (fused ? 3 * 256 * 8 : 0);
     // FEDD_ROW_BLOCKS_PER_CU > 0: that many workgroups per CU walk the chunks
     // (measured slower on MI355X than one workgroup per chunk)
     const unsigned grid = (unsigned)(FEDD_ROW_BLOCKS_PER_CU > 0 ? std::min<int64_t>(c->vplan.nchunks,
                                                                                  (int64_t)c->num_cu * FEDD_ROW_BLOCKS_PER_CU)
                                                                 : c->vplan.nchunks);
     // B does not depend on the iterate: its rows are assembled on the side stream,
-    // overlapping the velocity rows.
-    const bool fork = !reassemble;
+    // overlapping the velocity rows (with the pressure residual in the fused modes).
+    const bool fork = !reassemble || fused;
     if (fork) {
       NsArgs b = ns_args(c, &q, c->pplan);
-      const size_t ldsb = (size_t)c->pplan.lds_doubles * 8;
+      const size_t ldsb = (reassemble ? 0 : (size_t)c->pplan.lds_doubles * 8) + (fused ? 256 * 8 : 0);
+      const unsigned gb = (unsigned)c->pplan.nchunks;
       FEDD_HIP(c, hipEventRecord(c->ev_fork, c->stream));
       FEDD_HIP(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
       ktimer_begin(c, c->side);
-      if (f32)
-        k_ns_pres_rows<float><<<(unsigned)c->pplan.nchunks, 256, ldsb, c->side>>>(b);
-      else
-        k_ns_pres_rows<double><<<(unsigned)c->pplan.nchunks, 256, ldsb, c->side>>>(b);
+      if (!fused) {
+        if (f32)
+          k_ns_pres_rows<float, 0, 1><<<gb, 256, ldsb, c->side>>>(b);
+        else
+          k_ns_pres_rows<double, 0, 1><<<gb, 256, ldsb, c->side>>>(b);
+      } else if (reassemble) {
+        k_ns_pres_rows<double, 1, 0><<<gb, 256, ldsb, c->side>>>(b);
+      } else {
+        if (f32)
+          k_ns_pres_rows<float, 1, 1><<<gb, 256, ldsb, c->side>>>(b);
+        else
+          k_ns_pres_rows<double, 1, 1><<<gb, 256, ldsb, c->side>>>(b);
+      }
       FEDD_HIP(c, hipGetLastError());
       ktimer_end(c, "ns_pres_rows", c->side);
       FEDD_HIP(c, hipEventRecord(c->ev_join, c->side));
     }
     ktimer_begin(c);
-    if (f32)
-      k_ns_vel_rows<float><<<grid, FEDD_ROW_BLOCK, lds, c->stream>>>(a);
-    else
-      k_ns_vel_rows<double><<<grid, FEDD_ROW_BLOCK, lds, c->stream>>>(a);
+    if (fused) {
+      if (f32)
+        k_ns_vel_rows<float, 1><<<grid, FEDD_ROW_BLOCK, lds, c->stream>>>(a);
+      else
+        k_ns_vel_rows<double, 1><<<grid, FEDD_ROW_BLOCK, lds, c->stream>>>(a);
+    } else {
+      if (f32)
+        k_ns_vel_rows<float, 0><<<grid, FEDD_ROW_BLOCK, lds, c->stream>>>(a);
+      else
+        k_ns_vel_rows<double, 0><<<grid, FEDD_ROW_BLOCK, lds, c->stream>>>(a);
+    }
     FEDD_HIP(c, hipGetLastError());
     ktimer_end(c, "ns_vel_rows");
     if (fork) FEDD_HIP(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));

@@ -18,7 +18,7 @@ BLOCK_A, BLOCK_BT, BLOCK_B, BLOCK_C = 0, 1, 2, 3
 VEC_U, VEC_P, VEC_RES_U, VEC_RES_P, VEC_RHS, VEC_FEXT, VEC_X, VEC_B = 0, 1, 2, 3, 4, 5, 6, 7
 FP64, FP32_LOCAL = 0, 1
 BC_RHS, BC_MINUS_VECTOR, BC_VECTOR_MINUS_BC = 0, 1, 2
-NS_JACOBIAN, NS_FIXEDPOINT, NS_RHS = 0, 1, 2
+NS_JACOBIAN, NS_FIXEDPOINT, NS_RHS, NS_JACOBIAN_RHS, NS_FIXEDPOINT_RHS = 0, 1, 2, 3, 4
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)

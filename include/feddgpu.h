@@ -76,7 +76,18 @@ enum fedd_precision {
                           accumulation and every residual (NS Rhs mode, elasticity) in FP64 */
 };
 
-enum fedd_ns_mode { FEDD_NS_JACOBIAN = 0, FEDD_NS_FIXEDPOINT = 1, FEDD_NS_RHS = 2 };
+enum fedd_ns_mode {
+  FEDD_NS_JACOBIAN = 0,
+  FEDD_NS_FIXEDPOINT = 1,
+  FEDD_NS_RHS = 2,
+  /* No single counterpart in the reference: one pass that does what "Rhs" and
+   * "Jacobian" (or "FixedPoint") do on the same iterate, as the Newton cycle
+   * calculateNonLinResidualVec + assemble("Newton") needs them
+   * (NonLinearSolver_def.hpp:371-377, NavierStokesAssFE_def.hpp:225-313).  The residual
+   * is FP64 in both precision modes. */
+  FEDD_NS_JACOBIAN_RHS = 3,
+  FEDD_NS_FIXEDPOINT_RHS = 4
+};
 
 /* Parameter keys of AssembleFENavierStokes (AssembleFENavierStokes_def.hpp:44-53):
  * Parameter/Viscosity, Parameter/Density, General/Linearization ("Newton" -> 1,
@@ -182,7 +193,8 @@ int fedd_assembly_rhs_points(fedd_ctx* ctx, const double* fq, int64_t n, int nco
  * (AssembleFENavierStokes_def.hpp:67-224).  mode: FEDD_NS_*.  reassemble = 1 fills
  * only A(0,0) (addFeBlock), 0 fills A, B^T, B (and the empty C) (addFeBlockMatrix).
  * Jacobian / FixedPoint read VEC_U (and VEC_P) and overwrite the block values;
- * Rhs writes VEC_RES_U and VEC_RES_P.  Needs GRAPH_NS. */
+ * Rhs writes VEC_RES_U and VEC_RES_P; the fused modes do both in one pass.  Every
+ * mode but Rhs advances the Newton-step counter.  Needs GRAPH_NS. */
 int fedd_assembly_navier_stokes(fedd_ctx* ctx, const fedd_ns_params* params, int mode,
                                 int reassemble, int precision);
 /* FE::assemblyLinearElasticity (FE_def.hpp:362-434): the "LinearElasticity"

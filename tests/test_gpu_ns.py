@@ -85,6 +85,38 @@ def test_ns_residual(gpu_ctx, coeff):
         assert np.all(rp == 0)
 
 
+@pytest.mark.parametrize("prec", [N.FP64, N.FP32_LOCAL])
+@pytest.mark.parametrize("fused,newton", [(N.NS_JACOBIAN_RHS, True), (N.NS_FIXEDPOINT_RHS, True),
+                                          (N.NS_JACOBIAN_RHS, False)])
+@pytest.mark.parametrize("reassemble", [False, True])
+def test_ns_fused_matrix_and_residual(gpu_ctx, prec, fused, newton, reassemble):
+    """The fused modes write the matrix of Jacobian / FixedPoint and the residual of Rhs on
+    the same iterate in one pass: the matrix meets the oracle's parity rule of its precision,
+    the residual (FP64 in both precisions) meets 1e-12, with a general coeff."""
+    M = 3
+    coeff = (0.5, -2.0, 3.0, 1.0)
+    p2, e2, p1, e1, u, p = _setup_ns(gpu_ctx, M, seed=4)
+    mat_mode = 0 if fused == N.NS_JACOBIAN_RHS else 1
+    ref = O.ns_assembly(p2, e2, e1, p1.shape[0], u, p, NU, RHO, newton, mat_mode, reassemble, coeff)
+    res = O.ns_assembly(p2, e2, e1, p1.shape[0], u, p, NU, RHO, newton, 2, True, coeff)
+    gpu_ctx.assembly_navier_stokes(NU, RHO, newton, N.NS_JACOBIAN, reassemble=False)  # fill B, B^T first
+    before_bt = gpu_ctx.block_values(N.BLOCK_BT)
+    s0 = gpu_ctx.newton_step()
+    gpu_ctx.assembly_navier_stokes(NU, RHO, newton, fused, reassemble=reassemble, precision=prec, coeff=coeff)
+    assert gpu_ctx.newton_step() == s0 + 1
+    tol = dict(rtol=1e-12, atol_row=1e-14) if prec == N.FP64 else dict(rtol=1e-5, atol_row=1e-6)
+    names = [(N.BLOCK_A, "A")] + ([] if reassemble else [(N.BLOCK_BT, "BT"), (N.BLOCK_B, "B")])
+    for blk, name in names:
+        v = gpu_ctx.block_values(blk)
+        assert parity_ratio(v, ref[name][2], ref[name][0], **tol) <= 1.0, name
+    if reassemble:  # addFeBlock only: B^T keeps its values
+        assert np.array_equal(gpu_ctx.block_values(N.BLOCK_BT), before_bt)
+    ru = gpu_ctx.vector_download(N.VEC_RES_U, 3 * p2.shape[0])
+    rp = gpu_ctx.vector_download(N.VEC_RES_P, p1.shape[0])
+    assert vec_rel_err(ru, res["res_u"]) <= 1e-12
+    assert vec_rel_err(rp, res["res_p"]) <= 1e-12
+
+
 def test_ns_fp32_local(gpu_ctx):
     M = 3
     p2, e2, p1, e1, u, p = _setup_ns(gpu_ctx, M)

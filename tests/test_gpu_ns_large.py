@@ -82,19 +82,24 @@ def _res_rows(ctx, p2, e2, p1, e1, u, p, vel_nodes, pres_nodes):
     return np.max(np.abs(gu - ru)) / su, np.max(np.abs(gp - rp)) / sp_
 
 
-def test_graph_replay_small(gpu_ctx):
-    """Captured Jacobian + residual replayed with new iterates: every replay checked
-    against the oracle, with eager work and synchronisations between replays."""
+@pytest.mark.parametrize("fused", [False, True])
+def test_graph_replay_small(gpu_ctx, fused):
+    """Captured Jacobian + residual (two calls, or the fused mode) replayed with new
+    iterates: every replay checked against the oracle, with eager work and
+    synchronisations between replays."""
     p2, e2, p1, e1, u, p = W.ns_problem(3, 0)
     _upload(gpu_ctx, p2, e2, p1, e1, u, p)
     gpu_ctx.assembly_navier_stokes(NU, RHO, True, N.NS_JACOBIAN, reassemble=False)
     gpu_ctx.assembly_navier_stokes(NU, RHO, True, N.NS_RHS, reassemble=False)  # allocates the residuals
     step0 = gpu_ctx.newton_step()
     gpu_ctx.capture_begin()
-    gpu_ctx.assembly_navier_stokes(NU, RHO, True, N.NS_JACOBIAN, reassemble=False)
-    gpu_ctx.assembly_navier_stokes(NU, RHO, True, N.NS_RHS, reassemble=False)
+    if fused:
+        gpu_ctx.assembly_navier_stokes(NU, RHO, True, N.NS_JACOBIAN_RHS, reassemble=False)
+    else:
+        gpu_ctx.assembly_navier_stokes(NU, RHO, True, N.NS_JACOBIAN, reassemble=False)
+        gpu_ctx.assembly_navier_stokes(NU, RHO, True, N.NS_RHS, reassemble=False)
     nodes = gpu_ctx.capture_end()
-    assert nodes >= 4  # B rows, velocity rows, velocity and pressure residual rows
+    assert nodes >= 2  # B rows (+ pressure residual), velocity rows (+ velocity residual)
     assert gpu_ctx.newton_step() == step0 + 1
     for rep in range(3):
         ur = u * (1.0 + 0.25 * rep)
@@ -124,16 +129,15 @@ def test_graph_replay_small(gpu_ctx):
 
 
 def test_ns_large_fp32_graph_iterations(gpu_ctx):
-    """64^3x6 = 1,572,864 tets, FP32-local Jacobian + FP64 residual captured once and
-    replayed for 10 iterate updates; sampled rows vs the oracle at iterations 0 and 9."""
+    """64^3x6 = 1,572,864 tets, FP32-local Jacobian + FP64 residual in one fused pass,
+    captured once and replayed for 10 iterate updates; sampled rows vs the oracle at
+    iterations 0 and 9."""
     M = 64
     p2, e2, p1, e1, u, p = W.ns_problem(M, 0)
     _upload(gpu_ctx, p2, e2, p1, e1, u, p)
-    gpu_ctx.assembly_navier_stokes(NU, RHO, True, N.NS_JACOBIAN, reassemble=False, precision=N.FP32_LOCAL)
-    gpu_ctx.assembly_navier_stokes(NU, RHO, True, N.NS_RHS, reassemble=False, precision=N.FP32_LOCAL)
+    gpu_ctx.assembly_navier_stokes(NU, RHO, True, N.NS_JACOBIAN_RHS, reassemble=False, precision=N.FP32_LOCAL)
     gpu_ctx.capture_begin()
-    gpu_ctx.assembly_navier_stokes(NU, RHO, True, N.NS_JACOBIAN, reassemble=False, precision=N.FP32_LOCAL)
-    gpu_ctx.assembly_navier_stokes(NU, RHO, True, N.NS_RHS, reassemble=False, precision=N.FP32_LOCAL)
+    gpu_ctx.assembly_navier_stokes(NU, RHO, True, N.NS_JACOBIAN_RHS, reassemble=False, precision=N.FP32_LOCAL)
     gpu_ctx.capture_end()
     vn, pn = _sample(p2.shape[0], 200, 3), _sample(p1.shape[0], 100, 4)
     rng = np.random.default_rng(7)
