@@ -11,7 +11,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfeddgpu.so")
 
-SOURCES = ["fedd_host.hip", "fedd_ns.hip", "fedd_elas.hip", "fedd_solve.hip", "fedd_mlp.hip", "fedd_mg.hip", "fedd_hdf5.hip"]
+SOURCES = ["fedd_host.hip", "fedd_ns.hip", "fedd_elas.hip", "fedd_solve.hip", "fedd_mlp.hip", "fedd_mg.hip", "fedd_hdf5.hip", "fedd_nsolve.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

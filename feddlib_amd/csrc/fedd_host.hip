@@ -821,7 +821,9 @@ static int64_t vec_len(fedd_ctx* c, int which) {
     case FEDD_VEC_U:
     case FEDD_VEC_RES_U: return 3 * c->mesh[0].npts;
     case FEDD_VEC_P:
-    case FEDD_VEC_RES_P: return c->mesh[1].npts;
+    case FEDD_VEC_RES_P:
+    case FEDD_VEC_X_P:
+    case FEDD_VEC_B_P: return c->mesh[1].npts;
     case FEDD_VEC_RHS: return c->mesh[0].npts;
     case FEDD_VEC_FEXT:
     case FEDD_VEC_X:

@@ -15,7 +15,7 @@ FEDD_OK = 0
 MESH_PRIMARY, MESH_PRESSURE = 0, 1
 GRAPH_SCALAR, GRAPH_VECTOR, GRAPH_NS = 0, 1, 2
 BLOCK_A, BLOCK_BT, BLOCK_B, BLOCK_C = 0, 1, 2, 3
-VEC_U, VEC_P, VEC_RES_U, VEC_RES_P, VEC_RHS, VEC_FEXT, VEC_X, VEC_B = 0, 1, 2, 3, 4, 5, 6, 7
+VEC_U, VEC_P, VEC_RES_U, VEC_RES_P, VEC_RHS, VEC_FEXT, VEC_X, VEC_B, VEC_X_P, VEC_B_P = 0, 1, 2, 3, 4, 5, 6, 7, 8, 9
 FP64, FP32_LOCAL = 0, 1
 BC_RHS, BC_MINUS_VECTOR, BC_VECTOR_MINUS_BC = 0, 1, 2
 NS_JACOBIAN, NS_FIXEDPOINT, NS_RHS, NS_JACOBIAN_RHS, NS_FIXEDPOINT_RHS = 0, 1, 2, 3, 4
@@ -46,6 +46,23 @@ class NewtonStats(ctypes.Structure):
                 ("krylov_iters", ctypes.c_int64),
                 ("seconds", ctypes.c_double), ("assembly_ms", ctypes.c_double), ("solve_ms", ctypes.c_double),
                 ("final_residual", ctypes.c_double), ("converged", ctypes.c_int), ("history_len", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class NsNewtonParams(ctypes.Structure):
+    _fields_ = [("viscosity", ctypes.c_double), ("density", ctypes.c_double), ("linearization", ctypes.c_int),
+                ("criterion", ctypes.c_int), ("rel_tol", ctypes.c_double), ("max_its", ctypes.c_int),
+                ("lin_rtol", ctypes.c_double), ("restart", ctypes.c_int), ("lin_maxit", ctypes.c_int)]
+
+
+class NsNewtonStats(ctypes.Structure):
+    _fields_ = [("newton_iters", ctypes.c_int), ("krylov_iters", ctypes.c_int64),
+                ("final_residual", ctypes.c_double), ("rel_residual", ctypes.c_double),
+                ("last_update", ctypes.c_double), ("last_lin_relres", ctypes.c_double),
+                ("converged", ctypes.c_int), ("history_len", ctypes.c_int), ("seconds", ctypes.c_double),
+                ("assembly_ms", ctypes.c_double), ("solve_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -106,6 +123,9 @@ _SIGS = {
     "fedd_traction_load": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, _dp]),
     "fedd_set_preconditioner": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fedd_linear_solve": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _ip, _dp]),
+    "fedd_ns_linear_solve": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _ip, _dp]),
+    "fedd_newton_navier_stokes": (ctypes.c_int, [_vp, ctypes.POINTER(NsNewtonParams), ctypes.POINTER(NsNewtonStats),
+                                                 _dp, ctypes.c_int]),
     "fedd_linear_solve_noalloc": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, ctypes.c_double, ctypes.c_int,
                                                  _ip, _dp]),
     "fedd_newton_elasticity": (ctypes.c_int, [_vp, ctypes.POINTER(NewtonParams), ctypes.POINTER(NewtonStats),
@@ -376,6 +396,21 @@ class Context:
         it, rr = ctypes.c_int(), ctypes.c_double()
         self._chk(self._L.fedd_linear_solve(self.h, rtol, maxit, ctypes.byref(it), ctypes.byref(rr)))
         return it.value, rr.value
+
+    def ns_linear_solve(self, rtol, restart=50, maxit=2000):
+        it, rr = ctypes.c_int(), ctypes.c_double()
+        self._chk(self._L.fedd_ns_linear_solve(self.h, rtol, restart, maxit, ctypes.byref(it), ctypes.byref(rr)))
+        return it.value, rr.value
+
+    def newton_navier_stokes(self, viscosity, density, linearization=1, criterion=0, rel_tol=1e-10, max_its=20,
+                             lin_rtol=1e-10, restart=60, lin_maxit=5000, history_cap=256):
+        P = NsNewtonParams(viscosity, density, linearization, criterion, rel_tol, max_its, lin_rtol, restart,
+                           lin_maxit)
+        st = NsNewtonStats()
+        h = np.zeros(history_cap)
+        self._chk(self._L.fedd_newton_navier_stokes(self.h, ctypes.byref(P), ctypes.byref(st), _p(h, _dp),
+                                                    history_cap))
+        return st.as_dict(), h[:st.history_len].copy()
 
     def linear_solve_noalloc(self, d_b, d_x, n, rtol, maxit):
         """fedd_linear_solve_noalloc on caller-owned device vectors (integer addresses)."""

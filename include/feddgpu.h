@@ -66,9 +66,11 @@ enum fedd_vector {
   FEDD_VEC_RHS = 4,    /* load vector of FE::assemblyRHS (1 per primary node) */
   FEDD_VEC_FEXT = 5,   /* external load of the elasticity problem, 3 per node */
   FEDD_VEC_X = 6,      /* linear-solve solution / Newton increment, 3 per node */
-  FEDD_VEC_B = 7       /* linear-solve right-hand side, 3 per node */
+  FEDD_VEC_B = 7,      /* linear-solve right-hand side, 3 per node */
+  FEDD_VEC_X_P = 8,    /* pressure part of the NS linear-solve solution, 1 per pressure node */
+  FEDD_VEC_B_P = 9     /* pressure part of the NS linear-solve right-hand side */
 };
-#define FEDD_NUM_VECTORS 8
+#define FEDD_NUM_VECTORS 10
 
 enum fedd_precision {
   FEDD_FP64 = 0,       /* everything in FP64 */
@@ -322,6 +324,50 @@ typedef struct {
  * history (may be NULL) receives ||R|| at every Newton iterate. */
 int fedd_newton_elasticity(fedd_ctx* ctx, const fedd_newton_params* params, fedd_newton_stats* stats,
                            double* history, int history_cap);
+
+/* Navier-Stokes saddle-point solve on the current NS blocks (after fedd_dirichlet_set_system):
+ * [A B^T; B C] [x_u; x_p] = [VEC_B; VEC_B_P] into VEC_X, VEC_X_P, from zero, by restarted
+ * right-preconditioned GMRES(restart) (Belos GMRES behind LinearSolver_def.hpp:23-137) with
+ * the upper block-triangular preconditioner [A~ B^T; 0 -S~], A~ the 3x3 block diagonal of A,
+ * S~ = diag(B diag(A)^-1 B^T).  iters: GMRES iterations; relres: final true residual
+ * ||b - J x|| / ||b||.  restart <= 120. */
+int fedd_ns_linear_solve(fedd_ctx* ctx, double rtol, int restart, int maxit, int* iters, double* relres);
+
+typedef struct {
+  double viscosity, density;  /* Parameter/Viscosity, Parameter/Density */
+  int linearization;          /* General/Linearization: 1 "Newton", 0 "FixedPoint" (Picard) */
+  int criterion;              /* Parameter/Criterion: 0 "Residual" (||r_k|| / ||r_0|| < rel_tol),
+                                 1 "Update" (||delta_k||_2 < rel_tol) */
+  double rel_tol;             /* Parameter/relNonLinTol */
+  int max_its;                /* Parameter/MaxNonLinIts */
+  double lin_rtol;            /* GMRES relative tolerance per Newton step */
+  int restart, lin_maxit;     /* GMRES restart length (<= 120) and iteration cap per step */
+} fedd_ns_newton_params;
+
+typedef struct {
+  int newton_iters;           /* linear solves (updates) */
+  int64_t krylov_iters;
+  double final_residual;      /* ||r|| at the last iterate */
+  double rel_residual;        /* ||r|| / ||r_0|| (criterion "Residual") */
+  double last_update;         /* ||delta|| of the last update */
+  double last_lin_relres;     /* true relative residual of the last GMRES solve */
+  int converged;
+  int history_len;
+  double seconds, assembly_ms, solve_ms;
+} fedd_ns_newton_stats;
+
+/* NonLinearSolver::solveNewton (NonLinearSolver_def.hpp:356-414) for NavierStokesAssFE
+ * (NavierStokesAssFE_def.hpp:225-313) on the device: VEC_U, VEC_P are the initial iterate
+ * and the solution.  Every iteration assembles the Jacobian (FixedPoint matrix for
+ * linearization 0) and F(u) in one fused pass, forms r = rhs - F(u) (velocity rhs
+ * VEC_FEXT if uploaded with 3 entries per node, else 0; pressure rhs 0), sets
+ * r = g - u at the Dirichlet dofs (BCBuilder::setBCMinusVector, values of
+ * fedd_dirichlet_values), tests the criterion, rewrites the Dirichlet rows
+ * (BCBuilder::setSystem) and adds the GMRES solution of J delta = r.  Unlike the
+ * reference, the residual of the final iterate is evaluated (and tested) after the last
+ * update.  history (may be NULL) receives ||r|| at every iterate. */
+int fedd_newton_navier_stokes(fedd_ctx* ctx, const fedd_ns_newton_params* params, fedd_ns_newton_stats* stats,
+                              double* history, int history_cap);
 
 /* ------------------------------------------------- learned tangent surrogate
  * None of these has a counterpart in the reference (FEDDLib always evaluates the
