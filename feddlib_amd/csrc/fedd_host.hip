@@ -997,7 +997,7 @@ int fedd_assembly_navier_stokes(fedd_ctx* c, const fedd_ns_params* p, int mode, 
 }
 
 static int elasticity_common(fedd_ctx* c, int model, double E, double nu, int precision, int with_res,
-                             const double* Evec = nullptr) {
+                             const double* Evec = nullptr, int tangent = 1) {
   if (!c) return FEDD_ERR_ARG;
   if (c->graph_kind != FEDD_GRAPH_VECTOR) return fail(c, FEDD_ERR_STATE, "needs a vector (P2) graph");
   if (c->mesh[0].npe != 10) return fail(c, FEDD_ERR_STATE, "the hyperelastic elements need a P2 (tet-10) mesh");
@@ -1013,7 +1013,7 @@ static int elasticity_common(fedd_ctx* c, int model, double E, double nu, int pr
       c->vec_n[FEDD_VEC_RES_U] = n;
     }
   }
-  return launch_elasticity(c, model, E, nu, precision, with_res, 1, Evec);
+  return launch_elasticity(c, model, E, nu, precision, with_res, tangent, Evec);
 }
 
 int fedd_assembly_linear_elasticity(fedd_ctx* c, double E, double nu, int precision) {
@@ -1044,6 +1044,19 @@ int fedd_assembly_nonlinear_elasticity_efield(fedd_ctx* c, int neo_hooke_model, 
   int rc = upload(c, c->d_emod, E_elem, nelem * 8);
   if (rc) return rc;
   return elasticity_common(c, neo_hooke_model == 1 ? 1 : 2, E_elem[0], nu, precision, 1, c->d_emod.as<double>());
+}
+
+int fedd_assembly_nonlinear_elasticity_tangent(fedd_ctx* c, int neo_hooke_model, double E, double nu, int precision,
+                                                int tangent) {
+  if (!c) return FEDD_ERR_ARG;
+  if (tangent != 0 && tangent != 1) return fail(c, FEDD_ERR_ARG, "tangent must be 0 (exact) or 1 (learned)");
+  const int model = neo_hooke_model == 1 ? 1 : 2;
+  if (tangent == 1) {
+    if (!c->mlp_ready) return fail(c, FEDD_ERR_STATE, "learned tangent requested but no network");
+    if (!mlp_matches(c, model, E, nu))
+      return fail(c, FEDD_ERR_STATE, "the surrogate was built for another law or other material constants");
+  }
+  return elasticity_common(c, model, E, nu, precision, 1, nullptr, tangent == 1 ? 2 : 1);
 }
 
 int fedd_newton_step(fedd_ctx* c, int* step) {

@@ -638,6 +638,35 @@ int pcg_mg_solve(Ctx* c, double rtol, double atol, int maxit, int* iters, double
   return FEDD_OK;
 }
 
+// ||x||_2 of a device vector of length n (synchronises the context stream).
+__global__ __launch_bounds__(256) void k_sumsq(int64_t n, const double* __restrict__ x, double* __restrict__ part) {
+  __shared__ double red[256];
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const double v = i < n ? x[i] : 0.0;
+  red[threadIdx.x] = v * v;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+static int vec_norm(Ctx* c, const double* x, int64_t n, double* out) {
+  const int64_t nblk = nb(n, 256);
+  int rc;
+  if ((rc = ensure(c, c->d_part, nblk * 8))) return rc;
+  if ((rc = ensure(c, c->d_scal, PCG_SCAL * 8))) return rc;
+  k_sumsq<<<nblk, 256, 0, c->stream>>>(n, x, c->d_part.as<double>());
+  k_reduce<<<1, 1024, 0, c->stream>>>(c->d_part.as<double>(), nblk, c->d_scal.as<double>(), 7, 0);
+  FEDD_HIP(c, hipGetLastError());
+  double h = 0.0;
+  FEDD_HIP(c, hipMemcpyAsync(&h, c->d_scal.as<double>() + 7, 8, hipMemcpyDeviceToHost, c->stream));
+  FEDD_HIP(c, hipStreamSynchronize(c->stream));
+  *out = std::sqrt(h);
+  return FEDD_OK;
+}
+
 int newton_residual(Ctx* c, double lam, double* norm) {
   const int64_t n = 3 * c->mesh[0].npts;
   const int64_t nblk = nb(n, 256);
@@ -862,7 +891,7 @@ int fedd_newton_elasticity(fedd_ctx* cc, const fedd_newton_params* P, fedd_newto
   if (c->graph_kind != FEDD_GRAPH_VECTOR || c->mesh[0].npe != 10)
     return fail(c, FEDD_ERR_STATE, "needs a vector graph on a P2 (tet-10) mesh");
   if (P->load_steps < 1 || P->max_newton < 1 || !(P->abs_tol > 0) || P->preconditioner < 0 ||
-      P->preconditioner > 2)
+      P->preconditioner > 2 || P->criterion < 0 || P->criterion > 2 || (P->criterion > 0 && !(P->rel_tol > 0)))
     return fail(c, FEDD_ERR_ARG, "bad Newton parameters");
   const int model = P->neo_hooke_model == 1 ? 1 : 2;
   if ((P->tangent == 1 || P->collect_samples) && !c->mlp_ready)
@@ -903,8 +932,13 @@ int fedd_newton_elasticity(fedd_ctx* cc, const fedd_newton_params* P, fedd_newto
   for (int step = 1; step <= P->load_steps; ++step) {
     const double lam = (double)step / P->load_steps;
     bool surrogate = P->tangent == 1;
-    double prev = INFINITY, norm = 0.0;
+    double prev = INFINITY, norm = 0.0, r0 = 0.0;
     bool converged = false;
+    // converged: ||R|| < abs_tol, or criterion "Residual": ||R|| / ||R_0|| < rel_tol
+    auto done = [&](double r) {
+      st->rel_residual = r0 > 0.0 ? r / r0 : 0.0;
+      return r < P->abs_tol || (P->criterion == 1 && r < P->rel_tol * r0);
+    };
     for (int it = 0; it <= P->max_newton; ++it) {
       FEDD_HIP(c, hipEventRecord(ea, c->stream));
       if ((rc = launch_elasticity(c, model, P->E, P->nu, P->precision, 1, surrogate ? 2 : 1))) return rc;
@@ -917,7 +951,8 @@ int fedd_newton_elasticity(fedd_ctx* cc, const fedd_newton_params* P, fedd_newto
       st->assembly_ms += ms;
       if (history && hist < history_cap) history[hist++] = norm;
       final_norm = norm;
-      if (norm < P->abs_tol) {
+      if (it == 0) r0 = norm;
+      if (done(norm)) {
         converged = true;
         break;
       }
@@ -938,7 +973,7 @@ int fedd_newton_elasticity(fedd_ctx* cc, const fedd_newton_params* P, fedd_newto
         if ((rc = newton_residual(c, lam, &norm))) return rc;
         FEDD_HIP(c, hipEventElapsedTime(&ms, ea, eb));
         st->assembly_ms += ms;
-        if (norm < P->abs_tol) {
+        if (done(norm)) {
           converged = true;
           break;
         }
@@ -978,6 +1013,14 @@ int fedd_newton_elasticity(fedd_ctx* cc, const fedd_newton_params* P, fedd_newto
       FEDD_HIP(c, hipGetLastError());
       st->newton_iters++;
       prev = norm;
+      if (P->criterion == 2) {  // "Update": ||du||_2 (one synchronisation per update)
+        if ((rc = vec_norm(c, c->vec[FEDD_VEC_X].as<double>(), n, &st->last_update))) return rc;
+        if (st->last_update < P->rel_tol) {
+          converged = true;
+          final_norm = norm;
+          break;
+        }
+      }
     }
     if (!converged) all_converged = false;
   }

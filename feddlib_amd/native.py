@@ -38,14 +38,16 @@ class NewtonParams(ctypes.Structure):
                 ("load_steps", ctypes.c_int), ("abs_tol", ctypes.c_double), ("max_newton", ctypes.c_int),
                 ("lin_rtol", ctypes.c_double), ("lin_maxit", ctypes.c_int), ("tangent", ctypes.c_int),
                 ("precision", ctypes.c_int), ("stall_ratio", ctypes.c_double), ("collect_samples", ctypes.c_int),
-                ("preconditioner", ctypes.c_int), ("mg_reuse", ctypes.c_int)]
+                ("preconditioner", ctypes.c_int), ("mg_reuse", ctypes.c_int), ("criterion", ctypes.c_int),
+                ("rel_tol", ctypes.c_double)]
 
 
 class NewtonStats(ctypes.Structure):
     _fields_ = [("newton_iters", ctypes.c_int), ("fallbacks", ctypes.c_int), ("rejected", ctypes.c_int),
                 ("krylov_iters", ctypes.c_int64),
                 ("seconds", ctypes.c_double), ("assembly_ms", ctypes.c_double), ("solve_ms", ctypes.c_double),
-                ("final_residual", ctypes.c_double), ("converged", ctypes.c_int), ("history_len", ctypes.c_int)]
+                ("final_residual", ctypes.c_double), ("converged", ctypes.c_int), ("history_len", ctypes.c_int),
+                ("rel_residual", ctypes.c_double), ("last_update", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -123,6 +125,8 @@ _SIGS = {
     "fedd_traction_load": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, _dp]),
     "fedd_set_preconditioner": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fedd_linear_solve": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _ip, _dp]),
+    "fedd_assembly_nonlinear_elasticity_tangent": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                                  ctypes.c_int, ctypes.c_int]),
     "fedd_ns_linear_solve": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _ip, _dp]),
     "fedd_newton_navier_stokes": (ctypes.c_int, [_vp, ctypes.POINTER(NsNewtonParams), ctypes.POINTER(NsNewtonStats),
                                                  _dp, ctypes.c_int]),
@@ -363,6 +367,10 @@ class Context:
     def assembly_nonlinear_elasticity(self, neo_hooke_model, E, nu, precision=FP64):
         self._chk(self._L.fedd_assembly_nonlinear_elasticity(self.h, neo_hooke_model, E, nu, precision))
 
+    def assembly_nonlinear_elasticity_tangent(self, neo_hooke_model, E, nu, precision=FP64, tangent=0):
+        self._chk(self._L.fedd_assembly_nonlinear_elasticity_tangent(self.h, neo_hooke_model, E, nu, precision,
+                                                                     tangent))
+
     def assembly_nonlinear_elasticity_efield(self, neo_hooke_model, E_elem, nu, precision=FP64):
         E_elem = np.ascontiguousarray(E_elem, np.float64)
         self._chk(self._L.fedd_assembly_nonlinear_elasticity_efield(self.h, neo_hooke_model, _p(E_elem, _dp),
@@ -421,9 +429,11 @@ class Context:
 
     def newton_elasticity(self, model=1, E=1000.0, nu=0.3, load_steps=5, abs_tol=1e-8, max_newton=30,
                           lin_rtol=1e-10, lin_maxit=20000, tangent=0, precision=FP64, stall_ratio=0.5,
-                          history_cap=512, collect_samples=False, preconditioner=0, mg_reuse=False):
+                          history_cap=512, collect_samples=False, preconditioner=0, mg_reuse=False, criterion=0,
+                          rel_tol=0.0):
         P = NewtonParams(model, E, nu, load_steps, abs_tol, max_newton, lin_rtol, lin_maxit, tangent, precision,
-                         stall_ratio, 1 if collect_samples else 0, preconditioner, 1 if mg_reuse else 0)
+                         stall_ratio, 1 if collect_samples else 0, preconditioner, 1 if mg_reuse else 0, criterion,
+                         rel_tol)
         st = NewtonStats()
         hist = np.zeros(history_cap)
         self._chk(self._L.fedd_newton_elasticity(self.h, ctypes.byref(P), ctypes.byref(st), _p(hist, _dp),

@@ -162,9 +162,11 @@ int fedd_hdf5_read_vector(const char* path, const char* name, double* out, int64
 
 /* ------------------------------------------------------------- colouring */
 /* Element colouring of the primary mesh: elements of one colour share no node.
- * No counterpart in the reference (it assembles serially per rank); used by the
- * element-parallel scatter kernel.  colour_ptr has ncolours+1 entries,
- * elem_perm nelem entries (elements sorted by colour, ascending id). */
+ * No counterpart in the reference (it assembles serially per rank).  No kernel of this
+ * library uses it: the row gathers write every CRS value once without colouring.  It is
+ * exported for callers that scatter element-parallel themselves (e.g. a threaded CPU
+ * assembly).  colour_ptr has ncolours+1 entries, elem_perm nelem entries (elements
+ * sorted by colour, ascending id). */
 int fedd_colour_build(fedd_ctx* ctx, int* ncolours);
 int fedd_colour_export(fedd_ctx* ctx, int64_t* colour_ptr, int32_t* elem_perm);
 
@@ -219,6 +221,13 @@ int fedd_assembly_nonlinear_elasticity(fedd_ctx* ctx, int neo_hooke_model, doubl
  * updateParameter("E", eModVecA[T]) per element): E_elem has one entry per element. */
 int fedd_assembly_nonlinear_elasticity_efield(fedd_ctx* ctx, int neo_hooke_model, const double* E_elem,
                                               int64_t nelem, double nu, int precision);
+/* fedd_assembly_nonlinear_elasticity with a choice of tangent: tangent 0 the exact law
+ * (the same call), 1 the learned surrogate of fedd_mlp_* at every quadrature point (the
+ * residual VEC_RES_U is always the exact law's).  No counterpart in the reference, whose
+ * tangent is always AceGen's.  Needs a network built (fedd_mlp_init) for this law and
+ * these constants, else FEDD_ERR_STATE. */
+int fedd_assembly_nonlinear_elasticity_tangent(fedd_ctx* ctx, int neo_hooke_model, double E, double nu,
+                                                int precision, int tangent);
 /* ------------------------------------------------------- nonlinear solve */
 /* Dirichlet dofs of the primary field (the dofs BCBuilder::addBC flags,
  * BCBuilder_def.hpp:590-700): one entry per constrained dof, dof = node for a scalar graph,
@@ -304,6 +313,12 @@ typedef struct {
                            iterations, n0 those of the first solve after the rebuild.  Only
                            the preconditioner changes: every solve still meets its
                            tolerance for the current tangent. */
+  int criterion;        /* Parameter/Criterion of NonLinearSolver::solveNewton
+                           (NonLinearSolver_def.hpp:367-402), in addition to abs_tol:
+                           0 none; 1 "Residual": a load step also ends when
+                           ||R_k|| / ||R_0|| < rel_tol (R_0 its first residual); 2 "Update":
+                           when ||du_k||_2 < rel_tol after an update */
+  double rel_tol;       /* Parameter/relNonLinTol of criteria 1 and 2 */
 } fedd_newton_params;
 
 typedef struct {
@@ -317,6 +332,8 @@ typedef struct {
   double final_residual;
   int converged;
   int history_len;
+  double rel_residual;   /* ||R|| / ||R_0|| at the end of the last load step (logged for every criterion) */
+  double last_update;    /* ||du||_2 of the last update (criterion 2 only, else 0) */
 } fedd_newton_stats;
 
 /* NonLinearSolver::solveNewton (NonLinearSolver_def.hpp:356-414) for the hyperelastic

@@ -174,3 +174,38 @@ def test_surrogate_rejects_mismatched_law(mlp_ctx):
     with pytest.raises(N.FeddError) as e:
         mlp_ctx.newton_elasticity(model=2, E=EMOD, nu=NU, tangent=1)
     assert e.value.code == -3
+
+
+def test_standalone_learned_tangent_assembly(mlp_ctx):
+    """fedd_assembly_nonlinear_elasticity_tangent: the learned tangent assembled on its
+    own, beside the exact one, on the same graph.  The residual is the exact law's in both;
+    the trained network's matrix is close to the exact tangent, much closer than the
+    network-free CC0 tangent (zero network), and the surrogate refuses another law."""
+    ctx = mlp_ctx
+    pts, el, dofs = _problem(3)
+    ctx.mesh_upload(N.MESH_PRIMARY, pts, el)
+    ctx.graph_build(N.GRAPH_VECTOR)
+    # a smooth deformation with |grad u| <= 0.1, inside the training distribution F = I + N(0, 0.1)
+    x, y, z = pts[:, 0], pts[:, 1], pts[:, 2]
+    u = 0.1 * np.stack([z * x, -0.5 * z * y, 0.3 * x * y], axis=1).ravel()
+    ctx.vector_upload(N.VEC_U, u)
+    ctx.assembly_nonlinear_elasticity_tangent(1, EMOD, NU, N.FP64, tangent=0)
+    K_e = ctx.block_values(N.BLOCK_A)
+    r_e = ctx.vector_download(N.VEC_RES_U, u.size)
+    ctx.assembly_nonlinear_elasticity(1, EMOD, NU, N.FP64)  # the plain entry point is the exact tangent
+    assert np.max(np.abs(ctx.block_values(N.BLOCK_A) - K_e)) <= 1e-12 * np.max(np.abs(K_e))
+    ctx.mlp_add_samples_random(400000, 0.1, 2)
+    ctx.mlp_train(5000, 8192, 3e-3, 0)
+    ctx.assembly_nonlinear_elasticity_tangent(1, EMOD, NU, N.FP64, tangent=1)
+    K_m = ctx.block_values(N.BLOCK_A)
+    r_m = ctx.vector_download(N.VEC_RES_U, u.size)
+    assert np.max(np.abs(r_m - r_e)) <= 1e-12 * np.max(np.abs(r_e))
+    err_m = np.linalg.norm(K_m - K_e) / np.linalg.norm(K_e)
+    ctx.mlp_set_params(np.zeros(N.MLP_PARAMS, np.float32))
+    ctx.assembly_nonlinear_elasticity_tangent(1, EMOD, NU, N.FP64, tangent=1)
+    err_0 = np.linalg.norm(ctx.block_values(N.BLOCK_A) - K_e) / np.linalg.norm(K_e)
+    print("\nlearned tangent rel. Frobenius error %.3e, zero network %.3e" % (err_m, err_0))
+    assert err_m < 0.05 and err_m < 0.5 * err_0
+    with pytest.raises(N.FeddError) as e:
+        ctx.assembly_nonlinear_elasticity_tangent(2, EMOD, NU, N.FP64, tangent=1)
+    assert e.value.code == -3

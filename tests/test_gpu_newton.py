@@ -140,3 +140,28 @@ def test_dirichlet_rows_and_columns(gpu_ctx):
         expect = np.where(hit, (rows == ci).astype(np.float64), before)
         assert np.array_equal(after, expect), trial
     gpu_ctx.dirichlet_set(np.zeros(0, np.int32))
+
+
+@pytest.mark.parametrize("criterion", [1, 2])
+def test_newton_relative_criteria(gpu_ctx, criterion):
+    """Criterion "Residual" (||R_k|| / ||R_0|| < relNonLinTol per load step) and "Update"
+    (||du_k|| < relNonLinTol), NonLinearSolver_def.hpp:367-402: they end the load steps
+    earlier than the absolute test alone, and the reported figures obey the rule."""
+    M = 2
+    pts, el, dofs = _problem(M)
+    t = np.array([40.0, 0.0, 60.0])
+    gpu_ctx.mesh_upload(N.MESH_PRIMARY, pts, el)
+    gpu_ctx.graph_build(N.GRAPH_VECTOR)
+    gpu_ctx.dirichlet_set(dofs)
+    gpu_ctx.traction_load(2, 1.0, t)
+    gpu_ctx.vector_upload(N.VEC_U, np.zeros(3 * pts.shape[0]))
+    base, hb = gpu_ctx.newton_elasticity(model=1, E=EMOD, nu=NU, load_steps=3, abs_tol=1e-14, max_newton=12)
+    gpu_ctx.vector_upload(N.VEC_U, np.zeros(3 * pts.shape[0]))
+    st, hist = gpu_ctx.newton_elasticity(model=1, E=EMOD, nu=NU, load_steps=3, abs_tol=1e-14, max_newton=12,
+                                         criterion=criterion, rel_tol=1e-6)
+    assert st["converged"] == 1
+    assert st["newton_iters"] < base["newton_iters"]
+    if criterion == 1:
+        assert st["rel_residual"] < 1e-6
+    else:
+        assert 0 < st["last_update"] < 1e-6
