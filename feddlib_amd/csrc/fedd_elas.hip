@@ -132,6 +132,12 @@ struct ElasArgs {
   double* vals;
   double* res;
   int with_res;
+  // FP32-local: chunks whose FP32 tangent holds a non-finite entry are appended to
+  // fix_list (count in fix_list[nchunks]); the FP64 fix-up launch (fixup = 1)
+  // re-assembles exactly those chunks
+  int32_t* fix_list;
+  int64_t nchunks;
+  int fixup;
 };
 
 // One thread per (element, quadrature point), original labelling.
@@ -236,7 +242,7 @@ struct DPhiQ {
 template <typename T, int M, int MODEL>
 __device__ __forceinline__ void elas_point(const ElasArgs& a, int32_t e, const int8_t* pm, int r,
                                            const uint8_t* mm, double* __restrict__ row, int deg,
-                                           double fres[3]) {
+                                           double fres[3], T& chk) {
   const int morig = M < 4 ? pm[M] : 4;
   const double* rec = a.qp + 52 * ((int64_t)e * a.nq + morig);
   double dphi[10][3];
@@ -342,6 +348,7 @@ __device__ __forceinline__ void elas_point(const ElasArgs& a, int32_t e, const i
         T s = (i == j) ? geo : (T)0;
 #pragma unroll
         for (int J = 0; J < 6; ++J) s += BC[i][J] * Bh[J];
+        chk += s;  // a non-finite entry makes the running sum non-finite
         atomicAdd(rc + i * 3 * deg + j, (double)s);
       }
     }
@@ -350,24 +357,41 @@ __device__ __forceinline__ void elas_point(const ElasArgs& a, int32_t e, const i
 
 template <typename T, int MODEL>
 __device__ __forceinline__ void elas_pair(const ElasArgs& a, int32_t e, const uint8_t* mm,
-                                          double* __restrict__ row, int deg, double fres[3]) {
+                                          double* __restrict__ row, int deg, double fres[3], T& chk) {
   const int li = mm[0];
   const int8_t* pm = c_eperm2[li];
   const int r = li < 4 ? 0 : 4;
-  elas_point<T, 0, MODEL>(a, e, pm, r, mm, row, deg, fres);
-  elas_point<T, 1, MODEL>(a, e, pm, r, mm, row, deg, fres);
-  elas_point<T, 2, MODEL>(a, e, pm, r, mm, row, deg, fres);
-  elas_point<T, 3, MODEL>(a, e, pm, r, mm, row, deg, fres);
-  if (MODEL != 0) elas_point<T, 4, MODEL>(a, e, pm, r, mm, row, deg, fres);
+  elas_point<T, 0, MODEL>(a, e, pm, r, mm, row, deg, fres, chk);
+  elas_point<T, 1, MODEL>(a, e, pm, r, mm, row, deg, fres, chk);
+  elas_point<T, 2, MODEL>(a, e, pm, r, mm, row, deg, fres, chk);
+  elas_point<T, 3, MODEL>(a, e, pm, r, mm, row, deg, fres, chk);
+  if (MODEL != 0) elas_point<T, 4, MODEL>(a, e, pm, r, mm, row, deg, fres, chk);
 }
 
 // Row gather: pair (node i, element e, local li) adds node i's 3 tangent rows and
 // 3 residual entries of element e into LDS; the chunk is then written once.
+// fixup = 1 (FP64 only): the workgroups walk the chunks listed by an FP32-local launch
+// (none listed: every workgroup returns at once).
+template <typename T>
+__device__ __forceinline__ void elas_chunk(const ElasArgs& a, int64_t ch, double* lds, int2* tab);
+
 template <typename T>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FEDD_ELAS_WAVES))) void k_elas_rows(ElasArgs a) {
   extern __shared__ double lds[];
   __shared__ int2 tab[256];
-  const int64_t ch = xcd_chunk(blockIdx.x, gridDim.x);
+  if (a.fixup) {
+    const int32_t n = a.fix_list[a.nchunks];
+    for (int32_t k = blockIdx.x; k < n; k += gridDim.x) {
+      elas_chunk<T>(a, a.fix_list[k], lds, tab);
+      __syncthreads();  // LDS and tab are reused by the next listed chunk
+    }
+    return;
+  }
+  elas_chunk<T>(a, xcd_chunk(blockIdx.x, gridDim.x), lds, tab);
+}
+
+template <typename T>
+__device__ __forceinline__ void elas_chunk(const ElasArgs& a, int64_t ch, double* lds, int2* tab) {
   const int n0 = a.chunk_node[ch], n1 = a.chunk_node[ch + 1];
   const int q0 = a.chunk_pair[ch], q1 = a.chunk_pair[ch + 1];
   const int64_t na0 = a.nptr[n0], na1 = a.nptr[n1];
@@ -379,27 +403,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FEDD_ELAS_W
   }
   for (int k = threadIdx.x; k < aLen + rLen; k += blockDim.x) lds[k] = 0.0;
   __syncthreads();
+  int bad = 0;
   for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
     const int32_t e = a.pair_elem[q];
     const uint4 mv = a.pair_meta[q];
     const uint8_t* mm = reinterpret_cast<const uint8_t*>(&mv);
     const int2 nd = tab[mm[1]];
     double fres[3] = {0.0, 0.0, 0.0};
+    T chk = 0;
     if (a.model == 0)
-      elas_pair<T, 0>(a, e, mm, lds + nd.x, nd.y, fres);
+      elas_pair<T, 0>(a, e, mm, lds + nd.x, nd.y, fres, chk);
     else
-      elas_pair<T, 1>(a, e, mm, lds + nd.x, nd.y, fres);
+      elas_pair<T, 1>(a, e, mm, lds + nd.x, nd.y, fres, chk);
+    if (sizeof(T) == 4 && !isfinite((float)chk)) bad = 1;
     if (a.with_res) {
       double* rr = lds + aLen + 3 * mm[1];
 #pragma unroll
       for (int i = 0; i < 3; ++i) atomicAdd(rr + i, -fres[i]);
     }
   }
-  __syncthreads();
+  if (__syncthreads_or(bad) && threadIdx.x == 0 && sizeof(T) == 4)
+    a.fix_list[atomicAdd(a.fix_list + a.nchunks, 1)] = (int32_t)ch;
   double* out = a.vals + 9 * na0;
   for (int k = threadIdx.x; k < aLen; k += blockDim.x) __builtin_nontemporal_store(lds[k], out + k);
   if (a.with_res)
     for (int k = threadIdx.x; k < rLen; k += blockDim.x) a.res[3 * (int64_t)n0 + k] = lds[aLen + k];
+}
+
+__global__ void k_fill_count(int32_t* p) {
+  if (threadIdx.x == 0) *p = 0;
 }
 
 int launch_elasticity(Ctx* c, int model, double Emod, double nu, int precision, int with_res, int tangent,
@@ -444,6 +476,15 @@ int launch_elasticity(Ctx* c, int model, double Emod, double nu, int precision, 
     if (rc) return rc;
   }
   const size_t lds = (size_t)(pl.lds_doubles + 3 * 256) * 8;
+  a.nchunks = pl.nchunks;
+  if (precision == FEDD_FP32_LOCAL) {
+    if (c->d_elas_fix.bytes < (size_t)(pl.nchunks + 1) * 4) {
+      int rc = alloc(c, c->d_elas_fix, (size_t)(pl.nchunks + 1) * 4);
+      if (rc) return rc;
+    }
+    a.fix_list = c->d_elas_fix.as<int32_t>();
+    k_fill_count<<<1, 64, 0, c->stream>>>(a.fix_list + pl.nchunks);  // a kernel, not a memset (capturable)
+  }
   ktimer_begin(c);
   if (precision == FEDD_FP32_LOCAL)
     k_elas_rows<float><<<(unsigned)pl.nchunks, 256, lds, c->stream>>>(a);
@@ -451,6 +492,13 @@ int launch_elasticity(Ctx* c, int model, double Emod, double nu, int precision, 
     k_elas_rows<double><<<(unsigned)pl.nchunks, 256, lds, c->stream>>>(a);
   FEDD_HIP(c, hipGetLastError());
   ktimer_end(c, "elas_rows");
+  if (precision == FEDD_FP32_LOCAL) {
+    // FP64 re-assembly of the chunks whose FP32 tangent left FP32's range (usually none)
+    a.fixup = 1;
+    k_elas_rows<double><<<(unsigned)std::min<int64_t>(pl.nchunks, c->num_cu), 256, lds, c->stream>>>(a);
+    FEDD_HIP(c, hipGetLastError());
+  }
+  c->elas_fix_valid = precision == FEDD_FP32_LOCAL;
   return FEDD_OK;
 }
 

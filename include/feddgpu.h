@@ -228,6 +228,12 @@ int fedd_assembly_nonlinear_elasticity_efield(fedd_ctx* ctx, int neo_hooke_model
  * these constants, else FEDD_ERR_STATE. */
 int fedd_assembly_nonlinear_elasticity_tangent(fedd_ctx* ctx, int neo_hooke_model, double E, double nu,
                                                 int precision, int tangent);
+/* FP32-local elasticity: a chunk of rows whose FP32 tangent holds a non-finite entry (the
+ * law's tangent can leave FP32's range on nearly degenerate elements) is re-assembled in
+ * FP64 by the same call, on the device, so no non-finite FP32 entry is ever written where
+ * the FP64 path is finite.  This returns how many chunks the last elasticity assembly
+ * re-did in FP64 (0 for FP64 assemblies); it synchronises the context stream. */
+int fedd_elasticity_fp64_fixups(fedd_ctx* ctx, int64_t* nchunks);
 /* ------------------------------------------------------- nonlinear solve */
 /* Dirichlet dofs of the primary field (the dofs BCBuilder::addBC flags,
  * BCBuilder_def.hpp:590-700): one entry per constrained dof, dof = node for a scalar graph,
