@@ -86,7 +86,10 @@ def element_matrix(fe, xs, lam, mu):
     return K
 
 
-def linelas_xdim(fe, pts, el, lam, mu):
+def linelas_xdim(fe, pts, el, lam, mu, element=None):
+    """element="round32": every element matrix value rounded to FP32 before the FP64 sum;
+    element="scale": each element value replaced by the element matrix's largest |value|
+    (test switches, as pyoracle.element_round32 / pyoracle.element_scale)."""
     pts = np.asarray(pts, float)
     el = np.asarray(el, np.int64)
     E, npe = el.shape
@@ -103,6 +106,10 @@ def linelas_xdim(fe, pts, el, lam, mu):
          + mu * np.einsum("eqib,eqja->eqiajb", G, G)
          + lam * np.einsum("eqia,eqjb->eqiajb", G, G))
     K = np.einsum("q,eqiajb->eiajb", qw, K) * adet[:, None, None, None, None]
+    if element == "round32":
+        K = K.astype(np.float32).astype(np.float64)
+    elif element == "scale":  # K[e, i, a, j, b]: the largest |value| of each element
+        K = np.broadcast_to(np.max(np.abs(K), axis=(1, 2, 3, 4), keepdims=True), K.shape).copy()
     dof = 3 * el[:, :, None] + np.arange(3)[None, None, :]  # (E, npe, 3)
     rows = np.broadcast_to(dof[:, :, :, None, None], K.shape).ravel()
     cols = np.broadcast_to(dof[:, None, None, :, :], K.shape).ravel()

@@ -25,13 +25,31 @@ typedef std::vector<vec2_d> vec3_d;
 // columns of each row and sums duplicates (Matrix_def.hpp:192-199).  Duplicates
 // are summed in insertion order (stable sort), explicit zeros are kept.
 // ---------------------------------------------------------------------------
+// Checker switch (orc_set_element_mode): 1 rounds every element value to FP32 before it
+// is summed in FP64 (the error an FP32 element matrix causes by itself); 2 replaces every
+// value of an element block (A, B^T, B of Navier-Stokes; the whole matrix otherwise) by the
+// block's largest |value| (elem_scale), so each entry sums sum_e max |K_e| over its
+// elements, the scale of the rounding error of element entries computed in FP32.
+static int g_elem_mode = 0;
+
+// Mode 2: every entry of the nr x nc block at (r0, c0) of the row-major element matrix
+// (leading dimension ld) becomes the block's largest |value|.
+void elem_scale(double* el, int ld, int r0, int nr, int c0, int nc) {
+  if (g_elem_mode != 2) return;
+  double m = 0.0;
+  for (int i = 0; i < nr; ++i)
+    for (int j = 0; j < nc; ++j) m = std::max(m, std::fabs(el[(r0 + i) * ld + c0 + j]));
+  for (int i = 0; i < nr; ++i)
+    for (int j = 0; j < nc; ++j) el[(r0 + i) * ld + c0 + j] = m;
+}
+
 struct Crs {
   std::vector<std::vector<std::pair<int32_t, double>>> rows;
   bool filled = false;
   explicit Crs(int64_t n) : rows(n) {}
   void insert(int64_t row, const int32_t* cols, const double* vals, int n) {
     auto& r = rows[row];
-    for (int k = 0; k < n; ++k) r.emplace_back(cols[k], vals[k]);
+    for (int k = 0; k < n; ++k) r.emplace_back(cols[k], g_elem_mode == 1 ? (double)(float)vals[k] : vals[k]);
   }
   void fill_complete() {
     for (auto& r : rows) {
@@ -806,6 +824,8 @@ int orc_degree_pair(int fe1, int fe2, int t1, int t2, int extra) {
   return degree_pair(fe1, fe2, t1, t2, extra);
 }
 
+void orc_set_element_mode(int mode) { g_elem_mode = (mode == 1 || mode == 2) ? mode : 0; }
+
 void* orc_assembly_laplace(int fe, const double* pts, int64_t npts, const int32_t* elems, int64_t E) {
   int n = nloc(fe);
   Crs* A = new Crs(npts);
@@ -823,17 +843,17 @@ void* orc_assembly_laplace(int fe, const double* pts, int64_t npts, const int32_
     double absDetB = std::fabs(compute_inverse(B, Binv));
     vec3_d dPT;
     apply_btinv(dPhi, dPT, Binv);
-    for (int i = 0; i < n; ++i) {
-      double value[10] = {0};
-      int32_t idx[10];
+    double value[10][10] = {};
+    int32_t idx[10];
+    for (int i = 0; i < n; ++i)
       for (int j = 0; j < n; ++j) {
         for (size_t q = 0; q < w.size(); ++q)
-          for (int d = 0; d < 3; ++d) value[j] += w[q] * dPT[q][i][d] * dPT[q][j][d];
-        value[j] *= absDetB;
+          for (int d = 0; d < 3; ++d) value[i][j] += w[q] * dPT[q][i][d] * dPT[q][j][d];
+        value[i][j] *= absDetB;
         idx[j] = nd[j];
       }
-      A->insert(nd[i], idx, value, n);
-    }
+    elem_scale(&value[0][0], 10, 0, n, 0, n);
+    for (int i = 0; i < n; ++i) A->insert(nd[i], idx, value[i], n);
   }
   A->fill_complete();
   return A;
@@ -868,6 +888,7 @@ void* orc_assembly_laplace_assfe(int fe, int dofs, const double* pts, int64_t np
         v *= absDetB;
         for (int d = 0; d < dofs; ++d) el[(i * dofs + d) * ld + j * dofs + d] = v;
       }
+    elem_scale(el.data(), ld, 0, ld, 0, ld);
     add_fe_block(*A, el.data(), ld, nd, n, dofs);  // FE_def.hpp:1846
   }
   A->fill_complete();
@@ -991,6 +1012,9 @@ void orc_assembly_ns(const double* pts2, int64_t np2, const int32_t* el2, const 
         ns_element(xl, sol, nu, rho, newton, coeff, el.data(), nullptr, nullptr);
       else
         ns_element(xl, sol, nu, rho, newton, coeff, nullptr, el.data(), nullptr);
+      elem_scale(el.data(), NSD, 0, 30, 0, 30);
+      elem_scale(el.data(), NSD, 0, 30, 30, 4);
+      elem_scale(el.data(), NSD, 30, 4, 0, 30);
       if (reassemble)
         add_fe_block(*a, el.data(), NSD, n2, 10, 3);
       else
@@ -1111,6 +1135,7 @@ void* orc_assembly_elasticity(int model, double Emod, double nu, const double* p
     orc_nh_skr(model, Emod, nu, xl, ul, s, pv);
     for (int i = 0; i < 30; ++i)
       for (int j = 0; j < 30; ++j) el[i * 30 + j] = -s[30 * j + i];  // :119
+    elem_scale(el, 30, 0, 30, 0, 30);
     add_fe_block(*A, el, 30, nd, 10, 3);
     if (res && model != 0)
       for (int i = 0; i < 10; ++i)

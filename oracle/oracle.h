@@ -60,6 +60,12 @@ int orc_degree_pair(int fe1, int fe2, int type1, int type2, int extra);
 
 /* ---- Laplace ---- */
 /* FE::assemblyLaplace (FE_def.hpp:2437-2500): classic path, rule from (Grad,Grad). */
+/* Test switch for matrix assemblies: 0 plain; 1 every element matrix value rounded to FP32
+   before it is summed in FP64 (the floor of an FP32 element-matrix path); 2 every value of
+   an element block (Navier-Stokes A, B^T, B; the whole element matrix otherwise) replaced
+   by the block's largest |value|, so that each entry holds sum_e max |K_e|. */
+void orc_set_element_mode(int mode);
+
 void* orc_assembly_laplace(int fe, const double* pts, int64_t npts, const int32_t* elems, int64_t E);
 /* FE::assemblyLaplaceAssFE via AssembleFE_Laplace (AssembleFE_Laplace_def.hpp:34-96),
  * dofs = 1 (scalar) or 3 (vector field), element rule from the "UN Grad=2" overload. */

@@ -62,6 +62,7 @@ def lib():
                                                           ctypes.c_int64, _dp, _dp, ctypes.c_double,
                                                           ctypes.c_double, ctypes.c_int]),
             "orc_nh_skr": (None, [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _dp]),
+            "orc_set_element_mode": (None, [ctypes.c_int]),
             "orc_assembly_elasticity": (_vp, [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp,
                                               ctypes.c_int64, _ip, ctypes.c_int64, _dp, _dp]),
         }
@@ -86,6 +87,32 @@ def ref_lib():
 
 def P(a, t=_dp):
     return a.ctypes.data_as(t)
+
+
+class _element_mode:
+    mode = 0
+
+    def __enter__(self):
+        lib().orc_set_element_mode(self.mode)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_element_mode(0)
+        return False
+
+
+class element_round32(_element_mode):
+    """Context: every element matrix value is rounded to FP32 before it is summed in FP64
+    (matrix assemblies only; the oracle's residual vectors stay FP64)."""
+    mode = 1
+
+
+class element_scale(_element_mode):
+    """Context: matrix assemblies sum, for each entry, the largest |value| of each
+    contributing element block (Navier-Stokes A, B^T, B; the whole element matrix
+    otherwise): sum_e max |K_e|, the scale of the rounding error of element entries
+    computed in FP32."""
+    mode = 2
 
 
 def mesh(M, fe):

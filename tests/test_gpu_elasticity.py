@@ -11,7 +11,7 @@ import pytest
 import pyoracle as O
 from feddlib_amd import native as N
 from feddlib_amd import workloads as W
-from helpers import assert_graph_equal, parity_ratio, rel_err_rowfloor, vec_rel_err
+from helpers import assert_graph_equal, fp32_local_check, parity_ratio, rel_err_rowfloor, vec_rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -70,8 +70,9 @@ def test_elasticity_fp32_local(gpu_ctx, model):
     ref, rref = O.elasticity(model, EMOD, NU, pts, el, u)
     gpu_ctx.assembly_nonlinear_elasticity(1 if model == 1 else 2, EMOD, NU, N.FP32_LOCAL)
     v = gpu_ctx.block_values(N.BLOCK_A)
-    ratio = parity_ratio(v, ref[2], ref[0], rtol=1e-5, atol_row=1e-6)
-    assert ratio <= 1.0, (ratio, rel_err_rowfloor(v, ref[2], ref[0]))
+    with O.element_scale():
+        scale, _ = O.elasticity(model, EMOD, NU, pts, el, u)
+    fp32_local_check(v, ref, scale)
     # the residual stays FP64 in the FP32-local path
     r = gpu_ctx.vector_download(N.VEC_RES_U, 3 * pts.shape[0])
     assert vec_rel_err(r, rref) <= 1e-12
@@ -200,14 +201,19 @@ def test_elasticity_curved_shuffled_mesh(gpu_ctx, model):
     gpu_ctx.vector_upload(N.VEC_U, u)
     ref, rref = O.elasticity(model, EMOD, NU, pts, el, u)
     assert_graph_equal(gpu_ctx.block_graph(N.BLOCK_A), ref[:2])
-    for prec, rtol, atol_row in ((N.FP64, 1e-12, 1e-14), (N.FP32_LOCAL, 1e-5, 1e-6)):
+    with O.element_scale():
+        scale, _ = O.elasticity(model, EMOD, NU, pts, el, u)
+    for prec in (N.FP64, N.FP32_LOCAL):
         if model == 0:
             gpu_ctx.assembly_linear_elasticity(EMOD, NU, prec)
         else:
             gpu_ctx.assembly_nonlinear_elasticity(model, EMOD, NU, prec)
         v = gpu_ctx.block_values(N.BLOCK_A)
-        ratio = parity_ratio(v, ref[2], ref[0], rtol=rtol, atol_row=atol_row)
-        assert ratio <= 1.0, (prec, ratio, rel_err_rowfloor(v, ref[2], ref[0]))
+        if prec == N.FP64:
+            ratio = parity_ratio(v, ref[2], ref[0])
+            assert ratio <= 1.0, (ratio, rel_err_rowfloor(v, ref[2], ref[0]))
+        else:
+            fp32_local_check(v, ref, scale)
         if model != 0:
             r = gpu_ctx.vector_download(N.VEC_RES_U, 3 * pts.shape[0])
             assert vec_rel_err(r, rref) <= 1e-12

@@ -7,8 +7,9 @@ AceGen-pinned element) assembles only those elements.
 
 Two iterates:
 * u scaled by 2/M, so that grad u stays as small as in the small-mesh cases
-  (max ~0.05): FP64 rows satisfy the 1e-12 rule, FP32-local rows the 1e-5 rule, the
-  residual (always FP64) 1e-12;
+  (max ~0.05): FP64 rows satisfy the 1e-12 rule, FP32-local rows helpers.fp32_local_check
+  (1e-6 per matrix, per entry the 1e-5 rule plus the FP32 summation bound), the residual
+  (always FP64) 1e-12;
 * the configured iterate, 0.05 N(0,1) per dof and one smoothing pass, not scaled with
   h: at this size grad u reaches 3 and 1.4% of the elements invert.  Single elements
   (the most strained ones) still agree with the oracle to 1e-9 of their largest entry
@@ -16,7 +17,10 @@ Two iterates:
   at such strains).  Assembled rows sum large contributions of strongly deformed
   neighbours that cancel, which amplifies that level relative to the row: rows sampled
   among nodes whose incident elements keep 0.5 <= det F <= 2 (vertex-gradient
-  estimate) are bounded by |a - b| <= 1e-5 max_row |b| (measured up to 3.4e-6).
+  estimate) are bounded by |a - b| <= 1e-5 max_row |b| (measured up to 3.4e-6).  The
+  FP32-local rows of the same iterate are reported as quantiles of that figure: the
+  element tangents there cancel by orders of magnitude (B^T CC B with CC from the FP64
+  law), which FP32 products cannot resolve; the median row is bounded by 5e-4.
 """
 import numpy as np
 import pytest
@@ -24,7 +28,7 @@ import pytest
 import pyoracle as O
 from feddlib_amd import native as N
 from feddlib_amd import workloads as W
-from helpers import incident_elements, subset_rows_parity
+from helpers import fp32_local_check, incident_elements, subset_rows_parity
 
 pytestmark = pytest.mark.gpu
 
@@ -49,15 +53,16 @@ def big_problem():
     return pts, el, u, nodes, np.ascontiguousarray(el[S])
 
 
-def _row_abs_err(got, ref, rows):
+def _row_errs(got, ref, rows):
+    """max |a - b| / max_row |b| of each of `rows`."""
     gr, gc, gv = got
     rr, rc, rv = ref
-    worst = 0.0
-    for r in rows:
+    out = np.zeros(len(rows))
+    for k, r in enumerate(rows):
         a, b = slice(gr[r], gr[r + 1]), slice(rr[r], rr[r + 1])
         assert np.array_equal(gc[a], rc[b])
-        worst = max(worst, float(np.max(np.abs(gv[a] - rv[b])) / np.max(np.abs(rv[b]))))
-    return worst
+        out[k] = np.max(np.abs(gv[a] - rv[b])) / np.max(np.abs(rv[b]))
+    return out
 
 
 def _assemble(ctx, model, prec):
@@ -79,21 +84,39 @@ def test_elasticity_full_size_rows(gpu_ctx, big_problem, model):
     us = u * (2.0 / M)
     gpu_ctx.vector_upload(N.VEC_U, us)
     ref, rres = O.elasticity(model, EMOD, NU, pts, sub, us)
-    for prec, rtol, atol in ((N.FP64, 1e-12, 1e-14), (N.FP32_LOCAL, 1e-5, 1e-6)):
+    with O.element_scale():
+        scale, _ = O.elasticity(model, EMOD, NU, pts, sub, us)
+    for prec in (N.FP64, N.FP32_LOCAL):
         _assemble(gpu_ctx, model, prec)
         v = gpu_ctx.block_values(N.BLOCK_A)
-        worst = subset_rows_parity((rp, ci, v), ref, rows, rtol, atol)
-        assert worst <= 1.0, (model, prec, worst)
+        if prec == N.FP64:
+            worst = subset_rows_parity((rp, ci, v), ref, rows)
+            assert worst <= 1.0, (model, worst)
+        else:
+            got = np.zeros(len(ref[2]))  # the GPU's sampled rows on the oracle's pattern
+            for r in rows:
+                assert np.array_equal(ci[rp[r]:rp[r + 1]], ref[1][ref[0][r]:ref[0][r + 1]])
+                got[ref[0][r]:ref[0][r + 1]] = v[rp[r]:rp[r + 1]]
+            fp32_local_check(got, ref, scale, rows=rows)
         if model != 0:
             r = gpu_ctx.vector_download(N.VEC_RES_U, 3 * pts.shape[0])[rows]
             assert np.max(np.abs(r - rres[rows])) <= 1e-12 * np.max(np.abs(rres[rows])), (model, prec)
     # the configured iterate (large strains)
     gpu_ctx.vector_upload(N.VEC_U, u)
     ref, rres = O.elasticity(model, EMOD, NU, pts, sub, u)
-    _assemble(gpu_ctx, model, N.FP64)
-    err = _row_abs_err((rp, ci, gpu_ctx.block_values(N.BLOCK_A)), ref, rows)
-    print("model %d, configured iterate: max |a-b| / max_row |b| = %.2e" % (model, err))
-    assert err <= 1e-5, (model, err)
+    errs = {}
+    for prec in (N.FP64, N.FP32_LOCAL):
+        _assemble(gpu_ctx, model, prec)
+        v = gpu_ctx.block_values(N.BLOCK_A)
+        assert np.all(np.isfinite(v))
+        errs[prec] = _row_errs((rp, ci, v), ref, rows)
+    q = (50, 90, 99, 100)
+    for prec, name in ((N.FP64, "FP64"), (N.FP32_LOCAL, "FP32-local")):
+        print("model %d, configured iterate, %s: max |a-b| / max_row |b| per row, quantiles %s: %s" % (
+            model, name, q, " ".join("%.1e" % x for x in np.percentile(errs[prec], q))))
+    assert np.max(errs[N.FP64]) <= 1e-5, model
+    # FP32-local rows: median measured 7.4e-5..8.0e-5 of the row (90th percentile 1e-2)
+    assert np.percentile(errs[N.FP32_LOCAL], 50) <= 5e-4, model
 
 
 @pytest.mark.parametrize("model", [0, 1, 2])

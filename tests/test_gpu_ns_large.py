@@ -14,7 +14,8 @@ import pytest
 import pyoracle as O
 from feddlib_amd import native as N
 from feddlib_amd import workloads as W
-from helpers import assert_graph_equal, incident_elements, parity_ratio, subset_rows_parity
+from helpers import (assert_graph_equal, fp32_local_check, incident_elements, parity_ratio,
+                     subset_rows_parity)
 
 pytestmark = pytest.mark.gpu
 
@@ -29,23 +30,31 @@ def _upload(ctx, p2, e2, p1, e1, u, p):
     ctx.vector_upload(N.VEC_P, p)
 
 
-def _check_rows(ctx, p2, e2, p1, e1, u, p, vel_nodes, pres_nodes, rtol, atol_row, mode=N.NS_JACOBIAN):
-    """Oracle on the elements touching the sampled nodes; compare their rows."""
+def _check_rows(ctx, p2, e2, p1, e1, u, p, vel_nodes, pres_nodes, fp32=False):
+    """Oracle on the elements touching the sampled nodes; compare their rows: the FP64
+    parity rule, or the FP32-local rule (helpers.fp32_local_check).  Returns the worst
+    parity ratio (FP64) or the (row-floor error, per-matrix error, C needed) per block."""
     S = incident_elements(e2, vel_nodes)
-    ref = O.ns_assembly(p2, np.ascontiguousarray(e2[S]), np.ascontiguousarray(e1[S]), p1.shape[0], u, p, NU, RHO,
-                        True, 0, False)
+    Sp = incident_elements(e1, pres_nodes)
     rows = np.sort((3 * vel_nodes[:, None] + np.arange(3)).ravel())
     worst = {}
-    for blk, name in ((N.BLOCK_A, "A"), (N.BLOCK_BT, "BT")):
+    for blk, name, sel, rr in ((N.BLOCK_A, "A", S, rows), (N.BLOCK_BT, "BT", S, rows),
+                               (N.BLOCK_B, "B", Sp, np.sort(pres_nodes))):
+        sub = (p2, np.ascontiguousarray(e2[sel]), np.ascontiguousarray(e1[sel]), p1.shape[0], u, p, NU, RHO, True, 0,
+               False)
+        ref = O.ns_assembly(*sub)[name]
         rp, ci = ctx.block_graph(blk)
         v = ctx.block_values(blk)
-        worst[name] = subset_rows_parity((rp, ci, v), ref[name], rows, rtol, atol_row)
-    Sp = incident_elements(e1, pres_nodes)
-    refp = O.ns_assembly(p2, np.ascontiguousarray(e2[Sp]), np.ascontiguousarray(e1[Sp]), p1.shape[0], u, p, NU,
-                         RHO, True, 0, False)
-    rp, ci = ctx.block_graph(N.BLOCK_B)
-    worst["B"] = subset_rows_parity((rp, ci, ctx.block_values(N.BLOCK_B)), refp["B"], np.sort(pres_nodes), rtol,
-                                    atol_row)
+        if not fp32:
+            worst[name] = subset_rows_parity((rp, ci, v), ref, rr)
+            continue
+        got = np.zeros(len(ref[2]))  # the GPU's values of the sampled rows on the oracle's pattern
+        for r in rr:
+            assert np.array_equal(ci[rp[r]:rp[r + 1]], ref[1][ref[0][r]:ref[0][r + 1]]), "row %d: columns differ" % r
+            got[ref[0][r]:ref[0][r + 1]] = v[rp[r]:rp[r + 1]]
+        with O.element_scale():
+            scale = O.ns_assembly(*sub)[name]
+        worst[name] = fp32_local_check(got, ref, scale, rows=rr)
     return worst
 
 
@@ -59,12 +68,24 @@ def test_ns_jacobian_full_size_rows(gpu_ctx):
     _upload(gpu_ctx, p2, e2, p1, e1, u, p)
     gpu_ctx.assembly_navier_stokes(NU, RHO, True, N.NS_JACOBIAN, reassemble=False)
     vn, pn = _sample(p2.shape[0], 400, 1), _sample(p1.shape[0], 200, 2)
-    worst = _check_rows(gpu_ctx, p2, e2, p1, e1, u, p, vn, pn, 1e-12, 1e-14)
+    worst = _check_rows(gpu_ctx, p2, e2, p1, e1, u, p, vn, pn)
     assert max(worst.values()) <= 1.0, worst
     # sizes are the closed-form polynomials
     M = 40
     assert gpu_ctx.block_info(N.BLOCK_A)[2] == 9 * (230 * M ** 3 + 138 * M ** 2 + 24 * M + 1)
     assert gpu_ctx.block_info(N.BLOCK_BT)[2] == 3 * (65 * M ** 3 + 57 * M ** 2 + 15 * M + 1)
+
+
+def test_ns_jacobian_full_size_rows_fp32_local(gpu_ctx):
+    """40^3x6, FP32-local: 400 velocity and 200 pressure rows vs the FP64 oracle, per
+    matrix within 1e-6 and per entry within the FP32-local rule of helpers.fp32_local_check."""
+    p2, e2, p1, e1, u, p = W.ns_problem(40, 0)
+    _upload(gpu_ctx, p2, e2, p1, e1, u, p)
+    gpu_ctx.assembly_navier_stokes(NU, RHO, True, N.NS_JACOBIAN, reassemble=False, precision=N.FP32_LOCAL)
+    vn, pn = _sample(p2.shape[0], 400, 1), _sample(p1.shape[0], 200, 2)
+    worst = _check_rows(gpu_ctx, p2, e2, p1, e1, u, p, vn, pn, fp32=True)
+    for name, (rf, pm, ceff) in worst.items():
+        print("\n40^3 FP32-local %s: row-floor %.2e, per-matrix %.2e, C needed %.2f" % (name, rf, pm, ceff))
 
 
 def _res_rows(ctx, p2, e2, p1, e1, u, p, vel_nodes, pres_nodes):
@@ -149,7 +170,6 @@ def test_ns_large_fp32_graph_iterations(gpu_ctx):
         gpu_ctx.graph_launch()
         gpu_ctx.synchronize()
         if it in (0, 9):
-            worst = _check_rows(gpu_ctx, p2, e2, p1, e1, ui, pi, vn, pn, 1e-5, 1e-6)
-            assert max(worst.values()) <= 1.0, (it, worst)
+            _check_rows(gpu_ctx, p2, e2, p1, e1, ui, pi, vn, pn, fp32=True)
             eu, ep = _res_rows(gpu_ctx, p2, e2, p1, e1, ui, pi, vn, pn)
             assert eu <= 1e-12 and ep <= 1e-12, (it, eu, ep)  # the residual stays FP64

@@ -103,9 +103,9 @@ def test_gpu_stokes_reproduces_reference_solution(gpu_ctx, prec):
 @pytest.mark.parametrize("prec", [N.FP64, N.FP32_LOCAL])
 def test_gpu_linelas_xdim_matches_oracle(gpu_ctx, fe, prec):
     """fedd_assembly_linear_elasticity_xdim against the oracle's FE::assemblyLinElasXDim,
-    entry by entry on the vector graph (FP64 1e-12 / FP32-local 1e-5 parity rule)."""
+    entry by entry on the vector graph (FP64 1e-12 parity rule / helpers.fp32_local_check)."""
     import linelas as LE
-    from helpers import parity_ratio
+    from helpers import fp32_local_check, parity_ratio
     pts, _, el = N.mesh_structured(3, fe)
     n = 3 * len(pts)
     gpu_ctx.mesh_upload(N.MESH_PRIMARY, pts, el)
@@ -117,8 +117,12 @@ def test_gpu_linelas_xdim_matches_oracle(gpu_ctx, fe, prec):
     rows = np.repeat(np.arange(n), np.diff(rp))
     ref = np.asarray(K[rows, ci]).ravel()
     assert K.nnz <= len(ci)  # no oracle entry outside the GPU graph
-    tol = dict(rtol=1e-12, atol_row=1e-14) if prec == N.FP64 else dict(rtol=1e-5, atol_row=1e-6)
-    assert parity_ratio(v, ref, rp, **tol) <= 1.0
+    if prec == N.FP64:
+        assert parity_ratio(v, ref, rp) <= 1.0
+    else:
+        Kscale = LE.linelas_xdim(fe, pts, el, RP.LINELAS_LAMBDA, RP.LINELAS_MU, element="scale")
+        scale = np.asarray(Kscale[rows, ci]).ravel()
+        fp32_local_check(v, (rp, ci, ref), scale)
 
 
 @pytest.mark.parametrize("fe,pc", [(1, 0), (2, 1)])
