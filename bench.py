@@ -335,7 +335,10 @@ def elasticity_leg(ctx, N, W, M, reps=10):
                      "fp64_frac": 5e4 * E / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
         vals[name] = (ctx.block_values(N.BLOCK_A), ctx.vector_download(N.VEC_RES_U, 3 * n))
         if name == "fp32_local":
-            res[name]["chunks_reassembled_in_fp64"] = ctx.elasticity_fp64_fixups()
+            chunks, pairs = ctx.elasticity_fp64_counts()
+            res[name]["chunks_reassembled_in_fp64"] = chunks
+            res[name]["pairs_of_flagged_elements_in_fp64"] = pairs
+            res[name]["pairs_total"] = 10 * E
     rp, _ = ctx.block_graph(N.BLOCK_A)
     v64, r64 = vals["fp64"]
     v32, r32 = vals["fp32_local"]
@@ -345,8 +348,9 @@ def elasticity_leg(ctx, N, W, M, reps=10):
         residual=float(np.max(np.abs(r32 - r64)) / np.max(np.abs(r64))))
     res["fp32_local"]["errors_note"] = ("A against the FP64 assembly of the same iterate (row_floor, per_matrix, "
                                         "rowwise as for the NS blocks); the residual is FP64 in both paths.  At "
-                                        "this iterate grad u reaches 3 and 1.4% of the elements invert (det F <= 0), "
-                                        "so single element matrices are ill-conditioned")
+                                        "this iterate grad u reaches 3 and many quadrature points have det F <= 0; "
+                                        "elements with a point of kappa(F^T F) > 1e4 are computed in FP64 "
+                                        "(pairs_of_flagged_elements_in_fp64), which the FP32-local time includes")
     del vals, v64, v32
     print("# elasticity %d tets: %.3f ms fp64, %.3f ms fp32-local" % (E, res["fp64"]["ms"], res["fp32_local"]["ms"]),
           file=sys.stderr, flush=True)

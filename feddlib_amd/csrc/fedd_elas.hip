@@ -16,9 +16,18 @@
 // F (9), S (6 Voigt), the material tangent CC = dS/dE as a symmetric 6x6 Voigt
 // matrix (21), w_q det J_F (1), and the inverse isoparametric Jacobian in
 // barycentric-row form (12: rows dxi_a/dX for a = 1..3 and minus their sum), so
-// that a relabelled element's inverse Jacobian is a selection of three rows.  The tangent slot is where the learned surrogate writes
-// instead of the exact law.  The row gather then builds each node's 3 rows of
-// the tangent and its 3 residual entries in LDS and writes them once.
+// that a relabelled element's inverse Jacobian is a selection of three rows.  The tangent
+// slot is where the learned surrogate writes instead of the exact law.  The row gather
+// then builds each node's 3 rows of the tangent and its 3 residual entries in LDS and
+// writes them once.
+//
+// FP32-local guard: the FP32 error of B^T CC B at a point grows with the conditioning of
+// C = F^T F (measured on strained elements: about 10 u32 kappa of the element's largest
+// entry, kappa = |C|_F |adj C|_F / det C >= cond(C)); points with kappa > 1e4 (or J = 0, or
+// a non-finite F) flag their element (elem_flag, FP32-local launches only); the FP32 row
+// gather leaves every pair of a flagged element
+// out and lists its chunk, and an FP64 pass over the listed chunks accumulates exactly
+// those pairs in LDS and adds them to the rows written.
 #include "fedd_internal.h"
 #include "fedd_law.h"
 #include "fedd_perm.h"
@@ -28,6 +37,9 @@
 #ifndef FEDD_ELAS_WAVES
 #define FEDD_ELAS_WAVES 2
 #endif
+
+// kappa bound above which a point's FP32 tangent products are not trusted
+#define FEDD_FP32_KAPPA_MAX 1.0e4
 
 namespace fedd {
 
@@ -134,11 +146,32 @@ struct ElasArgs {
   int with_res;
   // FP32-local: chunks whose FP32 tangent holds a non-finite entry are appended to
   // fix_list (count in fix_list[nchunks]); the FP64 fix-up launch (fixup = 1)
-  // re-assembles exactly those chunks
+  // re-assembles exactly those chunks.  Chunks holding pairs of flagged elements are
+  // appended to flag_list (count in flag_list[nchunks], pairs added in
+  // flag_list[nchunks + 1]); the FP64 launch with fixup = 2 adds those pairs.
   int32_t* fix_list;
+  int32_t* flag_list;
+  int32_t* elem_flag;  // FP32-local: nonzero for elements with a flagged point (else null)
   int64_t nchunks;
   int fixup;
 };
+
+// kappa = |C|_F |adj C|_F / det C for C = F^T F (>= the condition number of C).
+__device__ inline double elas_kappa(const double F[3][3]) {
+  double C[3][3];
+#pragma unroll
+  for (int A = 0; A < 3; ++A)
+#pragma unroll
+    for (int B = 0; B < 3; ++B) C[A][B] = F[0][A] * F[0][B] + F[1][A] * F[1][B] + F[2][A] * F[2][B];
+  const double a00 = C[1][1] * C[2][2] - C[1][2] * C[1][2], a01 = C[0][2] * C[1][2] - C[0][1] * C[2][2],
+               a02 = C[0][1] * C[1][2] - C[0][2] * C[1][1], a11 = C[0][0] * C[2][2] - C[0][2] * C[0][2],
+               a12 = C[0][1] * C[0][2] - C[0][0] * C[1][2], a22 = C[0][0] * C[1][1] - C[0][1] * C[0][1];
+  const double det = C[0][0] * a00 + C[0][1] * a01 + C[0][2] * a02;
+  const double nc = C[0][0] * C[0][0] + C[1][1] * C[1][1] + C[2][2] * C[2][2] +
+                    2.0 * (C[0][1] * C[0][1] + C[0][2] * C[0][2] + C[1][2] * C[1][2]);
+  const double na = a00 * a00 + a11 * a11 + a22 * a22 + 2.0 * (a01 * a01 + a02 * a02 + a12 * a12);
+  return sqrt(nc * na) / det;
+}
 
 // One thread per (element, quadrature point), original labelling.
 __global__ __launch_bounds__(64) void k_elas_qp(ElasArgs a) {
@@ -201,6 +234,8 @@ __global__ __launch_bounds__(64) void k_elas_qp(ElasArgs a) {
     rec[46 + A] = Ji[2][A];
   }
   rec[49] = rec[50] = rec[51] = 0.0;
+  if (a.elem_flag && live && !(elas_kappa(F) <= FEDD_FP32_KAPPA_MAX))  // also J = 0, non-finite F
+    atomicOr(a.elem_flag + e, 1);
   const int64_t nout = (nrec - t0 < 64 ? nrec - t0 : 64) * 52;  // doubles this wave writes
   double* out = a.qp + 52 * t0;
 #pragma unroll
@@ -370,28 +405,53 @@ __device__ __forceinline__ void elas_pair(const ElasArgs& a, int32_t e, const ui
 
 // Row gather: pair (node i, element e, local li) adds node i's 3 tangent rows and
 // 3 residual entries of element e into LDS; the chunk is then written once.
-// fixup = 1 (FP64 only): the workgroups walk the chunks listed by an FP32-local launch
-// (none listed: every workgroup returns at once).
+// FP32-local follow-up launches (FP64): fixup = 1 re-assembles the chunks listed with a
+// non-finite FP32 entry, fixup = 2 adds the pairs of flagged elements to the chunks listed
+// with such pairs; the workgroups walk the list (none listed: all return at once).
 template <typename T>
-__device__ __forceinline__ void elas_chunk(const ElasArgs& a, int64_t ch, double* lds, int2* tab);
+__device__ __forceinline__ void elas_chunk(const ElasArgs& a, int64_t ch, double* lds, int2* tab, int* queue);
 
 template <typename T>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FEDD_ELAS_WAVES))) void k_elas_rows(ElasArgs a) {
   extern __shared__ double lds[];
   __shared__ int2 tab[256];
+  __shared__ int queue[513];  // fixup = 2: flagged pairs of the chunk (+ count)
   if (a.fixup) {
-    const int32_t n = a.fix_list[a.nchunks];
+    const int32_t* list = a.fixup == 1 ? a.fix_list : a.flag_list;
+    const int32_t n = list[a.nchunks];
     for (int32_t k = blockIdx.x; k < n; k += gridDim.x) {
-      elas_chunk<T>(a, a.fix_list[k], lds, tab);
+      elas_chunk<T>(a, list[k], lds, tab, queue);
       __syncthreads();  // LDS and tab are reused by the next listed chunk
     }
     return;
   }
-  elas_chunk<T>(a, xcd_chunk(blockIdx.x, gridDim.x), lds, tab);
+  elas_chunk<T>(a, xcd_chunk(blockIdx.x, gridDim.x), lds, tab, queue);
+}
+
+// Pair q of the chunk: node i's 3 tangent rows (and 3 residual entries) of element e.
+template <typename T>
+__device__ __forceinline__ void elas_pair_at(const ElasArgs& a, int q, const int2* tab, double* lds, int aLen,
+                                             int& bad) {
+  const int32_t e = a.pair_elem[q];
+  const uint4 mv = a.pair_meta[q];
+  const uint8_t* mm = reinterpret_cast<const uint8_t*>(&mv);
+  const int2 nd = tab[mm[1]];
+  double fres[3] = {0.0, 0.0, 0.0};
+  T chk = 0;
+  if (a.model == 0)
+    elas_pair<T, 0>(a, e, mm, lds + nd.x, nd.y, fres, chk);
+  else
+    elas_pair<T, 1>(a, e, mm, lds + nd.x, nd.y, fres, chk);
+  if (sizeof(T) == 4 && !isfinite((float)chk)) bad = 1;
+  if (a.with_res) {
+    double* rr = lds + aLen + 3 * mm[1];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) atomicAdd(rr + i, -fres[i]);
+  }
 }
 
 template <typename T>
-__device__ __forceinline__ void elas_chunk(const ElasArgs& a, int64_t ch, double* lds, int2* tab) {
+__device__ __forceinline__ void elas_chunk(const ElasArgs& a, int64_t ch, double* lds, int2* tab, int* queue) {
   const int n0 = a.chunk_node[ch], n1 = a.chunk_node[ch + 1];
   const int q0 = a.chunk_pair[ch], q1 = a.chunk_pair[ch + 1];
   const int64_t na0 = a.nptr[n0], na1 = a.nptr[n1];
@@ -402,29 +462,52 @@ __device__ __forceinline__ void elas_chunk(const ElasArgs& a, int64_t ch, double
     tab[t] = make_int2((int)(9 * (ai - na0)), (int)(a.nptr[n0 + t + 1] - ai));
   }
   for (int k = threadIdx.x; k < aLen + rLen; k += blockDim.x) lds[k] = 0.0;
+  if (threadIdx.x == 0) queue[512] = 0;
   __syncthreads();
-  int bad = 0;
-  for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
-    const int32_t e = a.pair_elem[q];
-    const uint4 mv = a.pair_meta[q];
-    const uint8_t* mm = reinterpret_cast<const uint8_t*>(&mv);
-    const int2 nd = tab[mm[1]];
-    double fres[3] = {0.0, 0.0, 0.0};
-    T chk = 0;
-    if (a.model == 0)
-      elas_pair<T, 0>(a, e, mm, lds + nd.x, nd.y, fres, chk);
-    else
-      elas_pair<T, 1>(a, e, mm, lds + nd.x, nd.y, fres, chk);
-    if (sizeof(T) == 4 && !isfinite((float)chk)) bad = 1;
-    if (a.with_res) {
-      double* rr = lds + aLen + 3 * mm[1];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) atomicAdd(rr + i, -fres[i]);
+  int bad = 0, flagged = 0;
+  if (a.fixup == 2) {
+    // FP64 pairs of flagged elements only, gathered into a queue so that every lane of a
+    // wave works on one (a flagged pair is rare among its chunk's pairs)
+    int done = 0;
+    for (int base = q0; base < q1; base += blockDim.x) {
+      const int q = base + threadIdx.x;
+      if (q < q1 && a.elem_flag[a.pair_elem[q]] != 0) queue[atomicAdd(queue + 512, 1)] = q;
+      __syncthreads();
+      const int n = queue[512];  // < 512: drained whenever it reaches 256
+      if (n >= 256 || base + (int)blockDim.x >= q1) {
+        for (int k = threadIdx.x; k < n; k += blockDim.x) elas_pair_at<T>(a, queue[k], tab, lds, aLen, bad);
+        done += n;
+        __syncthreads();
+        if (threadIdx.x == 0) queue[512] = 0;
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0 && done) atomicAdd(a.flag_list + a.nchunks + 1, done);
+  } else {
+    for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+      if (sizeof(T) == 4 && a.elem_flag[a.pair_elem[q]] != 0) {  // flagged: left to the FP64 pass
+        flagged = 1;
+        continue;
+      }
+      elas_pair_at<T>(a, q, tab, lds, aLen, bad);
     }
   }
-  if (__syncthreads_or(bad) && threadIdx.x == 0 && sizeof(T) == 4)
-    a.fix_list[atomicAdd(a.fix_list + a.nchunks, 1)] = (int32_t)ch;
+  if (sizeof(T) == 4) {
+    if (__syncthreads_or(bad) && threadIdx.x == 0) a.fix_list[atomicAdd(a.fix_list + a.nchunks, 1)] = (int32_t)ch;
+    if (__syncthreads_or(flagged) && threadIdx.x == 0)
+      a.flag_list[atomicAdd(a.flag_list + a.nchunks, 1)] = (int32_t)ch;
+  } else {
+    __syncthreads();
+  }
   double* out = a.vals + 9 * na0;
+  if (a.fixup == 2) {  // add the flagged pairs to the rows the FP32 launch wrote
+    for (int k = threadIdx.x; k < aLen; k += blockDim.x)
+      if (lds[k] != 0.0) out[k] += lds[k];
+    if (a.with_res)
+      for (int k = threadIdx.x; k < rLen; k += blockDim.x)
+        if (lds[aLen + k] != 0.0) a.res[3 * (int64_t)n0 + k] += lds[aLen + k];
+    return;
+  }
   for (int k = threadIdx.x; k < aLen; k += blockDim.x) __builtin_nontemporal_store(lds[k], out + k);
   if (a.with_res)
     for (int k = threadIdx.x; k < rLen; k += blockDim.x) a.res[3 * (int64_t)n0 + k] = lds[aLen + k];
@@ -432,6 +515,11 @@ __device__ __forceinline__ void elas_chunk(const ElasArgs& a, int64_t ch, double
 
 __global__ void k_fill_count(int32_t* p) {
   if (threadIdx.x == 0) *p = 0;
+}
+
+__global__ void k_zero_i32(int64_t n, int32_t* p) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
 }
 
 int launch_elasticity(Ctx* c, int model, double Emod, double nu, int precision, int with_res, int tangent,
@@ -467,6 +555,22 @@ int launch_elasticity(Ctx* c, int model, double Emod, double nu, int precision, 
   a.res = c->vec[FEDD_VEC_RES_U].as<double>();
   a.with_res = with_res;
   const int64_t nqp = m.nelem * a.nq;
+  if (precision == FEDD_FP32_LOCAL) {
+    if (c->d_elas_fix.bytes < (size_t)(pl.nchunks + 1) * 4) {
+      int rc = alloc(c, c->d_elas_fix, (size_t)(pl.nchunks + 1) * 4);
+      if (rc) return rc;
+    }
+    if (c->d_elas_flag.bytes < (size_t)(pl.nchunks + 2 + m.nelem) * 4) {
+      int rc = alloc(c, c->d_elas_flag, (size_t)(pl.nchunks + 2 + m.nelem) * 4);
+      if (rc) return rc;
+    }
+    a.fix_list = c->d_elas_fix.as<int32_t>();
+    a.flag_list = c->d_elas_flag.as<int32_t>();
+    a.elem_flag = a.flag_list + pl.nchunks + 2;
+    // kernels, not memsets (capturable)
+    k_fill_count<<<1, 64, 0, c->stream>>>(a.fix_list + pl.nchunks);
+    k_zero_i32<<<(unsigned)((m.nelem + 2 + 255) / 256), 256, 0, c->stream>>>(m.nelem + 2, a.flag_list + pl.nchunks);
+  }
   ktimer_begin(c);
   k_elas_qp<<<(unsigned)((nqp + 63) / 64), 64, 0, c->stream>>>(a);
   FEDD_HIP(c, hipGetLastError());
@@ -477,14 +581,6 @@ int launch_elasticity(Ctx* c, int model, double Emod, double nu, int precision, 
   }
   const size_t lds = (size_t)(pl.lds_doubles + 3 * 256) * 8;
   a.nchunks = pl.nchunks;
-  if (precision == FEDD_FP32_LOCAL) {
-    if (c->d_elas_fix.bytes < (size_t)(pl.nchunks + 1) * 4) {
-      int rc = alloc(c, c->d_elas_fix, (size_t)(pl.nchunks + 1) * 4);
-      if (rc) return rc;
-    }
-    a.fix_list = c->d_elas_fix.as<int32_t>();
-    k_fill_count<<<1, 64, 0, c->stream>>>(a.fix_list + pl.nchunks);  // a kernel, not a memset (capturable)
-  }
   ktimer_begin(c);
   if (precision == FEDD_FP32_LOCAL)
     k_elas_rows<float><<<(unsigned)pl.nchunks, 256, lds, c->stream>>>(a);
@@ -493,9 +589,20 @@ int launch_elasticity(Ctx* c, int model, double Emod, double nu, int precision, 
   FEDD_HIP(c, hipGetLastError());
   ktimer_end(c, "elas_rows");
   if (precision == FEDD_FP32_LOCAL) {
-    // FP64 re-assembly of the chunks whose FP32 tangent left FP32's range (usually none)
+    const unsigned gfix = (unsigned)std::min<int64_t>(pl.nchunks, c->num_cu);
+    // FP64 pairs of flagged elements, added to the rows just written: one workgroup per
+    // chunk, those beyond the listed count return at once.  Its cost is about that of
+    // one FP64 pass over the listed chunks (each chunk's few flagged pairs take one wave
+    // through the FP64 pair code): at 32^3 x 6 with grad u up to 3, where 10% of the
+    // pairs are flagged, FP32-local takes 2.0 ms and FP64 1.33 ms; with no flags it costs
+    // nothing
+    a.fixup = 2;
+    k_elas_rows<double><<<(unsigned)pl.nchunks, 256, lds, c->stream>>>(a);
+    FEDD_HIP(c, hipGetLastError());
+    // FP64 re-assembly of the chunks whose FP32 tangent left FP32's range (usually none);
+    // it rewrites whole rows, the pairs above included
     a.fixup = 1;
-    k_elas_rows<double><<<(unsigned)std::min<int64_t>(pl.nchunks, c->num_cu), 256, lds, c->stream>>>(a);
+    k_elas_rows<double><<<gfix, 256, lds, c->stream>>>(a);
     FEDD_HIP(c, hipGetLastError());
   }
   c->elas_fix_valid = precision == FEDD_FP32_LOCAL;

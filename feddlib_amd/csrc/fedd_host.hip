@@ -1059,17 +1059,21 @@ int fedd_assembly_nonlinear_elasticity_tangent(fedd_ctx* c, int neo_hooke_model,
   return elasticity_common(c, model, E, nu, precision, 1, nullptr, tangent == 1 ? 2 : 1);
 }
 
-int fedd_elasticity_fp64_fixups(fedd_ctx* c, int64_t* nchunks) {
+int fedd_elasticity_fp64_fixups(fedd_ctx* c, int64_t* nchunks, int64_t* npairs) {
   if (!c || !nchunks) return FEDD_ERR_ARG;
   *nchunks = 0;
+  if (npairs) *npairs = 0;
   if (!c->elas_fix_valid) return FEDD_OK;
   if (c->capturing) return fail(c, FEDD_ERR_STATE, "the query synchronises: it cannot be captured");
   (void)hipSetDevice(c->device);
-  int32_t h = 0;
-  FEDD_HIP(c, hipMemcpyAsync(&h, c->d_elas_fix.as<int32_t>() + c->vplan.nchunks, 4, hipMemcpyDeviceToHost,
+  int32_t h[2] = {0, 0};
+  FEDD_HIP(c, hipMemcpyAsync(&h[0], c->d_elas_fix.as<int32_t>() + c->vplan.nchunks, 4, hipMemcpyDeviceToHost,
+                             c->stream));
+  FEDD_HIP(c, hipMemcpyAsync(&h[1], c->d_elas_flag.as<int32_t>() + c->vplan.nchunks + 1, 4, hipMemcpyDeviceToHost,
                              c->stream));
   FEDD_HIP(c, hipStreamSynchronize(c->stream));
-  *nchunks = h;
+  *nchunks = h[0];
+  if (npairs) *npairs = h[1];
   return FEDD_OK;
 }
 

@@ -113,7 +113,8 @@ struct Ctx {
   DevBuf d_rhsq;                  // assemblyRHS: quadrature points / source values at them
   int qp_nq = 0;                  // points per element of the records in d_qp (0: none yet)
   DevBuf d_elas_fix;              // int32: chunks the last FP32-local elasticity launch re-did in FP64, + count
-  bool elas_fix_valid = false;    // d_elas_fix belongs to the last elasticity assembly
+  DevBuf d_elas_flag;             // int32: chunks holding pairs of flagged elements, + count, + pairs, + element flags
+  bool elas_fix_valid = false;    // d_elas_fix / d_elas_flag belong to the last elasticity assembly
   // colouring
   std::vector<int64_t> colour_ptr;
   std::vector<int32_t> colour_perm;

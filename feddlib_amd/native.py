@@ -125,7 +125,7 @@ _SIGS = {
     "fedd_traction_load": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, _dp]),
     "fedd_set_preconditioner": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fedd_linear_solve": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _ip, _dp]),
-    "fedd_elasticity_fp64_fixups": (ctypes.c_int, [_vp, _lp]),
+    "fedd_elasticity_fp64_fixups": (ctypes.c_int, [_vp, _lp, _lp]),
     "fedd_assembly_nonlinear_elasticity_tangent": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                                                   ctypes.c_int, ctypes.c_int]),
     "fedd_ns_linear_solve": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _ip, _dp]),
@@ -369,9 +369,15 @@ class Context:
         self._chk(self._L.fedd_assembly_nonlinear_elasticity(self.h, neo_hooke_model, E, nu, precision))
 
     def elasticity_fp64_fixups(self):
-        n = ctypes.c_int64()
-        self._chk(self._L.fedd_elasticity_fp64_fixups(self.h, ctypes.byref(n)))
-        return n.value
+        """Chunks the last FP32-local elasticity assembly re-assembled in FP64."""
+        return self.elasticity_fp64_counts()[0]
+
+    def elasticity_fp64_counts(self):
+        """(chunks re-assembled in FP64, (node, element) pairs of flagged elements computed
+        in FP64) of the last FP32-local elasticity assembly."""
+        n, p = ctypes.c_int64(), ctypes.c_int64()
+        self._chk(self._L.fedd_elasticity_fp64_fixups(self.h, ctypes.byref(n), ctypes.byref(p)))
+        return n.value, p.value
 
     def assembly_nonlinear_elasticity_tangent(self, neo_hooke_model, E, nu, precision=FP64, tangent=0):
         self._chk(self._L.fedd_assembly_nonlinear_elasticity_tangent(self.h, neo_hooke_model, E, nu, precision,

@@ -228,12 +228,16 @@ int fedd_assembly_nonlinear_elasticity_efield(fedd_ctx* ctx, int neo_hooke_model
  * these constants, else FEDD_ERR_STATE. */
 int fedd_assembly_nonlinear_elasticity_tangent(fedd_ctx* ctx, int neo_hooke_model, double E, double nu,
                                                 int precision, int tangent);
-/* FP32-local elasticity: a chunk of rows whose FP32 tangent holds a non-finite entry (the
- * law's tangent can leave FP32's range on nearly degenerate elements) is re-assembled in
- * FP64 by the same call, on the device, so no non-finite FP32 entry is ever written where
- * the FP64 path is finite.  This returns how many chunks the last elasticity assembly
- * re-did in FP64 (0 for FP64 assemblies); it synchronises the context stream. */
-int fedd_elasticity_fp64_fixups(fedd_ctx* ctx, int64_t* nchunks);
+/* FP32-local elasticity guards, both applied on the device by the same call:
+ *  - an element with a quadrature point where FP32 products cannot resolve the tangent
+ *    (kappa = |C|_F |adj C|_F / det C > 1e4 for C = F^T F, which includes J = 0: measured
+ *    FP32 error about 10 u32 kappa of the element's largest entry) has its (node, element)
+ *    pairs computed in FP64 instead; *npairs (may be null) counts them;
+ *  - a chunk of rows whose FP32 tangent still holds a non-finite entry (the law's tangent
+ *    can leave FP32's range) is re-assembled in FP64; *nchunks counts them,
+ * so no non-finite FP32 entry is written where the FP64 path is finite.  Counts of the
+ * last elasticity assembly (0 for FP64 assemblies); synchronises the context stream. */
+int fedd_elasticity_fp64_fixups(fedd_ctx* ctx, int64_t* nchunks, int64_t* npairs);
 /* ------------------------------------------------------- nonlinear solve */
 /* Dirichlet dofs of the primary field (the dofs BCBuilder::addBC flags,
  * BCBuilder_def.hpp:590-700): one entry per constrained dof, dof = node for a scalar graph,

@@ -18,9 +18,11 @@ element is max(5e-12, 4 x |oracle - AceGen|): where the oracle and AceGen agree,
 own formulation (stress and tangent derived from the strain energy) stays within 5e-12 of
 AceGen even at |F - I| = 2.3 (measured 3.7e-12; at the strains of the other elasticity
 tests it meets 1e-12).
-FP32-local: finite everywhere (non-finite chunks are re-assembled in FP64); its error
-(reported, up to O(1) on the ill-conditioned elements) is bounded by 1e-3 of the element's
-largest entry on the 16 randomly chosen elements (measured 5.8e-4).
+FP32-local: elements with a quadrature point of kappa(F^T F) > 1e4 are computed in FP64
+(without that guard their FP32 error reached 2.9 of the element's largest entry); the
+others stay within 1e-2 of it (the FP32 error grows as about 10 u32 kappa, 6e-3 at the
+threshold; measured 2.6e-3 at kappa = 3.4e3), and non-finite chunks would be
+re-assembled in FP64.
 """
 import os
 
@@ -58,6 +60,9 @@ def test_strained_elements_match_acegen(gpu_ctx, model, name):
     per = {}
     for prec in (N.FP64, N.FP32_LOCAL):
         gpu_ctx.assembly_nonlinear_elasticity(model, float(d["E"]), float(d["nu"]), prec)
+        if prec == N.FP32_LOCAL:
+            chunks, pairs = gpu_ctx.elasticity_fp64_counts()
+            assert pairs >= 10 and pairs % 10 == 0  # whole elements (disjoint: 10 pairs each)
         v = gpu_ctx.block_values(N.BLOCK_A)
         res = gpu_ctx.vector_download(N.VEC_RES_U, nd)
         assert np.all(np.isfinite(v)) and np.all(np.isfinite(res))
@@ -77,7 +82,7 @@ def test_strained_elements_match_acegen(gpu_ctx, model, name):
     assert np.all(per[N.FP64][0] <= bar), per[N.FP64][0] / bar
     assert np.max(per[N.FP64][1]) <= 1e-9  # residual p (AceGen vs oracle: 2e-11)
     assert np.max(per[N.FP32_LOCAL][1]) <= 1e-9  # the residual stays FP64
-    assert np.max(per[N.FP32_LOCAL][0][16:]) <= 1e-3
+    assert np.max(per[N.FP32_LOCAL][0]) <= 1e-2
 
 
 def test_fp32_local_full_state_has_no_nonfinite_entries(gpu_ctx):
