@@ -11,7 +11,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfeddgpu.so")
 
-SOURCES = ["fedd_host.hip", "fedd_ns.hip", "fedd_elas.hip", "fedd_solve.hip", "fedd_mlp.hip", "fedd_mg.hip", "fedd_hdf5.hip", "fedd_nsolve.hip"]
+SOURCES = ["fedd_host.hip", "fedd_ns.hip", "fedd_elas.hip", "fedd_solve.hip", "fedd_mlp.hip", "fedd_mg.hip", "fedd_hdf5.hip", "fedd_nsolve.hip",
+           "fedd_surface.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

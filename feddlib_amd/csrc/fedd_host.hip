@@ -489,7 +489,8 @@ int fedd_mesh_upload(fedd_ctx* c, int slot, const double* pts, int64_t npts,
   c->colour_ptr.clear();
   c->colour_perm.clear();
   c->newton_step = 0;
-  if (slot == 0) {  // the Dirichlet set belongs to the primary mesh
+  if (slot == 0) {  // the Dirichlet set and the surface triangles belong to the primary mesh
+    c->nface = 0;
     c->nbc = 0;
     c->nbc_dofs = 0;
     c->nbc_rows = -1;
@@ -826,6 +827,7 @@ static int64_t vec_len(fedd_ctx* c, int which) {
     case FEDD_VEC_B_P: return c->mesh[1].npts;
     case FEDD_VEC_RHS: return c->mesh[0].npts;
     case FEDD_VEC_FEXT:
+    case FEDD_VEC_FPRES:
     case FEDD_VEC_X:
     case FEDD_VEC_B: return 3 * c->mesh[0].npts;
   }

@@ -119,6 +119,11 @@ struct Ctx {
   std::vector<int64_t> colour_ptr;
   std::vector<int32_t> colour_perm;
   // vectors
+  // surface triangles of fedd_surface_set (6 node ids each, FEDDLib P2 triangle order) and
+  // the follower pressure per flag of fedd_newton_pressure
+  DevBuf d_faces, d_fflags;
+  int64_t nface = 0;
+  std::vector<double> newton_pressure;
   DevBuf vec[FEDD_NUM_VECTORS];
   int64_t vec_n[FEDD_NUM_VECTORS] = {};
   // Dirichlet constraints and Krylov work space (elasticity Newton solve)
@@ -134,6 +139,8 @@ struct Ctx {
   int64_t nbc_rows = -1;
   const void* bcrows_graph = nullptr;  // colind buffer the list was built for
   DevBuf d_kw[5];       // r, z, p, Ap and the block-Jacobi inverse (9 per node)
+  DevBuf d_bw[7];       // BiCGStab: r, r_hat, p, v, p_hat, s_hat, t
+  DevBuf d_bscal;       // BiCGStab dot products (16 sub-slots each)
   DevBuf d_part;        // per-block partial sums
   DevBuf d_scal;        // device scalars of the Krylov loop
   double* h_rr = nullptr;               // pinned: r.r partial sums read back by the CG loop (4 x 16)
@@ -210,6 +217,10 @@ int launch_stream_copy(Ctx* c, int64_t bytes, int iters, float* ms);
 int launch_elasticity(Ctx* c, int model, double Emod, double nu, int precision, int with_res, int tangent,
                       const double* Evec = nullptr);
 int launch_mlp_tangent(Ctx* c, int nq);
+// Follower pressure load (pressures scale * pressure[flag]) into vector `out` (zeroed first
+// unless accumulate) and, with_tangent, Kext added to block A.
+int surface_pressure(Ctx* c, const double* pressure, int nflag, double scale, int with_tangent, int out,
+                     bool accumulate);
 void mlp_free(Ctx* c);
 // True when the loaded network was built for this law and these constants.
 bool mlp_matches(const Ctx* c, int model, double Emod, double nu);

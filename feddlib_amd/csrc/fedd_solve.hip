@@ -109,16 +109,18 @@ __global__ void k_fill_i32(int64_t n, int32_t* __restrict__ a, int32_t v) {
   if (i < n) a[i] = v;
 }
 
-// R = -res - lam fext, zero on constrained dofs (res = assembled element rhs p).
+// R = -res - lam fext - fpres, zero on constrained dofs (res = assembled element rhs p,
+// fpres = follower pressure load at the load factor's pressure).
 __global__ __launch_bounds__(256) void k_newton_residual(int64_t n, const double* __restrict__ res,
                                                          const double* __restrict__ fext, double lam,
+                                                         const double* __restrict__ fpres,
                                                          const uint8_t* __restrict__ flag,
                                                          double* __restrict__ R, double* __restrict__ part) {
   __shared__ double red[256];
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   double v = 0.0;
   if (i < n) {
-    v = flag[i] ? 0.0 : -res[i] - lam * (fext ? fext[i] : 0.0);
+    v = flag[i] ? 0.0 : -res[i] - lam * (fext ? fext[i] : 0.0) - (fpres ? fpres[i] : 0.0);
     R[i] = v;
   }
   red[threadIdx.x] = v * v;
@@ -638,6 +640,141 @@ int pcg_mg_solve(Ctx* c, double rtol, double atol, int maxit, int* iters, double
   return FEDD_OK;
 }
 
+// ------------------------------------------------------------ BiCGStab
+// a.b into out0 and (if c) c.d into out1, 16 sub-slots each (block_atomic_sum).
+__global__ __launch_bounds__(256) void k_dots(int64_t n, const double* __restrict__ a, const double* __restrict__ b,
+                                              const double* __restrict__ c, const double* __restrict__ d,
+                                              double* __restrict__ out0, double* __restrict__ out1) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  double s0 = 0.0, s1 = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
+    s0 += a[i] * b[i];
+    if (c) s1 += c[i] * d[i];
+  }
+  block_atomic_sum(s0, out0);
+  if (c) block_atomic_sum(s1, out1);
+}
+
+// p = r + beta (p - omega v)
+__global__ __launch_bounds__(256) void k_bicg_p(int64_t n, double* __restrict__ p, const double* __restrict__ r,
+                                                const double* __restrict__ v, double beta, double omega) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = r[i] + beta * (p[i] - omega * v[i]);
+}
+
+// z = Binv r with the 3x3 node-block inverses
+__global__ __launch_bounds__(256) void k_bj_apply(int64_t nnode, const double* __restrict__ binv,
+                                                  const double* __restrict__ r, double* __restrict__ z) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nnode) return;
+  const double* B = binv + 9 * i;
+  const double r0 = r[3 * i], r1 = r[3 * i + 1], r2 = r[3 * i + 2];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) z[3 * i + d] = B[3 * d] * r0 + B[3 * d + 1] * r1 + B[3 * d + 2] * r2;
+}
+
+// Right-preconditioned BiCGStab (van der Vorst 1992) on block A: b = VEC_B, x = VEC_X from
+// 0, for the non-symmetric Newton matrices (follower loads).  precond: 0 block Jacobi, 1
+// geometric multigrid, 2 smoothed aggregation (one V-cycle per application; refresh as
+// for pcg_mg_solve).  Stops at ||r|| <= max(rtol ||b||, atol); three synchronisations per
+// iteration (the dot products).
+int pbicgstab_solve(Ctx* c, double rtol, double atol, int maxit, int* iters, double* relres, int precond,
+                    int refresh) {
+  const Block& A = c->blk[0];
+  const int64_t n = A.nrows, nnode = n / 3;
+  int rc;
+  if (precond >= 1) {
+    if ((rc = mg_setup(c, precond - 1))) return rc;
+    if (refresh == 1 && (rc = mg_refresh(c))) return rc;
+    if (refresh == 2 && (rc = mg_refresh_fine(c))) return rc;
+  }
+  for (int k = 0; k < 7; ++k)
+    if ((rc = ensure(c, c->d_bw[k], n * 8))) return rc;
+  if ((rc = ensure(c, c->d_kw[4], nnode * 9 * 8))) return rc;
+  if ((rc = ensure(c, c->d_bscal, 64 * 8))) return rc;
+  if ((rc = ensure_vec(c, FEDD_VEC_X, n))) return rc;
+  double *r = c->d_bw[0].as<double>(), *rh = c->d_bw[1].as<double>(), *p = c->d_bw[2].as<double>(),
+         *v = c->d_bw[3].as<double>(), *ph = c->d_bw[4].as<double>(), *sh = c->d_bw[5].as<double>(),
+         *t = c->d_bw[6].as<double>();
+  double* x = c->vec[FEDD_VEC_X].as<double>();
+  const double* b = c->vec[FEDD_VEC_B].as<double>();
+  double* bs = c->d_bscal.as<double>();
+  double* binv = c->d_kw[4].as<double>();
+  const int64_t* nptr = c->d_nptr.as<int64_t>();
+  const int32_t* nidx = c->d_nidx.as<int32_t>();
+  const double* va = A.d_vals.as<double>();
+  const unsigned gmax = (unsigned)c->num_cu * 8;
+  const unsigned g_spmv = std::min(nb(nnode * FEDD_SPMV_LANES, 256), gmax), g_dofs = nb(n, 256);
+  const unsigned g_red = std::min(nb(n, 256), gmax);
+  if (precond == 0) k_block_diag<<<nb(nnode, 256), 256, 0, c->stream>>>(nnode, A.d_rowptr.as<int64_t>(),
+                                                                          A.d_colind.as<int32_t>(), va, binv);
+  auto apply = [&](const double* in, double* out) -> int {
+    if (precond >= 1) return mg_vcycle(c, in, out);
+    k_bj_apply<<<nb(nnode, 256), 256, 0, c->stream>>>(nnode, binv, in, out);
+    return FEDD_OK;
+  };
+  auto spmv = [&](const double* in, double* out) {
+    k_bspmv_dot<FEDD_SPMV_LANES><<<g_spmv, 256, 0, c->stream>>>(nnode, nptr, nidx, va, in, out, bs + 32);
+  };
+  double h[32];
+  auto dots = [&](const double* a1, const double* b1, const double* c1, const double* d1, double* o0,
+                  double* o1) -> int {
+    k_fill<<<1, 64, 0, c->stream>>>(32, bs, 0.0);
+    k_dots<<<g_red, 256, 0, c->stream>>>(n, a1, b1, c1, d1, bs, bs + 16);
+    FEDD_HIP(c, hipGetLastError());
+    FEDD_HIP(c, hipMemcpyAsync(h, bs, 32 * 8, hipMemcpyDeviceToHost, c->stream));
+    FEDD_HIP(c, hipStreamSynchronize(c->stream));
+    double s0 = 0.0, s1 = 0.0;
+    for (int k = 0; k < 16; ++k) {
+      s0 += h[k];
+      s1 += h[16 + k];
+    }
+    *o0 = s0;
+    if (o1) *o1 = s1;
+    return FEDD_OK;
+  };
+  k_fill<<<g_dofs, 256, 0, c->stream>>>(n, x, 0.0);
+  FEDD_HIP(c, hipMemcpyAsync(r, b, n * 8, hipMemcpyDeviceToDevice, c->stream));
+  FEDD_HIP(c, hipMemcpyAsync(rh, b, n * 8, hipMemcpyDeviceToDevice, c->stream));
+  k_fill<<<g_dofs, 256, 0, c->stream>>>(n, p, 0.0);
+  k_fill<<<g_dofs, 256, 0, c->stream>>>(n, v, 0.0);
+  double rho = 1.0, alpha = 1.0, omega = 1.0, rho_new = 0.0, rr = 0.0;
+  if ((rc = dots(rh, r, r, r, &rho_new, &rr))) return rc;
+  const double bnorm = std::sqrt(rr);
+  const double target = std::max(rtol * bnorm, atol);
+  double rnorm = bnorm;
+  int it = 0;
+  while (rnorm > target && it < maxit) {
+    if (rho_new == 0.0 || omega == 0.0) break;  // breakdown: the caller sees relres
+    const double beta = (rho_new / rho) * (alpha / omega);
+    k_bicg_p<<<g_dofs, 256, 0, c->stream>>>(n, p, r, v, beta, omega);
+    if ((rc = apply(p, ph))) return rc;
+    spmv(ph, v);
+    double rhv = 0.0;
+    if ((rc = dots(rh, v, nullptr, nullptr, &rhv, nullptr))) return rc;
+    if (rhv == 0.0) break;
+    alpha = rho_new / rhv;
+    k_axpy<<<g_dofs, 256, 0, c->stream>>>(n, -alpha, v, r);  // r holds s
+    k_axpy<<<g_dofs, 256, 0, c->stream>>>(n, alpha, ph, x);
+    if ((rc = apply(r, sh))) return rc;
+    spmv(sh, t);
+    double ts = 0.0, tt = 0.0;
+    if ((rc = dots(t, r, t, t, &ts, &tt))) return rc;
+    omega = tt > 0.0 ? ts / tt : 0.0;
+    k_axpy<<<g_dofs, 256, 0, c->stream>>>(n, omega, sh, x);
+    k_axpy<<<g_dofs, 256, 0, c->stream>>>(n, -omega, t, r);
+    rho = rho_new;
+    ++it;
+    if ((rc = dots(rh, r, r, r, &rho_new, &rr))) return rc;
+    rnorm = std::sqrt(rr);
+    if (!(rnorm == rnorm)) break;  // NaN: let the caller reject the step
+  }
+  FEDD_HIP(c, hipGetLastError());
+  if (iters) *iters = it;
+  if (relres) *relres = bnorm > 0 ? rnorm / bnorm : 0.0;
+  return FEDD_OK;
+}
+
 // ||x||_2 of a device vector of length n (synchronises the context stream).
 __global__ __launch_bounds__(256) void k_sumsq(int64_t n, const double* __restrict__ x, double* __restrict__ part) {
   __shared__ double red[256];
@@ -667,7 +804,7 @@ static int vec_norm(Ctx* c, const double* x, int64_t n, double* out) {
   return FEDD_OK;
 }
 
-int newton_residual(Ctx* c, double lam, double* norm) {
+int newton_residual(Ctx* c, double lam, bool pressure, double* norm) {
   const int64_t n = 3 * c->mesh[0].npts;
   const int64_t nblk = nb(n, 256);
   int rc;
@@ -677,7 +814,8 @@ int newton_residual(Ctx* c, double lam, double* norm) {
   if ((rc = ensure(c, c->d_bcflag, n))) return rc;
   if (c->nbc == 0) FEDD_HIP(c, hipMemsetAsync(c->d_bcflag.p, 0, n, c->stream));
   const double* fext = c->vec_n[FEDD_VEC_FEXT] == n ? c->vec[FEDD_VEC_FEXT].as<double>() : nullptr;
-  k_newton_residual<<<nblk, 256, 0, c->stream>>>(n, c->vec[FEDD_VEC_RES_U].as<double>(), fext, lam,
+  const double* fp = pressure && c->vec_n[FEDD_VEC_FPRES] == n ? c->vec[FEDD_VEC_FPRES].as<double>() : nullptr;
+  k_newton_residual<<<nblk, 256, 0, c->stream>>>(n, c->vec[FEDD_VEC_RES_U].as<double>(), fext, lam, fp,
                                                  c->d_bcflag.as<uint8_t>(), c->vec[FEDD_VEC_B].as<double>(),
                                                  c->d_part.as<double>());
   k_reduce<<<1, 1024, 0, c->stream>>>(c->d_part.as<double>(), nblk, c->d_scal.as<double>(), 6, 0);
@@ -894,6 +1032,7 @@ int fedd_newton_elasticity(fedd_ctx* cc, const fedd_newton_params* P, fedd_newto
       P->preconditioner > 2 || P->criterion < 0 || P->criterion > 2 || (P->criterion > 0 && !(P->rel_tol > 0)))
     return fail(c, FEDD_ERR_ARG, "bad Newton parameters");
   const int model = P->neo_hooke_model == 1 ? 1 : 2;
+  const bool press = !c->newton_pressure.empty() && c->nface > 0;  // follower pressure load
   if ((P->tangent == 1 || P->collect_samples) && !c->mlp_ready)
     return fail(c, FEDD_ERR_STATE, "surrogate tangent or sample collection requested but no network");
   if (P->tangent == 1 && !mlp_matches(c, model, P->E, P->nu))
@@ -942,9 +1081,12 @@ int fedd_newton_elasticity(fedd_ctx* cc, const fedd_newton_params* P, fedd_newto
     for (int it = 0; it <= P->max_newton; ++it) {
       FEDD_HIP(c, hipEventRecord(ea, c->stream));
       if ((rc = launch_elasticity(c, model, P->E, P->nu, P->precision, 1, surrogate ? 2 : 1))) return rc;
+      if (press && (rc = surface_pressure(c, c->newton_pressure.data(), (int)c->newton_pressure.size(), lam, 1,
+                                          FEDD_VEC_FPRES, false)))
+        return rc;
       FEDD_HIP(c, hipEventRecord(eb, c->stream));
       if (P->collect_samples && (rc = mlp_collect_qp(c))) return rc;
-      if ((rc = newton_residual(c, lam, &norm))) return rc;
+      if ((rc = newton_residual(c, lam, press, &norm))) return rc;
       if ((rc = settle_solve())) return rc;
       float ms = 0.f;
       FEDD_HIP(c, hipEventElapsedTime(&ms, ea, eb));
@@ -969,8 +1111,11 @@ int fedd_newton_elasticity(fedd_ctx* cc, const fedd_newton_params* P, fedd_newto
         st->fallbacks++;
         FEDD_HIP(c, hipEventRecord(ea, c->stream));
         if ((rc = launch_elasticity(c, model, P->E, P->nu, P->precision, 1, 1))) return rc;
+        if (press && (rc = surface_pressure(c, c->newton_pressure.data(), (int)c->newton_pressure.size(), lam, 1,
+                                            FEDD_VEC_FPRES, false)))
+          return rc;
         FEDD_HIP(c, hipEventRecord(eb, c->stream));
-        if ((rc = newton_residual(c, lam, &norm))) return rc;
+        if ((rc = newton_residual(c, lam, press, &norm))) return rc;
         FEDD_HIP(c, hipEventElapsedTime(&ms, ea, eb));
         st->assembly_ms += ms;
         if (done(norm)) {
@@ -995,7 +1140,9 @@ int fedd_newton_elasticity(fedd_ctx* cc, const fedd_newton_params* P, fedd_newto
       const int refresh =
           P->mg_reuse && coarse_current && last_iters <= 2 * base_iters + 2 ? 2 : 1;
       FEDD_HIP(c, hipEventRecord(sa, c->stream));
-      if (P->preconditioner >= 1)  // b = -R, ||b|| = norm
+      if (press)  // A + Kext is not symmetric
+        rc = pbicgstab_solve(c, eta, 0.0, P->lin_maxit, &iters, &relres, P->preconditioner, refresh);
+      else if (P->preconditioner >= 1)  // b = -R, ||b|| = norm
         rc = pcg_mg_solve(c, eta, 0.0, P->lin_maxit, &iters, &relres, refresh, norm, P->preconditioner - 1);
       else
         rc = pcg_solve(c, eta, 0.0, P->lin_maxit, &iters, &relres);

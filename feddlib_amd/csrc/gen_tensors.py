@@ -3,7 +3,9 @@
 The element integrals of FEDDLib's P2-P1 Navier-Stokes element
 (AssembleFENavierStokes_def.hpp:150-461) and its P1/P2 Laplace elements are
 polynomials of degree <= 5 on affine tets, which the reference's own
-quadrature rules (Helper.cpp:1300-1517) integrate exactly.  The kernels use the
+quadrature rules (Helper.cpp:1300-1517) integrate exactly.  The follower pressure
+load on a 6-node surface triangle is a polynomial of degree 4 in the reference
+coordinates (N_a times the cross product of two linear tangent vectors).  The kernels use the
 equivalent exact reference tensors instead of quadrature loops; this script
 computes them in exact rational arithmetic (monomial rule
 int x^a y^b z^c = a! b! c! / (a+b+c+3)! over the unit tet) and writes them as
@@ -54,6 +56,12 @@ class Poly(dict):
     def integral(self):
         return sum((v * mono_int(*k) for k, v in self.items()), Fr(0))
 
+    def integral_tri(self):
+        """Over the unit triangle (z-free terms only): int x^a y^b = a! b! / (a+b+2)!."""
+        assert all(k[2] == 0 for k in self)
+        return sum((v * Fr(factorial(k[0]) * factorial(k[1]), factorial(k[0] + k[1] + 2)) for k, v in self.items()),
+                   Fr(0))
+
     def at(self, p):
         return sum((v * p[0] ** k[0] * p[1] ** k[1] * p[2] ** k[2] for k, v in self.items()), Fr(0))
 
@@ -68,6 +76,10 @@ P1 = [L0, X, Y, Z]  # Helper.cpp:660-675
 P2 = [L0 * (ONE + X * -2 + Y * -2 + Z * -2), X * (X * 2 + ONE * -1), Y * (Y * 2 + ONE * -1),
       Z * (Z * 2 + ONE * -1), X * L0 * 4, X * Y * 4, Y * L0 * 4, Z * L0 * 4, X * Z * 4, Y * Z * 4]
 VERT = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1)]
+# 6-node triangle in FEDDLib's 2D P2 order (Helper::phi, Helper.cpp:662-686): vertices (0,0), (1,0),
+# (0,1), then the edges 01, 12, 02
+T0 = ONE + X * -1 + Y * -1
+TRI = [T0 * (T0 * 2 + ONE * -1), X * (X * 2 + ONE * -1), Y * (Y * 2 + ONE * -1), X * T0 * 4, X * Y * 4, Y * T0 * 4]
 
 
 def lit(v):
@@ -147,6 +159,13 @@ def main():
     # Volume integrals of the basis (loads): MV2[k] = int phi_k (P2), MV1 = int lambda (P1)
     print(arr("MV2", [p.integral() for p in P2], (10,)))
     print(arr("MV1", [p.integral() for p in P1], (4,)))
+    # Follower pressure on the 6-node surface triangle (reference triangle, FEDDLib's 2D P2
+    # order): PT[a][b][c] = int N_a (d_xi N_b d_eta N_c - d_xi N_c d_eta N_b), so that
+    # int N_a (x_xi x x_eta) = sum_{b<c} PT[a][b][c] (x_b x x_c)
+    m = [[[(TRI[a] * TRI[b].d(0) * TRI[c].d(1)).integral_tri() for c in range(6)] for b in range(6)]
+         for a in range(6)]
+    pt = [m[a][b][c] - m[a][c][b] for a in range(6) for b in range(6) for c in range(6)]
+    print(arr("PT", pt, (6, 6, 6)))
     print("}  // namespace fedd_tensors")
 
 

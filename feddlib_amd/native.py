@@ -15,7 +15,8 @@ FEDD_OK = 0
 MESH_PRIMARY, MESH_PRESSURE = 0, 1
 GRAPH_SCALAR, GRAPH_VECTOR, GRAPH_NS = 0, 1, 2
 BLOCK_A, BLOCK_BT, BLOCK_B, BLOCK_C = 0, 1, 2, 3
-VEC_U, VEC_P, VEC_RES_U, VEC_RES_P, VEC_RHS, VEC_FEXT, VEC_X, VEC_B, VEC_X_P, VEC_B_P = 0, 1, 2, 3, 4, 5, 6, 7, 8, 9
+VEC_U, VEC_P, VEC_RES_U, VEC_RES_P, VEC_RHS, VEC_FEXT, VEC_X, VEC_B, VEC_X_P, VEC_B_P, VEC_FPRES = range(11)
+MAX_SURFACE_FLAGS = 16
 FP64, FP32_LOCAL = 0, 1
 BC_RHS, BC_MINUS_VECTOR, BC_VECTOR_MINUS_BC = 0, 1, 2
 NS_JACOBIAN, NS_FIXEDPOINT, NS_RHS, NS_JACOBIAN_RHS, NS_FIXEDPOINT_RHS = 0, 1, 2, 3, 4
@@ -123,6 +124,10 @@ _SIGS = {
     "fedd_dirichlet_set_system": (ctypes.c_int, [_vp]),
     "fedd_dirichlet_vector": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "fedd_traction_load": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, _dp]),
+    "fedd_structured_surfaces": (ctypes.c_int, [_dp, ctypes.c_int64, _ip, ctypes.c_int64, _dp, _ip, _ip, _lp]),
+    "fedd_surface_set": (ctypes.c_int, [_vp, _ip, _ip, ctypes.c_int64]),
+    "fedd_assembly_surface_pressure": (ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "fedd_newton_pressure": (ctypes.c_int, [_vp, _dp, ctypes.c_int]),
     "fedd_set_preconditioner": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fedd_linear_solve": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _ip, _dp]),
     "fedd_elasticity_fp64_fixups": (ctypes.c_int, [_vp, _lp, _lp]),
@@ -197,6 +202,27 @@ def mesh_structured(M, fe):
     if rc:
         raise FeddError(rc, "fedd_mesh_structured")
     return pts, bc, el
+
+
+def structured_surfaces(pts, elems, box=(0.0, 0.0, 0.0, 1.0, 1.0, 1.0)):
+    """MeshStructured::buildSurfaces(3, "P2"): the boundary faces of a P2 tet mesh of the
+    box (x0, y0, z0, L, W, H) as outward 6-node triangles (n, 6) and their side flags (n,)."""
+    L = lib()
+    pts = np.ascontiguousarray(pts, np.float64)
+    elems = np.ascontiguousarray(elems, np.int32)
+    b = np.ascontiguousarray(box, np.float64)
+    n = ctypes.c_int64()
+    rc = L.fedd_structured_surfaces(_p(pts, _dp), pts.shape[0], _p(elems, _ip), elems.shape[0], _p(b, _dp), None,
+                                    None, ctypes.byref(n))
+    if rc:
+        raise FeddError(rc, L.fedd_last_error(None).decode())
+    faces = np.zeros((n.value, 6), np.int32)
+    flags = np.zeros(n.value, np.int32)
+    rc = L.fedd_structured_surfaces(_p(pts, _dp), pts.shape[0], _p(elems, _ip), elems.shape[0], _p(b, _dp),
+                                    _p(faces, _ip), _p(flags, _ip), ctypes.byref(n))
+    if rc:
+        raise FeddError(rc, L.fedd_last_error(None).decode())
+    return faces, flags
 
 
 def hdf5_read_vector(path, name):
@@ -408,6 +434,24 @@ class Context:
     def traction_load(self, axis, value, t):
         t = np.ascontiguousarray(t, np.float64)
         self._chk(self._L.fedd_traction_load(self.h, axis, value, _p(t, _dp)))
+
+    def surface_set(self, faces, flags):
+        faces = np.ascontiguousarray(faces, np.int32)
+        flags = np.ascontiguousarray(flags, np.int32)
+        if faces.ndim != 2 or faces.shape[1] != 6 or flags.shape != (faces.shape[0],):
+            raise ValueError("faces (n, 6) and flags (n,)")
+        self._chk(self._L.fedd_surface_set(self.h, _p(faces, _ip), _p(flags, _ip), faces.shape[0]))
+
+    def assembly_surface_pressure(self, pressure, with_tangent=False, out=VEC_FEXT):
+        """pressure[flag]: FE::assembly(Nonlinear)SurfaceIntegralExternal into vector `out`
+        (and Kext added to block A with with_tangent)."""
+        pr = np.ascontiguousarray(pressure, np.float64)
+        self._chk(self._L.fedd_assembly_surface_pressure(self.h, _p(pr, _dp), len(pr), 1 if with_tangent else 0, out))
+
+    def newton_pressure(self, pressure):
+        """Follower pressure per flag for newton_elasticity (empty: none)."""
+        pr = np.ascontiguousarray(pressure, np.float64)
+        self._chk(self._L.fedd_newton_pressure(self.h, _p(pr, _dp) if len(pr) else None, len(pr)))
 
     def set_preconditioner(self, kind):
         self._chk(self._L.fedd_set_preconditioner(self.h, kind))

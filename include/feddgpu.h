@@ -68,9 +68,12 @@ enum fedd_vector {
   FEDD_VEC_X = 6,      /* linear-solve solution / Newton increment, 3 per node */
   FEDD_VEC_B = 7,      /* linear-solve right-hand side, 3 per node */
   FEDD_VEC_X_P = 8,    /* pressure part of the NS linear-solve solution, 1 per pressure node */
-  FEDD_VEC_B_P = 9     /* pressure part of the NS linear-solve right-hand side */
+  FEDD_VEC_B_P = 9,    /* pressure part of the NS linear-solve right-hand side */
+  FEDD_VEC_FPRES = 10  /* follower pressure load of the elasticity problem, 3 per node */
 };
-#define FEDD_NUM_VECTORS 10
+#define FEDD_NUM_VECTORS 11
+/* Surface flags a pressure table may cover (flags 0 .. FEDD_MAX_SURFACE_FLAGS - 1). */
+#define FEDD_MAX_SURFACE_FLAGS 16
 
 enum fedd_precision {
   FEDD_FP64 = 0,       /* everything in FP64 */
@@ -238,6 +241,35 @@ int fedd_assembly_nonlinear_elasticity_tangent(fedd_ctx* ctx, int neo_hooke_mode
  * so no non-finite FP32 entry is written where the FP64 path is finite.  Counts of the
  * last elasticity assembly (0 for FP64 assemblies); synchronises the context stream. */
 int fedd_elasticity_fp64_fixups(fedd_ctx* ctx, int64_t* nchunks, int64_t* npairs);
+/* ------------------------------------------------------- surface loads */
+/* MeshStructured::buildSurfaces(3, "P2") (MeshStructured_def.hpp:3707-3876) on a P2 tet
+ * mesh of the box {x0, y0, z0, L, W, H}: every element face lying on a side of the box, as
+ * a 6-node triangle in FEDDLib's P2 triangle order (vertices, then edges 01, 12, 02),
+ * oriented so that (x1 - x0) x (x2 - x0) points out of the box, with the side's flag
+ * (x = x0: 1, y = y0: 2, z = z0: 3, z = z0 + H: 4, y = y0 + W: 5, x = x0 + L: 6), in
+ * element order and, within an element, face order.  faces (6 per face) and flags may be
+ * null to query the count *nface.  Host code; no context. */
+int fedd_structured_surfaces(const double* pts, int64_t npts, const int32_t* elems, int64_t nelem,
+                             const double* box, int32_t* faces, int32_t* flags, int64_t* nface);
+/* The surface triangles of the primary mesh that pressure loads act on (the surface
+ * sub-elements of FiniteElement::getSubElements, FE_def.hpp:6886-6893): nface x 6 node ids
+ * as above and one flag (>= 0) each.  A primary mesh upload clears them. */
+int fedd_surface_set(fedd_ctx* ctx, const int32_t* faces, const int32_t* flags, int64_t nface);
+/* FE::assemblySurfaceIntegralExternal (FE_def.hpp:6859-6925, with_tangent = 0) and
+ * FE::assemblyNonlinearSurfaceIntegralExternal (:6932-7045, with_tangent = 1): vector `out`
+ * (FEDD_VEC_FEXT, FEDD_VEC_FPRES, FEDD_VEC_B or FEDD_VEC_RES_U) is zeroed, then every surface
+ * triangle with p = pressure[flag] != 0 (the reference's func(.., flag)) adds the follower
+ * load of a pressure acting against the outward normal of the current surface X + VEC_U,
+ *     f_a = -p int N_a (x_xi x x_eta) dxi deta,
+ * the load of AceGen's PressureTriangle3D6 (:6912-6916).  with_tangent also adds
+ * Kext = -df/du to block A of the vector graph (A + Kext, NonLinElasAssFE_def.hpp:295-300).
+ * Flags >= nflag carry no pressure. */
+int fedd_assembly_surface_pressure(fedd_ctx* ctx, const double* pressure, int nflag, int with_tangent, int out);
+/* Follower pressure per flag for fedd_newton_elasticity (nflag = 0: none): ramped with the
+ * load factor like the traction load, its load and Kext are re-assembled at every Newton
+ * iterate, and the Newton steps are solved by BiCGStab (Kext is not symmetric) with the
+ * same preconditioners. */
+int fedd_newton_pressure(fedd_ctx* ctx, const double* pressure, int nflag);
 /* ------------------------------------------------------- nonlinear solve */
 /* Dirichlet dofs of the primary field (the dofs BCBuilder::addBC flags,
  * BCBuilder_def.hpp:590-700): one entry per constrained dof, dof = node for a scalar graph,

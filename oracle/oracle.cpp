@@ -1145,4 +1145,191 @@ void* orc_assembly_elasticity(int model, double Emod, double nu, const double* p
   return A;
 }
 
+// ---------------------------------------------------------------------------
+// Follower pressure on the 6-node surface triangle: the load FE::assemblySurfaceIntegral-
+// External / assemblyNonlinearSurfaceIntegralExternal take from AceGen's
+// PressureTriangle3D6 (FE_def.hpp:6905-6917, 7000-7006; the AceGenInterface library is not
+// part of the reference).  Restated from the published formulation of a pressure acting
+// against the outward normal of the current surface (FE_def.hpp:6928-6929): with
+// x = X + u and the 6-node basis N_a(xi, eta) of the reference triangle,
+//     f_a = -p int N_a (x_xi x x_eta) dxi deta          (added to the load vector)
+//     Kext_(a i)(d j) = -d f_a,i / d u_d,j              (added to the tangent, A + Kext,
+//                                                        NonLinElasAssFE_def.hpp:295-300)
+// integrated here with Dunavant's 7-point rule (degree 5; the integrand has degree 4).
+// ---------------------------------------------------------------------------
+namespace {
+void tri6_basis(double xi, double eta, double N[6], double dX[6], double dE[6]) {
+  const double l = 1.0 - xi - eta;
+  N[0] = l * (2.0 * l - 1.0);
+  N[1] = xi * (2.0 * xi - 1.0);
+  N[2] = eta * (2.0 * eta - 1.0);
+  N[3] = 4.0 * xi * l;
+  N[4] = 4.0 * xi * eta;
+  N[5] = 4.0 * eta * l;
+  dX[0] = 1.0 - 4.0 * l;
+  dE[0] = 1.0 - 4.0 * l;
+  dX[1] = 4.0 * xi - 1.0;
+  dE[1] = 0.0;
+  dX[2] = 0.0;
+  dE[2] = 4.0 * eta - 1.0;
+  dX[3] = 4.0 * (l - xi);
+  dE[3] = -4.0 * xi;
+  dX[4] = 4.0 * eta;
+  dE[4] = 4.0 * xi;
+  dX[5] = -4.0 * eta;
+  dE[5] = 4.0 * (l - eta);
+}
+
+void cross3(const double a[3], const double b[3], double c[3]) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// Face-local node lists of MeshStructured::buildSurfaces, P2 (MeshStructured_def.hpp:3746-3751)
+const int kSurfP2[4][6] = {{1, 0, 2, 4, 6, 5}, {0, 3, 2, 7, 9, 6}, {1, 2, 3, 5, 9, 8}, {1, 0, 3, 4, 7, 8}};
+}  // namespace
+
+void orc_pressure_triangle(double p, const double* X, const double* u, double* f, double* K) {
+  // Dunavant's degree-5 rule on the unit triangle in closed form (weights sum to 1/2):
+  // centroid 9/40; orbits b = (6 -+ sqrt 15)/21, a = 1 - 2b, weights (155 -+ sqrt 15)/1200
+  const double r15 = std::sqrt(15.0);
+  const double b2 = (6.0 - r15) / 21.0, a2 = 1.0 - 2.0 * b2, w2 = (155.0 - r15) / 2400.0;
+  const double b1 = (6.0 + r15) / 21.0, a1 = 1.0 - 2.0 * b1, w1 = (155.0 + r15) / 2400.0;
+  const double w0 = 9.0 / 80.0;
+  const double qp[7][3] = {{1.0 / 3, 1.0 / 3, w0}, {a1, b1, w1}, {b1, a1, w1}, {b1, b1, w1},
+                           {a2, b2, w2}, {b2, a2, w2}, {b2, b2, w2}};
+  for (int k = 0; k < 18; ++k) f[k] = 0.0;
+  if (K)
+    for (int k = 0; k < 324; ++k) K[k] = 0.0;
+  for (auto& q : qp) {
+    double N[6], dX[6], dE[6];
+    tri6_basis(q[0], q[1], N, dX, dE);
+    double xx[3] = {0, 0, 0}, xe[3] = {0, 0, 0};
+    for (int b = 0; b < 6; ++b)
+      for (int i = 0; i < 3; ++i) {
+        const double xb = X[3 * b + i] + u[3 * b + i];
+        xx[i] += dX[b] * xb;
+        xe[i] += dE[b] * xb;
+      }
+    double nrm[3];
+    cross3(xx, xe, nrm);
+    for (int a = 0; a < 6; ++a)
+      for (int i = 0; i < 3; ++i) f[3 * a + i] -= p * q[2] * N[a] * nrm[i];
+    if (!K) continue;
+    for (int d = 0; d < 6; ++d)
+      for (int j = 0; j < 3; ++j) {
+        double ej[3] = {0, 0, 0};
+        ej[j] = 1.0;
+        double t1[3], t2[3];
+        cross3(ej, xe, t1);  // d(x_xi)/du_dj = dX[d] e_j
+        cross3(xx, ej, t2);  // d(x_eta)/du_dj = dE[d] e_j
+        for (int a = 0; a < 6; ++a)
+          for (int i = 0; i < 3; ++i)
+            K[(3 * a + i) * 18 + 3 * d + j] += p * q[2] * N[a] * (dX[d] * t1[i] + dE[d] * t2[i]);
+      }
+  }
+}
+
+// MeshStructured::buildSurfaces(3, "P2") (MeshStructured_def.hpp:3707-3852) for the box
+// [x0, x0 + L] x [y0, y0 + W] x [z0, z0 + H] (box = {x0, y0, z0, L, W, H}): every face of
+// every element whose midpoint lies on a side gets that side's flag (x=0: 1, y=0: 2, z=0: 3,
+// z=1: 4, y=1: 5, x=1: 6, the last test that matches wins, as there), oriented so that
+// (x_1 - x_0) x (x_2 - x_0) points out (flipSurface, :3857-3876).  Returns the count; faces
+// (6 ids each) and flags are written when non-null.
+int64_t orc_structured_surfaces(const double* pts, const int32_t* elems, int64_t E, const double* box,
+                                int32_t* faces, int32_t* flags, int32_t* elem_of) {
+  const double tol = 1e-12;
+  int64_t nf = 0;
+  for (int64_t T = 0; T < E; ++T) {
+    const int32_t* nl = elems + 10 * T;
+    for (int s = 0; s < 4; ++s) {
+      int32_t v[6];
+      for (int k = 0; k < 6; ++k) v[k] = nl[kSurfP2[s][k]];
+      auto P = [&](int k, int d) { return pts[3 * (int64_t)v[k] + d]; };
+      auto normal = [&](double vE[3]) {
+        const double p1[3] = {P(0, 0) - P(1, 0), P(0, 1) - P(1, 1), P(0, 2) - P(1, 2)};
+        const double p2[3] = {P(0, 0) - P(2, 0), P(0, 1) - P(2, 1), P(0, 2) - P(2, 2)};
+        cross3(p1, p2, vE);
+      };
+      auto flip = [&]() {  // (0, 2, 1, 5, 4, 3)
+        std::swap(v[1], v[2]);
+        std::swap(v[3], v[5]);
+      };
+      double vE[3];
+      normal(vE);
+      double mid[3];
+      for (int d = 0; d < 3; ++d) mid[d] = (P(0, d) + P(1, d) + P(2, d)) / 3.0;
+      int flag = 10;
+      if (mid[0] < box[0] + tol) {
+        flag = 1;
+        if (vE[0] > 0) flip();
+      }
+      if (mid[0] > box[0] + tol && mid[2] < box[2] + tol) {
+        flag = 3;
+        if (vE[2] > 0) flip();
+      }
+      if (mid[0] > box[0] + tol && mid[2] > box[2] + box[5] - tol) {
+        if (vE[2] < 0) flip();
+        flag = 4;
+      }
+      if (mid[0] > box[0] + tol && mid[1] < box[1] + tol) {
+        if (vE[1] > 0) flip();
+        flag = 2;
+      }
+      if (mid[0] > box[0] + tol && mid[1] > box[1] + box[4] - tol) {
+        if (vE[1] < 0) flip();
+        flag = 5;
+      }
+      if (mid[0] > box[0] + box[3] - tol && mid[1] > box[1] + tol && mid[1] < box[1] + box[4] - tol &&
+          mid[2] > box[2] + tol && mid[2] < box[2] + box[5] - tol) {
+        if (vE[0] < 0) flip();
+        flag = 6;
+      }
+      if (flag == 10) continue;
+      if (faces)
+        for (int k = 0; k < 6; ++k) faces[6 * nf + k] = v[k];
+      if (flags) flags[nf] = flag;
+      if (elem_of) elem_of[nf] = (int32_t)T;
+      ++nf;
+    }
+  }
+  return nf;
+}
+
+// The surface loop of assemblySurfaceIntegralExternal / assemblyNonlinearSurfaceIntegral-
+// External (FE_def.hpp:6859-6925, 6932-7045): f = 0, then for every face with
+// p = pressure[flag] != 0 the element load is added to f and, when Kext is requested, its
+// 18 x 18 matrix is inserted row by row (Kext->insertGlobalValues, :7032-7034).
+void* orc_assembly_surface_pressure(const double* pts, int64_t npts, const int32_t* faces, const int32_t* flags,
+                                    int64_t nf, const double* u, const double* pressure, int nflag, double* f,
+                                    int with_tangent) {
+  for (int64_t k = 0; k < 3 * npts; ++k) f[k] = 0.0;
+  Crs* Kx = with_tangent ? new Crs(3 * npts) : nullptr;
+  double X[18], U[18], fe[18], Ke[324];
+  for (int64_t s = 0; s < nf; ++s) {
+    const int fl = flags[s];
+    const double p = (fl >= 0 && fl < nflag) ? pressure[fl] : 0.0;
+    if (p == 0.0) continue;
+    const int32_t* v = faces + 6 * s;
+    for (int a = 0; a < 6; ++a)
+      for (int i = 0; i < 3; ++i) {
+        X[3 * a + i] = pts[3 * (int64_t)v[a] + i];
+        U[3 * a + i] = u ? u[3 * (int64_t)v[a] + i] : 0.0;
+      }
+    orc_pressure_triangle(p, X, U, fe, Kx ? Ke : nullptr);
+    for (int a = 0; a < 6; ++a)
+      for (int i = 0; i < 3; ++i) f[3 * (int64_t)v[a] + i] += fe[3 * a + i];
+    if (Kx) {
+      int32_t cols[18];
+      for (int b = 0; b < 6; ++b)
+        for (int d = 0; d < 3; ++d) cols[3 * b + d] = 3 * v[b] + d;
+      for (int a = 0; a < 6; ++a)
+        for (int i = 0; i < 3; ++i) Kx->insert(3 * (int64_t)v[a] + i, cols, Ke + (3 * a + i) * 18, 18);
+    }
+  }
+  if (Kx) Kx->fill_complete();
+  return Kx;
+}
+
 }  // extern "C"

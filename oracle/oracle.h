@@ -124,6 +124,19 @@ void orc_nh_skr(int model, double E, double nu, const double* xl, const double* 
 void* orc_assembly_elasticity(int model, double E, double nu, const double* pts, int64_t npts,
                               const int32_t* elems, int64_t E_count, const double* u, double* res);
 
+/* Follower pressure on a 6-node surface triangle (AceGen PressureTriangle3D6 restated):
+   load f[18] and, if K is non-null, Kext[18][18] = -df/du, at X + u. */
+void orc_pressure_triangle(double p, const double* X18, const double* u18, double* f, double* K);
+/* MeshStructured::buildSurfaces(3, "P2") on a box {x0, y0, z0, L, W, H}: returns the face
+   count; faces (6 ids), flags and owning elements written when non-null. */
+int64_t orc_structured_surfaces(const double* pts, const int32_t* elems, int64_t E, const double* box,
+                                int32_t* faces, int32_t* flags, int32_t* elem_of);
+/* The surface-load loop: f (3 npts) = sum of face loads with p = pressure[flag] != 0; returns
+   Kext as a CRS handle when with_tangent (else null). */
+void* orc_assembly_surface_pressure(const double* pts, int64_t npts, const int32_t* faces, const int32_t* flags,
+                                    int64_t nf, const double* u, const double* pressure, int nflag, double* f,
+                                    int with_tangent);
+
 #ifdef __cplusplus
 }
 #endif

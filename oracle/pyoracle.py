@@ -63,6 +63,10 @@ def lib():
                                                           ctypes.c_double, ctypes.c_int]),
             "orc_nh_skr": (None, [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _dp]),
             "orc_set_element_mode": (None, [ctypes.c_int]),
+            "orc_pressure_triangle": (None, [ctypes.c_double, _dp, _dp, _dp, _dp]),
+            "orc_structured_surfaces": (ctypes.c_int64, [_dp, _ip, ctypes.c_int64, _dp, _ip, _ip, _ip]),
+            "orc_assembly_surface_pressure": (_vp, [_dp, ctypes.c_int64, _ip, _ip, ctypes.c_int64, _dp, _dp,
+                                                    ctypes.c_int, _dp, ctypes.c_int]),
             "orc_assembly_elasticity": (_vp, [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp,
                                               ctypes.c_int64, _ip, ctypes.c_int64, _dp, _dp]),
         }
@@ -251,6 +255,44 @@ def ref_skr(model, E, nu, xl, ul):
     hp = np.zeros(10)
     f(P(v), P(d), P(ul), P(ul0), P(xl), P(s), P(p), P(ht), P(hp))
     return s.reshape(30, 30, order="F"), p
+
+
+def pressure_triangle(p, X, u, tangent=True):
+    """AceGen PressureTriangle3D6 restated: load f (6, 3) and Kext (18, 18) = -df/du."""
+    f = np.zeros(18)
+    K = np.zeros(324) if tangent else None
+    lib().orc_pressure_triangle(p, P(np.ascontiguousarray(X, np.float64).ravel()),
+                                P(np.ascontiguousarray(u, np.float64).ravel()), P(f),
+                                P(K) if tangent else None)
+    return f.reshape(6, 3), (K.reshape(18, 18) if tangent else None)
+
+
+def structured_surfaces(pts, el, box=(0.0, 0.0, 0.0, 1.0, 1.0, 1.0)):
+    """MeshStructured::buildSurfaces(3, "P2"): faces (n, 6), flags (n,), owning element (n,)."""
+    pts = np.ascontiguousarray(pts, np.float64)
+    el = np.ascontiguousarray(el, np.int32)
+    b = np.asarray(box, np.float64)
+    n = lib().orc_structured_surfaces(P(pts), P(el, _ip), el.shape[0], P(b), None, None, None)
+    faces = np.zeros((n, 6), np.int32)
+    flags = np.zeros(n, np.int32)
+    owner = np.zeros(n, np.int32)
+    lib().orc_structured_surfaces(P(pts), P(el, _ip), el.shape[0], P(b), P(faces, _ip), P(flags, _ip),
+                                  P(owner, _ip))
+    return faces, flags, owner
+
+
+def surface_pressure(pts, faces, flags, u, pressure, tangent=False):
+    """The surface-load loop of FE::assembly(Nonlinear)SurfaceIntegralExternal: load vector
+    f (3 npts) and, with tangent, Kext as (rowptr, colind, vals)."""
+    pts = np.ascontiguousarray(pts, np.float64)
+    f = np.zeros(3 * pts.shape[0])
+    pr = np.ascontiguousarray(pressure, np.float64)
+    uu = None if u is None else np.ascontiguousarray(u, np.float64).ravel()
+    h = lib().orc_assembly_surface_pressure(P(pts), pts.shape[0], P(np.ascontiguousarray(faces, np.int32), _ip),
+                                            P(np.ascontiguousarray(flags, np.int32), _ip), len(flags),
+                                            None if uu is None else P(uu), P(pr), len(pr), P(f),
+                                            1 if tangent else 0)
+    return f, (crs_take(h) if tangent else None)
 
 
 def elasticity(model, E, nu, pts, el, u):
