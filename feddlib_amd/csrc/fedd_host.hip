@@ -158,9 +158,15 @@ inline int find_slot(const Adjacency& adj, int64_t i, int32_t j) {
 }
 
 // Row-gather schedule.  Nodes are cut into contiguous chunks whose rows fit the
-// LDS budget; inside a chunk the (node, element) pairs are ordered vertex class
-// first, then edge class, interleaving nodes so that neighbouring lanes add into
-// different rows.  Slots are stored in canonical column order (fedd_perm.h).
+// LDS budget.  Inside a chunk the (node, element) pairs are ordered vertex class
+// first, then edge class.  Inside a class the pairs are laid out in segments of 64,
+// one wave's worth: a segment holds whole nodes (the plan inserts padding pairs,
+// element -1, which the kernels skip, so that no node straddles two segments; the
+// vertex class is padded to a whole number of segments), and inside a segment the
+// k-th pairs of its nodes follow each other, so neighbouring lanes add into different
+// rows and gather neighbouring elements.  Every row of a chunk is then added to by one
+// wave only, in a fixed order, so the assembled values are reproducible bit for bit.
+// Slots are stored in canonical column order (fedd_perm.h).
 struct PlanSpec {
   int rows_per_node;   // LDS rows per node of the main block (1 or 3)
   int comps_per_col;   // values per neighbour node per row (1 or 3)
@@ -170,19 +176,25 @@ struct PlanSpec {
   const Mesh* cols_main;    // mesh whose element nodes index `main`
   const Mesh* cols_second;  // mesh whose element nodes index `second`
   int lds_budget_doubles;
+  // false: no segments; the k-th pairs of all the chunk's nodes follow each other inside
+  // a class, a node's pairs may lie in several waves (their sums vary in the last bits
+  // from run to run), the vertex class is padded in the kernels only
+  bool segments = true;
 };
 
-#ifndef FEDD_PAIR_ORDER
-#define FEDD_PAIR_ORDER 0
-#endif
+constexpr int kSegment = 64;  // pairs per wave
 
 int build_plan(Ctx* c, const Incidence& inc, int64_t nnodes, const Mesh& rows_mesh,
                const PlanSpec& s, GatherPlan& plan) {
   std::vector<int32_t> chunk_node(1, 0), chunk_pair(1, 0), chunk_split;
-  int64_t npairs = inc.ptr[nnodes];
-  std::vector<int32_t> pair_elem(npairs);
-  std::vector<uint8_t> pair_meta(npairs * 16, 0);
+  const int64_t npairs = inc.ptr[nnodes];
+  std::vector<int32_t> pair_elem;
+  std::vector<uint8_t> pair_meta;
+  pair_elem.reserve(npairs + npairs / 4);
+  pair_meta.reserve(16 * (npairs + npairs / 4));
   int max_lds = 0;
+  bool ordered = s.segments;
+  std::vector<std::array<int64_t, 3>> order;  // (key, node, incidence entry)
   auto node_lds = [&](int64_t i) {
     int64_t v = (int64_t)s.rows_per_node * s.comps_per_col * s.main->deg(i);
     if (s.second) v += (int64_t)s.second_rows * s.second->deg(i);
@@ -194,9 +206,36 @@ int build_plan(Ctx* c, const Incidence& inc, int64_t nnodes, const Mesh& rows_me
     if (node_lds(i) > s.lds_budget_doubles)
       return fail(c, FEDD_ERR_LIMIT, "a single row block exceeds the LDS budget");
   }
+  auto pad_to = [&](int64_t q0, int64_t fill_to) {  // padding pairs up to chunk offset fill_to
+    while ((int64_t)pair_elem.size() - q0 < fill_to) {
+      pair_elem.push_back(-1);
+      pair_meta.insert(pair_meta.end(), 16, (uint8_t)0);
+    }
+  };
   int64_t n0 = 0;
-  int64_t pair_out = 0;
-  std::vector<std::array<int64_t, 3>> order;  // (key, node, q)
+  auto emit = [&](int64_t i, int64_t q) {  // pair (node i, incidence entry q) of the chunk at n0
+    const int li = inc.loc[q];
+    const int32_t e = inc.elem[q];
+    pair_elem.push_back(e);
+    uint8_t m[16] = {0};
+    m[0] = (uint8_t)li;
+    m[1] = (uint8_t)(i - n0);
+    // the relabelling acts on the column element: P2 columns use the tet-10
+    // permutation (its vertex part is the tet-4 one), P1 columns the tet-4 one
+    const int ncol = s.cols_main->npe;
+    const int8_t* perm = ncol == 10 ? kPermP2[li] : kPermP1[li];
+    for (int cpos = 0; cpos < ncol; ++cpos) {
+      int32_t j = s.cols_main->elems[(int64_t)e * ncol + perm[cpos]];
+      m[2 + cpos] = (uint8_t)find_slot(*s.main, i, j);
+    }
+    if (s.second) {
+      for (int cpos = 0; cpos < 4; ++cpos) {
+        int32_t j = s.cols_second->elems[(int64_t)e * 4 + perm[cpos]];
+        m[12 + cpos] = (uint8_t)find_slot(*s.second, i, j);
+      }
+    }
+    pair_meta.insert(pair_meta.end(), m, m + 16);
+  };
   while (n0 < nnodes) {
     int64_t n1 = n0, lds = 0;
     while (n1 < nnodes && n1 - n0 < 256 && lds + node_lds(n1) <= s.lds_budget_doubles &&
@@ -206,56 +245,66 @@ int build_plan(Ctx* c, const Incidence& inc, int64_t nnodes, const Mesh& rows_me
     }
     if (n1 == n0) n1 = n0 + 1;  // cannot happen given the checks above
     max_lds = std::max<int64_t>(max_lds, lds);
-    order.clear();
-    for (int64_t i = n0; i < n1; ++i) {
-      int rank[2] = {0, 0};
-      for (int64_t q = inc.ptr[i]; q < inc.ptr[i + 1]; ++q) {
-        int li = inc.loc[q];
-        int cls = (rows_mesh.npe == 10 && li >= 4) ? 1 : 0;
-        // class first; inside a class the k-th pairs of all nodes follow each
-        // other, so neighbouring lanes add into different rows
-        // (FEDD_PAIR_ORDER 1: by element inside a class, lanes share element data)
-        int64_t key = FEDD_PAIR_ORDER == 1 ? (((int64_t)cls << 60) | ((int64_t)inc.elem[q] << 20) | (i - n0))
-                                           : (((int64_t)cls << 60) | ((int64_t)rank[cls]++ << 20) | (i - n0));
-        order.push_back({key, i, q});
-      }
-    }
-    std::sort(order.begin(), order.end());
+    const int64_t q0 = (int64_t)pair_elem.size();
     int32_t split = -1;
-    for (size_t k = 0; k < order.size(); ++k) {
-      int64_t i = order[k][1], q = order[k][2];
-      int cls = (int)(order[k][0] >> 60);
-      if (cls == 1 && split < 0) split = (int32_t)(pair_out);
-      int32_t e = inc.elem[q];
-      int li = inc.loc[q];
-      pair_elem[pair_out] = e;
-      uint8_t* m = &pair_meta[pair_out * 16];
-      m[0] = (uint8_t)li;
-      m[1] = (uint8_t)(i - n0);
-      // the relabelling acts on the column element: P2 columns use the tet-10
-      // permutation (its vertex part is the tet-4 one), P1 columns the tet-4 one
-      int ncol = s.cols_main->npe;
-      const int8_t* perm = ncol == 10 ? kPermP2[li] : kPermP1[li];
-      for (int cpos = 0; cpos < ncol; ++cpos) {
-        int32_t j = s.cols_main->elems[(int64_t)e * ncol + perm[cpos]];
-        m[2 + cpos] = (uint8_t)find_slot(*s.main, i, j);
+    for (int cls = 0; cls < 2; ++cls) {
+      if (cls == 1) {
+        // the edge class starts on a segment boundary (when the chunk has one)
+        bool any = false;
+        for (int64_t q = inc.ptr[n0]; q < inc.ptr[n1] && !any; ++q) any = rows_mesh.npe == 10 && inc.loc[q] >= 4;
+        const int64_t used = (int64_t)pair_elem.size() - q0;
+        if (any && s.segments) pad_to(q0, (used + kSegment - 1) / kSegment * kSegment);
+        split = (int32_t)pair_elem.size();
       }
-      if (s.second) {
-        for (int cpos = 0; cpos < 4; ++cpos) {
-          int32_t j = s.cols_second->elems[(int64_t)e * 4 + perm[cpos]];
-          m[12 + cpos] = (uint8_t)find_slot(*s.second, i, j);
+      // (key, node, incidence entry) of this class, in emission order
+      order.clear();
+      for (int64_t i = n0; i < n1; ++i) {
+        int64_t g = 0;
+        for (int64_t q = inc.ptr[i]; q < inc.ptr[i + 1]; ++q)
+          if (((rows_mesh.npe == 10 && inc.loc[q] >= 4) ? 1 : 0) == cls)
+            order.push_back({(g << 20) | (i - n0), i, q}), ++g;
+      }
+      if (!s.segments) {
+        // the k-th pairs of all nodes follow each other, so neighbouring lanes add into
+        // different rows and gather neighbouring elements
+        std::sort(order.begin(), order.end());
+        for (auto& o : order) emit(o[1], o[2]);
+        continue;
+      }
+      // segments: whole nodes per segment, interleaved inside it like the order above
+      size_t k = 0;
+      while (k < order.size()) {
+        const int64_t used = (int64_t)pair_elem.size() - q0;
+        std::vector<std::array<int64_t, 3>> seg;
+        int64_t fill = used % kSegment;
+        while (k < order.size()) {
+          size_t g = 0;  // pairs of node order[k][1]
+          while (k + g < order.size() && order[k + g][1] == order[k][1]) ++g;
+          if (fill > 0 && fill + (int64_t)g > kSegment) break;  // starts the next segment
+          if ((int64_t)g > kSegment) ordered = false;  // spans segments: its rows race between waves
+          seg.insert(seg.end(), order.begin() + k, order.begin() + k + g);
+          fill += (int64_t)g;
+          k += g;
+          if (fill >= kSegment) break;
+        }
+        std::sort(seg.begin(), seg.end());
+        for (auto& o : seg) emit(o[1], o[2]);
+        if (k < order.size()) {
+          const int64_t u = (int64_t)pair_elem.size() - q0;
+          if (u % kSegment) pad_to(q0, u - u % kSegment + kSegment);
         }
       }
-      ++pair_out;
     }
-    if (split < 0) split = (int32_t)pair_out;
     chunk_split.push_back(split);
     chunk_node.push_back((int32_t)n1);
-    chunk_pair.push_back((int32_t)pair_out);
+    chunk_pair.push_back((int32_t)pair_elem.size());
     n0 = n1;
   }
+  if (pair_elem.size() > (size_t)std::numeric_limits<int32_t>::max())
+    return fail(c, FEDD_ERR_LIMIT, "the row-gather plan holds more than 2^31 pairs");
+  plan.ordered = ordered;
   plan.nchunks = (int64_t)chunk_split.size();
-  plan.npairs = npairs;
+  plan.npairs = (int64_t)pair_elem.size();
   plan.lds_doubles = max_lds;
   int rc;
   if ((rc = upload(c, plan.d_chunk_node, chunk_node.data(), chunk_node.size() * 4))) return rc;
@@ -263,6 +312,7 @@ int build_plan(Ctx* c, const Incidence& inc, int64_t nnodes, const Mesh& rows_me
   if ((rc = upload(c, plan.d_chunk_split, chunk_split.data(), chunk_split.size() * 4))) return rc;
   if ((rc = upload(c, plan.d_pair_elem, pair_elem.data(), pair_elem.size() * 4))) return rc;
   if ((rc = upload(c, plan.d_pair_meta, pair_meta.data(), pair_meta.size()))) return rc;
+  FEDD_HIP(c, hipStreamSynchronize(c->stream));  // the host arrays are freed on return
   return FEDD_OK;
 }
 
@@ -591,6 +641,9 @@ int fedd_graph_build(fedd_ctx* c, int kind) {
     vs.rows_per_node = 3;
     vs.comps_per_col = 3;
   }
+  // segments when reproducible sums are asked for (fedd_set_reproducible); the
+  // hyperelastic row gather keeps the rank-interleaved order
+  vs.segments = c->reproducible && kind != FEDD_GRAPH_VECTOR;
   if (kind == FEDD_GRAPH_NS) {
     vs.second = &g.bt;
     vs.second_rows = 3;
@@ -607,6 +660,7 @@ int fedd_graph_build(fedd_ctx* c, int kind) {
     ps.main = &g.b;
     ps.cols_main = &m;
     ps.lds_budget_doubles = kLdsBudget;
+    ps.segments = c->reproducible;
     if ((rc = build_plan(c, pinc, c->mesh[1].npts, c->mesh[1], ps, c->pplan))) {
       c->graph_kind = GK_NONE;
       return rc;
@@ -614,6 +668,12 @@ int fedd_graph_build(fedd_ctx* c, int kind) {
   }
   FEDD_HIP(c, hipStreamSynchronize(c->stream));
   c->newton_step = 0;
+  return FEDD_OK;
+}
+
+int fedd_set_reproducible(fedd_ctx* c, int on) {
+  if (!c || (on != 0 && on != 1)) return FEDD_ERR_ARG;
+  c->reproducible = on != 0;
   return FEDD_OK;
 }
 

@@ -80,9 +80,11 @@ struct Block {
 };
 
 // Row-gather schedule over node chunks (see fedd_ns.hip for the layout).
+// Pairs come in segments of 64 (one wave each); padding pairs have element -1.
 struct GatherPlan {
-  int64_t nchunks = 0, npairs = 0;
+  int64_t nchunks = 0, npairs = 0;  // npairs includes the padding pairs
   int lds_doubles = 0;          // max LDS doubles of any chunk
+  bool ordered = true;          // every node's pairs lie in one segment (reproducible sums)
   DevBuf d_chunk_node;          // int32 [nchunks+1]: first node of each chunk
   DevBuf d_chunk_pair;          // int32 [nchunks+1]: first pair of each chunk
   DevBuf d_chunk_split;         // int32 [nchunks]: first edge-class pair of each chunk
@@ -108,6 +110,7 @@ struct Ctx {
   DevBuf d_nptr, d_pptr, d_bptr;  // int64 node-level row pointers (A, B^T, B)
   DevBuf d_nidx;                  // int32 node-level column indices of A (node blocks)
   GatherPlan vplan, pplan;        // velocity (primary) and pressure row gathers
+  bool reproducible = false;      // fedd_set_reproducible: segmented plans at the next graph build
   DevBuf d_emod;                  // per-element Young's modulus field (E-field elasticity overload)
   DevBuf d_qp;                    // elasticity quadrature-point records (52 doubles each)
   DevBuf d_rhsq;                  // assemblyRHS: quadrature points / source values at them

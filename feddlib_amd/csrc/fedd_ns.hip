@@ -524,6 +524,7 @@ __global__ __launch_bounds__(FEDD_ROW_BLOCK) __attribute__((amdgpu_waves_per_eu(
       if (vert && t >= nv) continue;
       const int q = vert ? q0 + t : qs + (t - nvpad);
       const int32_t e = a.pair_elem[q];
+      if (e < 0) continue;  // padding pair
       const uint4 mv = a.pair_meta[q];
       const uint8_t* m = reinterpret_cast<const uint8_t*>(&mv);
       const int4 nd = tab[m[1]];
@@ -567,6 +568,7 @@ __global__ __launch_bounds__(256) void k_ns_pres_rows(NsArgs a) {
   __syncthreads();
   for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
     const int32_t e = a.pair_elem[q];
+    if (e < 0) continue;  // padding pair
     const uint4 mv = a.pair_meta[q];
     const uint8_t* m = reinterpret_cast<const uint8_t*>(&mv);
     const int li = m[0];
@@ -679,6 +681,7 @@ __global__ __launch_bounds__(256) void k_ns_res_vel(NsArgs a) {
   __syncthreads();
   for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
     const int32_t e = a.pair_elem[q];
+    if (e < 0) continue;  // padding pair
     const uint4 mv = a.pair_meta[q];
     const uint8_t* m = reinterpret_cast<const uint8_t*>(&mv);
     double r[3];
@@ -706,6 +709,7 @@ __global__ __launch_bounds__(256) void k_ns_res_pres(NsArgs a) {
   __syncthreads();
   for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
     const int32_t e = a.pair_elem[q];
+    if (e < 0) continue;  // padding pair
     const uint4 mv = a.pair_meta[q];
     const uint8_t* m = reinterpret_cast<const uint8_t*>(&mv);
     const int li = m[0];
@@ -807,6 +811,7 @@ __global__ __launch_bounds__(256) void k_laplace_rows(LapArgs a) {
   __syncthreads();
   for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
     const int32_t e = a.pair_elem[q];
+    if (e < 0) continue;  // padding pair
     const uint4 mv = a.pair_meta[q];
     const uint8_t* m = reinterpret_cast<const uint8_t*>(&mv);
     double* row = lds + (a.nptr[n0 + m[1]] - na0);
@@ -887,6 +892,7 @@ __global__ __launch_bounds__(256) void k_linelas_rows(LapArgs a, double lam, dou
   __syncthreads();
   for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
     const int32_t e = a.pair_elem[q];
+    if (e < 0) continue;  // padding pair
     const uint4 mv = a.pair_meta[q];
     const uint8_t* m = reinterpret_cast<const uint8_t*>(&mv);
     const int2 nd = tab[m[1]];
@@ -911,6 +917,7 @@ __global__ __launch_bounds__(256) void k_rhs_rows(LapArgs a) {
   __syncthreads();
   for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
     const int32_t e = a.pair_elem[q];
+    if (e < 0) continue;  // padding pair
     const uint4 mv = a.pair_meta[q];
     const uint8_t* m = reinterpret_cast<const uint8_t*>(&mv);
     const int li = m[0];
@@ -1015,6 +1022,7 @@ __global__ __launch_bounds__(256) void k_rhs_points_rows(LapArgs a, const double
   __syncthreads();
   for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
     const int32_t e = a.pair_elem[q];
+    if (e < 0) continue;  // padding pair
     const uint4 mv = a.pair_meta[q];
     const uint8_t* m = reinterpret_cast<const uint8_t*>(&mv);
     const int li = m[0];

@@ -223,11 +223,12 @@ __global__ __launch_bounds__(256) void k_lincomb(const double* __restrict__ V, i
   }
 }
 
-// y = a x + b y  (x, y of length n)
+// y = a x + b y  (x, y of length n).  b = 0: y = a x without reading y, which may be
+// fresh device memory whose bits are not a finite number (0 * NaN is NaN).
 __global__ __launch_bounds__(256) void k_axpby(int64_t n, double a, const double* __restrict__ x, double b,
                                                double* __restrict__ y) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i < n) y[i] = a * x[i] + b * y[i];
+  if (i < n) y[i] = b == 0.0 ? a * x[i] : a * x[i] + b * y[i];
 }
 
 __global__ __launch_bounds__(256) void k_zero(int64_t n, double* __restrict__ y) {

@@ -158,6 +158,7 @@ _SIGS = {
     "fedd_event_elapsed": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int,
                                           ctypes.POINTER(ctypes.c_float)]),
     "fedd_kernel_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "fedd_set_reproducible": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fedd_kernel_time": (ctypes.c_int, [_vp, ctypes.c_char_p, _dp, _lp]),
 }
 
@@ -296,6 +297,10 @@ class Context:
         elems = np.ascontiguousarray(elems, np.int32)
         self._chk(self._L.fedd_mesh_upload(self.h, slot, _p(pts, _dp), pts.shape[0], _p(elems, _ip),
                                            elems.shape[0], elems.shape[1]))
+
+    def set_reproducible(self, on=True):
+        """Bitwise-repeatable row-gather sums for graphs built afterwards (see the header)."""
+        self._chk(self._L.fedd_set_reproducible(self.h, 1 if on else 0))
 
     def graph_build(self, kind):
         self._chk(self._L.fedd_graph_build(self.h, kind))

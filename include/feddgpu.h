@@ -135,6 +135,16 @@ int fedd_mesh_upload(fedd_ctx* ctx, int slot, const double* pts, int64_t npts,
  * assemblyLaplace), plus the node incidence and scatter indices the kernels use.
  * Rows never hold more than 255 distinct neighbour nodes (FEDD_ERR_LIMIT). */
 int fedd_graph_build(fedd_ctx* ctx, int kind);
+/* Summation order of the row gathers.  No counterpart in the reference, which sums an
+ * entry's element contributions in a fixed order (Tpetra fillComplete), so its matrices
+ * repeat bit for bit.  The GPU row gathers (on = 0, the default) add them in the order in
+ * which a workgroup's waves reach LDS: repeated assemblies agree to the parity rules but
+ * not always in the last bits.  on = 1: the Navier-Stokes velocity and pressure plans and
+ * the scalar (Laplace, load vector) plans place all of a node's pairs in one wave, so
+ * repeated assemblies of the same state give identical bits; the NS velocity rows then
+ * take about 15% longer (40^3 x 6).  The vector-graph (elasticity) plan is not changed.
+ * Applies to graphs built after the call. */
+int fedd_set_reproducible(fedd_ctx* ctx, int on);
 int fedd_block_info(fedd_ctx* ctx, int block, int64_t* nrows, int64_t* ncols, int64_t* nnz);
 /* Host copies: rowptr has nrows+1 entries, colind nnz (local = global ids on one rank). */
 int fedd_block_graph(fedd_ctx* ctx, int block, int64_t* rowptr, int32_t* colind);
