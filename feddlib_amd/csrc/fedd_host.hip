@@ -176,6 +176,7 @@ struct PlanSpec {
   const Mesh* cols_main;    // mesh whose element nodes index `main`
   const Mesh* cols_second;  // mesh whose element nodes index `second`
   int lds_budget_doubles;
+  int max_nodes = 256;  // most nodes per chunk
   // false: no segments; the k-th pairs of all the chunk's nodes follow each other inside
   // a class, a node's pairs may lie in several waves (their sums vary in the last bits
   // from run to run), the vertex class is padded in the kernels only
@@ -238,7 +239,7 @@ int build_plan(Ctx* c, const Incidence& inc, int64_t nnodes, const Mesh& rows_me
   };
   while (n0 < nnodes) {
     int64_t n1 = n0, lds = 0;
-    while (n1 < nnodes && n1 - n0 < 256 && lds + node_lds(n1) <= s.lds_budget_doubles &&
+    while (n1 < nnodes && n1 - n0 < s.max_nodes && lds + node_lds(n1) <= s.lds_budget_doubles &&
            (inc.ptr[n1 + 1] - inc.ptr[n0]) <= 1024) {
       lds += node_lds(n1);
       ++n1;
@@ -645,6 +646,9 @@ int fedd_graph_build(fedd_ctx* c, int kind) {
   // hyperelastic row gather keeps the rank-interleaved order
   vs.segments = c->reproducible && kind != FEDD_GRAPH_VECTOR;
   if (kind == FEDD_GRAPH_NS) {
+    // the velocity rows' chunks are sized for k_ns_vel_rows (its LDS table and occupancy)
+    vs.lds_budget_doubles = kNsLdsBudget;
+    vs.max_nodes = kNsChunkNodes;
     vs.second = &g.bt;
     vs.second_rows = 3;
     vs.cols_second = &c->mesh[1];

@@ -94,6 +94,23 @@ struct GatherPlan {
 
 enum GraphKind { GK_NONE = -1 };
 
+// Navier-Stokes velocity row gather (k_ns_vel_rows): LDS budget of a chunk's rows in
+// doubles, most nodes per chunk (the size of its per-node table), threads per workgroup.
+// 36 KiB of rows + a 1 KiB table: four workgroups of three waves fit a CU's 160 KiB of
+// LDS and its 12 waves of 168 VGPRs (three waves per SIMD).
+#ifndef FEDD_NS_LDS_BUDGET
+#define FEDD_NS_LDS_BUDGET 4608
+#endif
+#ifndef FEDD_NS_CHUNK_NODES
+#define FEDD_NS_CHUNK_NODES 64
+#endif
+#ifndef FEDD_NS_ROW_THREADS
+#define FEDD_NS_ROW_THREADS 192
+#endif
+constexpr int kNsLdsBudget = FEDD_NS_LDS_BUDGET;
+constexpr int kNsChunkNodes = FEDD_NS_CHUNK_NODES;
+constexpr int kNsRowThreads = FEDD_NS_ROW_THREADS;
+
 struct Ctx {
   int device = 0;
   int num_cu = 256;  // compute units of the device (hipDeviceProp_t::multiProcessorCount)

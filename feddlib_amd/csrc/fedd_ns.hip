@@ -24,11 +24,8 @@
 #include <algorithm>
 #include <type_traits>
 
-#ifndef FEDD_ROW_BLOCK
-#define FEDD_ROW_BLOCK 256
-#endif
 #ifndef FEDD_ROW_WAVES
-#define FEDD_ROW_WAVES 2
+#define FEDD_ROW_WAVES 3
 #endif
 #ifndef FEDD_ROW_BLOCKS_PER_CU
 #define FEDD_ROW_BLOCKS_PER_CU 0  // 0: one workgroup per chunk
@@ -483,9 +480,9 @@ __device__ __forceinline__ void block_chunks(int64_t nchunks, int64_t& c0, int64
 #define FEDD_ROW_WAVES_RES 3
 #endif
 template <typename T, int RES>
-__global__ __launch_bounds__(FEDD_ROW_BLOCK) __attribute__((amdgpu_waves_per_eu(RES ? FEDD_ROW_WAVES_RES : FEDD_ROW_WAVES))) void k_ns_vel_rows(NsArgs a) {
+__global__ __launch_bounds__(kNsRowThreads) __attribute__((amdgpu_waves_per_eu(RES ? FEDD_ROW_WAVES_RES : FEDD_ROW_WAVES))) void k_ns_vel_rows(NsArgs a) {
   extern __shared__ double lds[];
-  __shared__ int4 tab[256];  // per chunk node: A-row offset, deg, B^T-row offset, pdeg
+  __shared__ int4 tab[kNsChunkNodes];  // per chunk node: A-row offset, deg, B^T-row offset, pdeg
   int64_t c0, c1;
   block_chunks(a.nchunks, c0, c1);
   for (int64_t ch = c0; ch < c1; ++ch) {
@@ -1119,7 +1116,7 @@ int launch_ns_assembly(Ctx* c, const fedd_ns_params* p, int mode, int reassemble
     NsArgs a = ns_args(c, &q, c->vplan);
     a.with_bt = reassemble ? 0 : 1;
     // fused: 3 residual slots per chunk node after the rows (at most 256 nodes per chunk)
-    const size_t lds = (size_t)c->vplan.lds_doubles * 8 + (fused ? 3 * 256 * 8 : 0);
+    const size_t lds = (size_t)c->vplan.lds_doubles * 8 + (fused ? 3 * kNsChunkNodes * 8 : 0);
     // FEDD_ROW_BLOCKS_PER_CU > 0: that many workgroups per CU walk the chunks
     // (measured slower on MI355X than one workgroup per chunk)
     const unsigned grid = (unsigned)(FEDD_ROW_BLOCKS_PER_CU > 0 ? std::min<int64_t>(c->vplan.nchunks,
@@ -1155,14 +1152,14 @@ int launch_ns_assembly(Ctx* c, const fedd_ns_params* p, int mode, int reassemble
     ktimer_begin(c);
     if (fused) {
       if (f32)
-        k_ns_vel_rows<float, 1><<<grid, FEDD_ROW_BLOCK, lds, c->stream>>>(a);
+        k_ns_vel_rows<float, 1><<<grid, kNsRowThreads, lds, c->stream>>>(a);
       else
-        k_ns_vel_rows<double, 1><<<grid, FEDD_ROW_BLOCK, lds, c->stream>>>(a);
+        k_ns_vel_rows<double, 1><<<grid, kNsRowThreads, lds, c->stream>>>(a);
     } else {
       if (f32)
-        k_ns_vel_rows<float, 0><<<grid, FEDD_ROW_BLOCK, lds, c->stream>>>(a);
+        k_ns_vel_rows<float, 0><<<grid, kNsRowThreads, lds, c->stream>>>(a);
       else
-        k_ns_vel_rows<double, 0><<<grid, FEDD_ROW_BLOCK, lds, c->stream>>>(a);
+        k_ns_vel_rows<double, 0><<<grid, kNsRowThreads, lds, c->stream>>>(a);
     }
     FEDD_HIP(c, hipGetLastError());
     ktimer_end(c, "ns_vel_rows");
