@@ -302,13 +302,22 @@ def _unstructured_ns(M, seed=5, curved=False):
     return p2n, e2n, p1n, e1n, u, p
 
 
-@pytest.mark.parametrize("curved", [False, True])
-def test_ns_unstructured_mesh(gpu_ctx, curved):
+@pytest.mark.parametrize("curved,reproducible", [(False, False), (True, False), (True, True)])
+def test_ns_unstructured_mesh(gpu_ctx, curved, reproducible):
     """Every mode on a mesh that is not the lattice: the row gather's relabelling and
     exact tensors hold in any vertex order and numbering.  FEDDLib maps every element
     affinely from its four vertices (Helper::buildTransformation, Helper.cpp:196-222), also
-    when the edge nodes are off the midpoints, and so do the row kernels."""
+    when the edge nodes are off the midpoints, and so do the row kernels.  Also with the
+    reproducible (segmented) plans, whose node groups have irregular sizes on this mesh."""
     p2, e2, p1, e1, u, p = _unstructured_ns(3, curved=curved)
+    gpu_ctx.set_reproducible(reproducible)
+    try:
+        _unstructured_modes(gpu_ctx, p2, e2, p1, e1, u, p)
+    finally:
+        gpu_ctx.set_reproducible(False)
+
+
+def _unstructured_modes(gpu_ctx, p2, e2, p1, e1, u, p):
     gpu_ctx.mesh_upload(N.MESH_PRIMARY, p2, e2)
     gpu_ctx.mesh_upload(N.MESH_PRESSURE, p1, e1)
     gpu_ctx.graph_build(N.GRAPH_NS)
