@@ -492,8 +492,9 @@ def newton_leg(ctx, N, M, exact_only=False):
     # HBM bytes of the fine-level passes each iteration makes (a lower bound: the coarse
     # levels of the hierarchy, 1/14 of the fine matrix at this size, and their Galerkin
     # rebuilds are not counted):
-    #   CG iteration: block SpMV over the FP64 matrix (8 B per value, 4 B column per 3x3
-    #     block, p read and A p written) and the two vector kernels (x, r, z, p and A p
+    #   CG iteration: block SpMV over the matrix (block-Jacobi CG: FP64, 8 B per value;
+    #     multigrid CG inside Newton: the smoother's FP32 copy, 4 B per value; 4 B column per
+    #     3x3 block, p read and A p written) and the two vector kernels (x, r, z, p and A p
     #     read or written, the 3x3 block inverse read by the update): 112 B per dof;
     #   V-cycle (multigrid CG): two passes over the FP32 copy of the fine matrix (residual
     #     after the sweep from zero, post-smoothing sweep; 4 B per value, 4 B column per
@@ -503,11 +504,12 @@ def newton_leg(ctx, N, M, exact_only=False):
     #     matrix (8 B read, 4 B written per value).
     n = 3 * nnode
     spmv_bytes = 8 * nnz + 4 * nnz // 9 + 112 * n
+    spmv32_bytes = 4 * nnz + 4 * nnz // 9 + 112 * n
     vcycle_bytes = 2 * (4 * nnz + 4 * nnz // 9) + 48 * n
     asm_bytes = 8 * nnz + 4 * 10 * nelem + 24 * nnode + 8 * n + 8 * n
 
     def frac(st, mg=True):
-        b = st["krylov_iters"] * (spmv_bytes + (vcycle_bytes if mg else 0)) + \
+        b = st["krylov_iters"] * ((spmv32_bytes + vcycle_bytes) if mg else spmv_bytes) + \
             st["newton_iters"] * (asm_bytes + (12 * nnz if mg else 0))
         return b / st["seconds"] / 1e9 / HBM_PEAK_GBS
     nqp = 5 * nelem
@@ -515,13 +517,14 @@ def newton_leg(ctx, N, M, exact_only=False):
     return {
         "workload": "Neo-Hookean (skr2) Newton, unit cube %d^3x6 = %d P2 tets, %d dofs, z=0 clamped, traction "
                     "(40,0,60) on z=1 in 5 load steps, ||r||_2 < 1e-8, CG preconditioned by a geometric "
-                    "multigrid V-cycle (Galerkin P2->P1->coarser P1; coarse levels kept until the CG count grows, finest level in FP32 for smoothing), Eisenstat-Walker forcing; timing includes the multigrid setup" % (M, nelem, n),
+                    "multigrid V-cycle (Galerkin P2->P1->coarser P1; coarse levels kept until the CG count grows, finest level in FP32 for smoothing and for the CG products), Eisenstat-Walker forcing; timing includes the multigrid setup" % (M, nelem, n),
         "metric": "Neo-Hookean Newton time-to-1e-8, MLP vs exact tangent",
         "unit": "s", "higher_is_better": False,
-        "roofline_note": "fine-level passes only (lower bound): per CG iteration one FP64 block SpMV "
-                         "(%d B) + multigrid V-cycle with two FP32 matrix passes (%d B), per Newton step "
-                         "one assembly (%d B) + FP32 matrix copy; bytes / seconds / 8 TB/s"
-                         % (spmv_bytes, vcycle_bytes, asm_bytes),
+        "roofline_note": "fine-level passes only (lower bound): per CG iteration one block SpMV over the "
+                         "FP32 copy of the matrix (%d B; block-Jacobi CG: FP64, %d B) + multigrid V-cycle "
+                         "with two FP32 matrix passes (%d B), per Newton step one assembly (%d B) + FP32 "
+                         "matrix copy; bytes / seconds / 8 TB/s"
+                         % (spmv32_bytes, spmv_bytes, vcycle_bytes, asm_bytes),
         "exact": dict({k: st_e[k] for k in keys}, roofline_frac=frac(st_e)),
         "exact_block_jacobi_cg": dict({k: st_bj[k] for k in keys}, roofline_frac=frac(st_bj, mg=False),
                                       solution_max_rel_diff=float(np.max(np.abs(u_bj - u_e)) / np.max(np.abs(u_e)))),

@@ -254,6 +254,9 @@ int mg_refresh(Ctx* c);
 // Finest level only (smoother blocks and FP32 copy); coarse levels keep their operators.
 int mg_refresh_fine(Ctx* c);
 int mg_vcycle(Ctx* c, const double* b, double* x);
+// FP32 copy of the finest operator (block A's values, same order) that the smoother reads,
+// as of the last mg_refresh / mg_refresh_fine; null when the hierarchy keeps none.
+const float* mg_fine_vals32(Ctx* c);
 void mg_free(Ctx* c);
 // Looks up this device's copy of the no-allocation solve's work array (fedd_solve.hip).
 void noalloc_init(Ctx* c);

@@ -1295,6 +1295,11 @@ int mg_refresh_fine(Ctx* c) {
   return FEDD_OK;
 }
 
+const float* mg_fine_vals32(Ctx* c) {
+  if (!c->mg || c->mg->lv.empty()) return nullptr;
+  return c->mg->lv[0].vals32.as<float>();
+}
+
 // x = V-cycle(b) on the finest level (3 dofs per node).
 int mg_vcycle(Ctx* c, const double* b, double* x) {
   MgHost* g = c->mg;

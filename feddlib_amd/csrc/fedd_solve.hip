@@ -188,10 +188,11 @@ __device__ __forceinline__ double sum16(const double* p) {
 // node i are contiguous (row 3i+d starts at 9 nptr[i] + 3 deg_i d) and entry q of a
 // row multiplies x[3 nidx[nptr[i] + q/3] + q%3].  G lanes per node walk each row
 // with unit stride (coalesced value reads); x.y is added to *dot.
-template <int G>
+// V = float: the operator's FP32 copy (products and sums stay FP64).
+template <int G, typename V = double>
 __global__ __launch_bounds__(256) void k_bspmv_dot(int64_t nnode, const int64_t* __restrict__ nptr,
                                                    const int32_t* __restrict__ nidx,
-                                                   const double* __restrict__ vals, const double* __restrict__ x,
+                                                   const V* __restrict__ vals, const double* __restrict__ x,
                                                    double* __restrict__ y, double* __restrict__ dot) {
   const int lane = threadIdx.x % G;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x / G;
@@ -203,13 +204,13 @@ __global__ __launch_bounds__(256) void k_bspmv_dot(int64_t nnode, const int64_t*
     if (i < nnode) {
       const int64_t k0 = nptr[i];
       const int L = 3 * (int)(nptr[i + 1] - k0);
-      const double* v = vals + 9 * k0;
+      const V* v = vals + 9 * k0;
       const int32_t* ni = nidx + k0;
       for (int q = lane; q < L; q += G) {
         const int k = (q * 0xAAAB) >> 17;  // q / 3 for q < 2^15
         const double xv = x[3 * (int64_t)ni[k] + (q - 3 * k)];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) s[d] += v[d * L + q] * xv;
+        for (int d = 0; d < 3; ++d) s[d] += (double)v[d * L + q] * xv;
       }
     }
 #pragma unroll
@@ -551,8 +552,11 @@ void noalloc_init(Ctx* c) {
 // synchronisation.  The convergence test reads r.r of iteration t-1 from pinned host
 // memory while iteration t is already queued (at most one iteration beyond the target).
 // refresh: 0 keep the hierarchy, 1 rebuild it for the current A, 2 finest level only.
+#ifndef FEDD_CG_FP32_SPMV
+#define FEDD_CG_FP32_SPMV 1
+#endif
 int pcg_mg_solve(Ctx* c, double rtol, double atol, int maxit, int* iters, double* relres, int refresh,
-                 double bnorm_known, int mode) {
+                 double bnorm_known, int mode, int fp32_spmv) {
   const Block& A = c->blk[0];
   const int64_t n = A.nrows, nnode = n / 3;
   int rc;
@@ -607,10 +611,20 @@ int pcg_mg_solve(Ctx* c, double rtol, double atol, int maxit, int* iters, double
   double rnorm = bnorm;
   int it = 0;
   const double target = std::max(rtol * bnorm, atol);
+  // fp32_spmv (Newton steps): the CG products read the FP32 copy of A that the smoother
+  // reads (just refreshed), half the bytes of the iteration's largest pass.  The Krylov
+  // solve then targets A rounded to FP32, so the step's true residual carries an extra
+  // O(2^-24 cond(A)) relative term on top of the forcing term: an inexact-Newton step
+  // still, while Newton's residual stays exact FP64.  Plain linear solves keep FP64.
+  const float* va32 = FEDD_CG_FP32_SPMV && fp32_spmv && refresh != 0 ? mg_fine_vals32(c) : nullptr;
   if (bnorm > 0.0 && rnorm > target) {
     while (it < maxit) {
       const int t = it;
-      k_bspmv_dot<FEDD_SPMV_LANES><<<g_spmv, 256, 0, c->stream>>>(nnode, nptr, nidx, va, p, Ap, scal + PCG_PAP(t));
+      if (va32)
+        k_bspmv_dot<FEDD_SPMV_LANES, float><<<g_spmv, 256, 0, c->stream>>>(nnode, nptr, nidx, va32, p, Ap,
+                                                                          scal + PCG_PAP(t));
+      else
+        k_bspmv_dot<FEDD_SPMV_LANES><<<g_spmv, 256, 0, c->stream>>>(nnode, nptr, nidx, va, p, Ap, scal + PCG_PAP(t));
       k_pcg_xr<<<g_red, 256, 0, c->stream>>>(n, x, r, p, Ap, scal, t);
       if ((rc = mg_vcycle(c, r, z))) return rc;
       k_pcg_dot<<<g_red, 256, 0, c->stream>>>(n, r, z, scal, t, 0);
@@ -972,7 +986,7 @@ int fedd_linear_solve(fedd_ctx* cc, double rtol, int maxit, int* iters, double* 
   if (c->graph_kind != FEDD_GRAPH_VECTOR) return fail(c, FEDD_ERR_STATE, "needs a vector graph");
   if (c->vec_n[FEDD_VEC_B] != c->blk[0].nrows) return fail(c, FEDD_ERR_STATE, "right-hand side not set");
   (void)hipSetDevice(c->device);
-  if (c->precond >= 1) return pcg_mg_solve(c, rtol, 0.0, maxit, iters, relres, 1, 0.0, c->precond - 1);
+  if (c->precond >= 1) return pcg_mg_solve(c, rtol, 0.0, maxit, iters, relres, 1, 0.0, c->precond - 1, 0);
   return pcg_solve(c, rtol, 0.0, maxit, iters, relres);
 }
 
@@ -1143,7 +1157,7 @@ int fedd_newton_elasticity(fedd_ctx* cc, const fedd_newton_params* P, fedd_newto
       if (press)  // A + Kext is not symmetric
         rc = pbicgstab_solve(c, eta, 0.0, P->lin_maxit, &iters, &relres, P->preconditioner, refresh);
       else if (P->preconditioner >= 1)  // b = -R, ||b|| = norm
-        rc = pcg_mg_solve(c, eta, 0.0, P->lin_maxit, &iters, &relres, refresh, norm, P->preconditioner - 1);
+        rc = pcg_mg_solve(c, eta, 0.0, P->lin_maxit, &iters, &relres, refresh, norm, P->preconditioner - 1, 1);
       else
         rc = pcg_solve(c, eta, 0.0, P->lin_maxit, &iters, &relres);
       if (rc) return rc;
