@@ -291,21 +291,23 @@ __device__ __forceinline__ void elas_point(const ElasArgs& a, int32_t e, const i
   }
   // Physical basis gradients from the FP64 inverse Jacobian, rounded once to T: on
   // distorted (curved) elements an FP32 inverse Jacobian costs the FP32-local tangent
-  // its 1e-5 bound.
-  T G[10][3];
-#pragma unroll
-  for (int c = 0; c < 10; ++c)
+  // its 1e-5 bound.  Each is formed where it is used (the row's here, the columns' in
+  // the column loop), so that the ten gradients are not all live at once.
+  auto grad = [&](int c, T G[3]) {
 #pragma unroll
     for (int A = 0; A < 3; ++A) {
       double v = 0.0;
 #pragma unroll
       for (int ax = 0; ax < 3; ++ax)
         if (dphi[c][ax] != 0.0) v += dphi[c][ax] * Jd[ax][A];
-      G[c][A] = (T)v;
+      G[A] = (T)v;
     }
-  T g[3];
+  };
+  T g[3], g4[3];
+  grad(0, g);
+  grad(4, g4);
 #pragma unroll
-  for (int A = 0; A < 3; ++A) g[A] = r == 0 ? G[0][A] : G[4][A];
+  for (int A = 0; A < 3; ++A) g[A] = r == 0 ? g[A] : g4[A];
   const double wdet = rec[36];
   double Fd[3][3];
 #pragma unroll
@@ -367,17 +369,19 @@ __device__ __forceinline__ void elas_point(const ElasArgs& a, int32_t e, const i
   Sg[2] = w * ((T)Sd[4] * g[0] + (T)Sd[5] * g[1] + (T)Sd[2] * g[2]);
 #pragma unroll
   for (int c = 0; c < 10; ++c) {
-    const T geo = Sg[0] * G[c][0] + Sg[1] * G[c][1] + Sg[2] * G[c][2];
+    T Gc[3];
+    grad(c, Gc);
+    const T geo = Sg[0] * Gc[0] + Sg[1] * Gc[1] + Sg[2] * Gc[2];
     double* rc = row + 3 * mm[2 + c];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       T Bh[6];
-      Bh[0] = F[j][0] * G[c][0];
-      Bh[1] = F[j][1] * G[c][1];
-      Bh[2] = F[j][2] * G[c][2];
-      Bh[3] = F[j][0] * G[c][1] + F[j][1] * G[c][0];
-      Bh[4] = F[j][0] * G[c][2] + F[j][2] * G[c][0];
-      Bh[5] = F[j][1] * G[c][2] + F[j][2] * G[c][1];
+      Bh[0] = F[j][0] * Gc[0];
+      Bh[1] = F[j][1] * Gc[1];
+      Bh[2] = F[j][2] * Gc[2];
+      Bh[3] = F[j][0] * Gc[1] + F[j][1] * Gc[0];
+      Bh[4] = F[j][0] * Gc[2] + F[j][2] * Gc[0];
+      Bh[5] = F[j][1] * Gc[2] + F[j][2] * Gc[1];
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
         T s = (i == j) ? geo : (T)0;
