@@ -8,10 +8,15 @@
 // The Newton solver always uses the exact residual; the surrogate only replaces
 // the tangent modulus and the exact tangent is the fallback on a stall.
 //
-// Compute: BF16 MFMA (v_mfma_f32_16x16x16_bf16) with FP32 accumulation, FP32
-// master weights and Adam.  Layers are evaluated transposed, Y^T = W X^T, so a
-// wave holds 16 samples and the accumulator tile of one layer (features x
-// samples) is, lane for lane, the B operand of the next layer.
+// Compute: BF16 MFMA with FP32 accumulation, FP32 master weights and Adam.
+// Layers are evaluated transposed, Y^T = W X^T, so a wave holds 16 samples and the
+// accumulator tile of one layer (features x samples) is, lane for lane, the B
+// operand of the next layer.  Products over 32 or more features use gfx950's
+// v_mfma_f32_16x16x32_bf16: two consecutive 16-feature tiles form one K = 32 step,
+// each lane's 8 operand values being its 4 rows of the first tile and its 4 rows
+// of the second (a permutation of K, applied alike to A and B).  Products over 16
+// (the input layer, and the weight gradients, whose K is the 16 samples of a tile)
+// use v_mfma_f32_16x16x16_bf16.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -20,7 +25,7 @@
 #include "fedd_law.h"
 
 #ifndef FEDD_MLP_BPC
-#define FEDD_MLP_BPC 2  // inference workgroups per CU
+#define FEDD_MLP_BPC 3  // inference workgroups per CU (three waves per SIMD fit in 168 VGPRs)
 #endif
 #ifndef FEDD_MLP_TPW
 #define FEDD_MLP_TPW 2  // 16-sample tiles per wave in a training step
@@ -29,6 +34,8 @@
 namespace fedd {
 
 typedef short s4 __attribute__((ext_vector_type(4)));
+typedef short s8 __attribute__((ext_vector_type(8)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 // Parameter layout (floats): W1[64][16] b1[64] W2[64][64] b2[64] W3[32][64] b3[32].
@@ -45,7 +52,12 @@ enum : int {
   Q_W1 = 0,
   Q_W2 = Q_W1 + 64 * 16,
   Q_W3 = Q_W2 + 64 * 64,
-  Q_TOTAL = Q_W3 + 32 * 64
+  Q_TOTAL = Q_W3 + 32 * 64,
+  // the three bias vectors packed: b1[64] b2[64] b3[32]
+  B_B1 = 0,
+  B_B2 = 64,
+  B_B3 = 128,
+  B_TOTAL = 160
 };
 
 // Normalisation of the surrogate: x = (F - I) / s_in, CC = CC0 + s_out * y.
@@ -62,13 +74,37 @@ __device__ __forceinline__ short f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(short b) { return __uint_as_float(((unsigned)(unsigned short)b) << 16); }
 
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// Standard normal CDF Phi(z) = (1 + erf(z / sqrt 2)) / 2, with erf from Abramowitz &
+// Stegun 7.1.26: erf(x) = 1 - t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-x^2),
+// t = 1 / (1 + p x), x >= 0, |error| <= 1.5e-7 (7e-8 on Phi), well below the bfloat16
+// rounding of the activations that follows.  e = exp(-z^2 / 2) is returned for the
+// derivative.  About a third of the instructions of erff, which bounds these kernels.
+__device__ __forceinline__ float norm_cdf(float z, float& e) {
+  const float t = __builtin_amdgcn_rcpf(fmaf(0.23164188826636f, fabsf(z), 1.0f));  // p / sqrt 2
+  const float poly =
+      t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, 1.061405429f, -1.453152027f), 1.421413741f), -0.284496736f), 0.254829592f);
+  e = __builtin_amdgcn_exp2f(z * z * -0.72134752044448f);  // exp(-z^2 / 2)
+  const float q = 0.5f * poly * e;
+  return z >= 0.f ? 1.0f - q : q;
+}
+__device__ __forceinline__ float gelu(float z) {
+  float e;
+  return z * norm_cdf(z, e);
+}
 __device__ __forceinline__ float gelu_d(float z) {
-  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.3989422804014327f * expf(-0.5f * z * z);
+  float e;
+  const float c = norm_cdf(z, e);
+  return fmaf(z * 0.3989422804014327f, e, c);
 }
 
 __device__ __forceinline__ f4 mfma(s4 a, s4 b, f4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+// K = 32 step from two K = 16 operand pairs (lo: features 0..15, hi: 16..31).
+__device__ __forceinline__ f4 mfma2(s4 a_lo, s4 a_hi, s4 b_lo, s4 b_hi, f4 c) {
+  const s8 a = __builtin_shufflevector(a_lo, a_hi, 0, 1, 2, 3, 4, 5, 6, 7);
+  const s8 b = __builtin_shufflevector(b_lo, b_hi, 0, 1, 2, 3, 4, 5, 6, 7);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8, a), __builtin_bit_cast(b8, b), c, 0, 0, 0);
 }
 
 // A operand from a row-major BF16 matrix with `ld` columns: rows r0 + (lane % 16),
@@ -83,19 +119,67 @@ struct Fwd {
   s4 h1[4], h2[4];
 };
 
+// Weight operands of the forward pass read from the LDS copy at every use (training:
+// its backward pass needs the registers).
+struct LdsWeights {
+  const short* wb;   // BF16 weights, Q_ layout
+  const float* bias; // b1, b2, b3 packed, B_ layout
+  int lane;
+  __device__ s4 a1(int t) const { return load_a(wb + Q_W1, 16, 16 * t, 0, lane); }
+  __device__ s4 a2(int t, int kk) const { return load_a(wb + Q_W2, 64, 16 * t, 16 * kk, lane); }
+  __device__ s4 a3(int t, int kk) const { return load_a(wb + Q_W3, 64, 16 * t, 16 * kk, lane); }
+  __device__ float b1(int t, int r) const { return bias[B_B1 + 16 * t + 4 * (lane >> 4) + r]; }
+  __device__ float b2(int t, int r) const { return bias[B_B2 + 16 * t + 4 * (lane >> 4) + r]; }
+  __device__ float b3(int t, int r) const { return bias[B_B3 + 16 * t + 4 * (lane >> 4) + r]; }
+};
+
+// The same operands held in registers for a whole inference kernel (56 VGPRs of A
+// operands, 40 of biases), loaded once per lane from the global BF16 copy and the
+// FP32 parameters.
+struct RegWeights {
+  s4 w1[4], w2[4][4], w3[2][4];
+  float c1[4][4], c2[4][4], c3[2][4];
+  __device__ void load(const short* __restrict__ wb, const float* __restrict__ p, int lane) {
+    const int g = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      w1[t] = load_a(wb + Q_W1, 16, 16 * t, 0, lane);
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) w2[t][kk] = load_a(wb + Q_W2, 64, 16 * t, 16 * kk, lane);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        c1[t][r] = p[P_B1 + 16 * t + 4 * g + r];
+        c2[t][r] = p[P_B2 + 16 * t + 4 * g + r];
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) w3[t][kk] = load_a(wb + Q_W3, 64, 16 * t, 16 * kk, lane);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) c3[t][r] = p[P_B3 + 16 * t + 4 * g + r];
+    }
+  }
+  __device__ s4 a1(int t) const { return w1[t]; }
+  __device__ s4 a2(int t, int kk) const { return w2[t][kk]; }
+  __device__ s4 a3(int t, int kk) const { return w3[t][kk]; }
+  __device__ float b1(int t, int r) const { return c1[t][r]; }
+  __device__ float b2(int t, int r) const { return c2[t][r]; }
+  __device__ float b3(int t, int r) const { return c3[t][r]; }
+};
+
 // Forward pass of one 16-sample tile; xb is the B operand of the input
 // (lane: sample lane % 16, features 4 (lane / 16) .. + 3).
-__device__ __forceinline__ void forward(const short* __restrict__ wb, const float* __restrict__ p, s4 xb, int lane,
-                                        Fwd& f) {
-  const int g = lane >> 4;
+template <class W>
+__device__ __forceinline__ void forward(const W& w, s4 xb, Fwd& f) {
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     f4 acc = {0.f, 0.f, 0.f, 0.f};
-    acc = mfma(load_a(wb + Q_W1, 16, 16 * t, 0, lane), xb, acc);
+    acc = mfma(w.a1(t), xb, acc);
     s4 h;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const float z = acc[r] + p[P_B1 + 16 * t + 4 * g + r];
+      const float z = acc[r] + w.b1(t, r);
       f.z1[t][r] = z;
       h[r] = f2bf(gelu(z));
     }
@@ -105,11 +189,11 @@ __device__ __forceinline__ void forward(const short* __restrict__ wb, const floa
   for (int t = 0; t < 4; ++t) {
     f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int kk = 0; kk < 4; ++kk) acc = mfma(load_a(wb + Q_W2, 64, 16 * t, 16 * kk, lane), f.h1[kk], acc);
+    for (int kk = 0; kk < 4; kk += 2) acc = mfma2(w.a2(t, kk), w.a2(t, kk + 1), f.h1[kk], f.h1[kk + 1], acc);
     s4 h;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const float z = acc[r] + p[P_B2 + 16 * t + 4 * g + r];
+      const float z = acc[r] + w.b2(t, r);
       f.z2[t][r] = z;
       h[r] = f2bf(gelu(z));
     }
@@ -119,10 +203,20 @@ __device__ __forceinline__ void forward(const short* __restrict__ wb, const floa
   for (int t = 0; t < 2; ++t) {
     f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int kk = 0; kk < 4; ++kk) acc = mfma(load_a(wb + Q_W3, 64, 16 * t, 16 * kk, lane), f.h2[kk], acc);
+    for (int kk = 0; kk < 4; kk += 2) acc = mfma2(w.a3(t, kk), w.a3(t, kk + 1), f.h2[kk], f.h2[kk + 1], acc);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) f.y[t][r] = acc[r] + p[P_B3 + 16 * t + 4 * g + r];
+    for (int r = 0; r < 4; ++r) f.y[t][r] = acc[r] + w.b3(t, r);
   }
+}
+
+// Stages the BF16 weights (16-byte copies) and the packed bias vectors in LDS.
+__device__ __forceinline__ void stage_params(short* wb, float* bias, const short* __restrict__ wb_g,
+                                             const float* __restrict__ p_g) {
+  for (int k = threadIdx.x; k < Q_TOTAL / 8; k += blockDim.x)
+    reinterpret_cast<uint4*>(wb)[k] = reinterpret_cast<const uint4*>(wb_g)[k];
+  for (int k = threadIdx.x; k < B_TOTAL; k += blockDim.x)
+    bias[k] = p_g[k < B_B2 ? P_B1 + k : k < B_B3 ? P_B2 + (k - B_B2) : P_B3 + (k - B_B3)];
+  __syncthreads();
 }
 
 // Input operand of sample rows x[n][9] (float): lane holds features 4g..4g+3 of sample j.
@@ -138,18 +232,17 @@ __device__ __forceinline__ s4 input_b(const float* __restrict__ x, int64_t s, bo
 }
 
 // Inference on the quadrature-point records of the elasticity kernel: reads F,
-// writes CC into the tangent slot.  One wave per 16 records, grid-stride.
-__global__ __launch_bounds__(256) void k_mlp_tangent(double* __restrict__ qp, int64_t nrec,
-                                                     const short* __restrict__ wb_g, const float* __restrict__ p_g,
-                                                     MlpNorm nrm) {
-  __shared__ short wb[Q_TOTAL];
-  __shared__ float pp[P_TOTAL];
-  for (int k = threadIdx.x; k < Q_TOTAL; k += blockDim.x) wb[k] = wb_g[k];
-  for (int k = threadIdx.x; k < P_TOTAL; k += blockDim.x) pp[k] = p_g[k];
-  __syncthreads();
+// writes CC into the tangent slot.  One wave per 16 records, grid-stride; the weights
+// stay in registers for the whole walk.  (Writing the 21 outputs of a tile through LDS
+// as whole 168-byte runs per record measured the same: the kernel is bound by the
+// GELU evaluations and the latency of the load -> three layers -> store chain.)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_mlp_tangent(
+    double* __restrict__ qp, int64_t nrec, const short* __restrict__ wb_g, const float* __restrict__ p_g, MlpNorm nrm) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int j = lane & 15, g = lane >> 4;
+  RegWeights w;
+  w.load(wb_g, p_g, lane);
   const int64_t ntile = (nrec + 15) / 16;
   for (int64_t tile = (int64_t)blockIdx.x * 4 + wave; tile < ntile; tile += (int64_t)gridDim.x * 4) {
     const int64_t s = tile * 16 + j;
@@ -163,7 +256,7 @@ __global__ __launch_bounds__(256) void k_mlp_tangent(double* __restrict__ qp, in
       xb[r] = f2bf(v);
     }
     Fwd f;
-    forward(wb, pp, xb, lane, f);
+    forward(w, xb, f);
     if (valid) {
 #pragma unroll
       for (int t = 0; t < 2; ++t)
@@ -177,18 +270,20 @@ __global__ __launch_bounds__(256) void k_mlp_tangent(double* __restrict__ qp, in
 }
 
 // Inference on plain rows (tests / checks): x[n][9] normalised input, y[n][21].
-__global__ __launch_bounds__(256) void k_mlp_forward_rows(const float* __restrict__ x, int64_t n,
-                                                          const short* __restrict__ wb, const float* __restrict__ p,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_mlp_forward_rows(const float* __restrict__ x, int64_t n,
+                                                          const short* __restrict__ wb_g, const float* __restrict__ p_g,
                                                           float* __restrict__ y) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int j = lane & 15, g = lane >> 4;
+  RegWeights w;
+  w.load(wb_g, p_g, lane);
   const int64_t ntile = (n + 15) / 16;
   for (int64_t tile = (int64_t)blockIdx.x * 4 + wave; tile < ntile; tile += (int64_t)gridDim.x * 4) {
     const int64_t s = tile * 16 + j;
     const bool valid = s < n;
     Fwd f;
-    forward(wb, p, input_b(x, s, valid, lane), lane, f);
+    forward(w, input_b(x, s, valid, lane), f);
     if (valid)
 #pragma unroll
       for (int t = 0; t < 2; ++t)
@@ -233,12 +328,10 @@ __global__ __launch_bounds__(256) void k_mlp_train(const float* __restrict__ x, 
                                                    const short* __restrict__ wb_g, const float* __restrict__ p_g,
                                                    float* __restrict__ grad, float* __restrict__ loss,
                                                    int tiles_per_wave) {
-  __shared__ short wb[Q_TOTAL];
-  __shared__ float pp[P_TOTAL];
+  __shared__ __align__(16) short wb[Q_TOTAL];
+  __shared__ float bias[B_TOTAL];
   __shared__ float tbuf[4][16 * 17];
-  for (int k = threadIdx.x; k < Q_TOTAL; k += blockDim.x) wb[k] = wb_g[k];
-  for (int k = threadIdx.x; k < P_TOTAL; k += blockDim.x) pp[k] = p_g[k];
-  __syncthreads();
+  stage_params(wb, bias, wb_g, p_g);
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int j = lane & 15, g = lane >> 4;
@@ -272,7 +365,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(const float* __restrict__ x, 
     const int64_t s = valid ? idx[bs] : 0;
     const s4 xb = input_b(x, s, valid, lane);
     Fwd f;
-    forward(wb, pp, xb, lane, f);
+    forward(LdsWeights{wb, bias, lane}, xb, f);
     // dL/dy = 2 (y - t) / B on the 21 real outputs
     f4 d3[2];
 #pragma unroll
@@ -308,14 +401,12 @@ __global__ __launch_bounds__(256) void k_mlp_train(const float* __restrict__ x, 
     f4 d2[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      f4 acc = {0.f, 0.f, 0.f, 0.f};
+      s4 a[2];
 #pragma unroll
-      for (int kk = 0; kk < 2; ++kk) {
-        s4 a;
+      for (int kk = 0; kk < 2; ++kk)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) a[r] = wb[Q_W3 + (16 * kk + 4 * g + r) * 64 + 16 * t + j];
-        acc = mfma(a, bd3[kk], acc);
-      }
+        for (int r = 0; r < 4; ++r) a[kk][r] = wb[Q_W3 + (16 * kk + 4 * g + r) * 64 + 16 * t + j];
+      f4 acc = mfma2(a[0], a[1], bd3[0], bd3[1], f4{0.f, 0.f, 0.f, 0.f});
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         acc[r] *= gelu_d(f.z2[t][r]);
@@ -342,14 +433,13 @@ __global__ __launch_bounds__(256) void k_mlp_train(const float* __restrict__ x, 
     f4 d1[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      f4 acc = {0.f, 0.f, 0.f, 0.f};
+      s4 a[4];
 #pragma unroll
-      for (int kk = 0; kk < 4; ++kk) {
-        s4 a;
+      for (int kk = 0; kk < 4; ++kk)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) a[r] = wb[Q_W2 + (16 * kk + 4 * g + r) * 64 + 16 * t + j];
-        acc = mfma(a, bd2[kk], acc);
-      }
+        for (int r = 0; r < 4; ++r) a[kk][r] = wb[Q_W2 + (16 * kk + 4 * g + r) * 64 + 16 * t + j];
+      f4 acc = mfma2(a[0], a[1], bd2[0], bd2[1], f4{0.f, 0.f, 0.f, 0.f});
+      acc = mfma2(a[2], a[3], bd2[2], bd2[3], acc);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         acc[r] *= gelu_d(f.z1[t][r]);
@@ -483,7 +573,7 @@ int launch_mlp_tangent(Ctx* c, int nq) {
   MlpHost* h = c->mlp;
   if (!h || !c->mlp_ready) return fail(c, FEDD_ERR_STATE, "no surrogate network loaded");
   const int64_t nrec = c->mesh[0].nelem * nq;
-  // one workgroup per CU stages the weights once and walks its tiles
+  // FEDD_MLP_BPC resident workgroups per CU load the weights once and walk their tiles
   const unsigned grid = (unsigned)std::min<int64_t>((nrec + 63) / 64, (int64_t)c->num_cu * FEDD_MLP_BPC);
   ktimer_begin(c);
   k_mlp_tangent<<<grid, 256, 0, c->stream>>>(c->d_qp.as<double>(), nrec, h->wb(), h->p(), h->nrm);
