@@ -31,13 +31,19 @@
 //   fine 2 / coarse 3: 123, 0.082 s    fine 1 / coarse 3: 145, 0.077 s
 //   fine 1 / coarse 2: 149, 0.074 s    fine 1 / coarse 1: 180, 0.077 s
 //   fine 3 / coarse 3: 118, 0.092 s
-// (smoothed aggregation: 526 iterations, 0.42 s -> 644, 0.37 s).  A fine sweep streams
-// the whole P2 operator; the extra CG iterations cost less than the sweeps they save.
+// A fine sweep streams the whole P2 operator; the extra CG iterations cost less than the
+// sweeps they save.
 #ifndef FEDD_MG_NU_COARSE
 #define FEDD_MG_NU_COARSE 2
 #endif
 #ifndef FEDD_MG_NU
 #define FEDD_MG_NU 1
+#endif
+// Coarse-level sweeps of the aggregation hierarchy (its first coarse level holds a sixth of
+// the finest level's blocks).  Same Newton solve with the aggregation hierarchy: 2 sweeps
+// 333 iterations, 0.241 s; 3: 297, 0.222 s; 4: 281, 0.226 s; 6: 265, 0.243 s.
+#ifndef FEDD_AMG_NU_COARSE
+#define FEDD_AMG_NU_COARSE 3
 #endif
 #ifndef FEDD_MG_OMEGA
 #define FEDD_MG_OMEGA 0.6  // block-Jacobi damping
@@ -65,7 +71,8 @@ struct MgLevel {
   DevBuf binv, free, x, x2, b, r;
   DevBuf vals32;  // finest level: FP32 copy of the operator used by the smoother
   // aggregation hierarchy: transfer from the next finer level F to this level
-  DevBuf agg;                // int32, per F node: its aggregate (a node of this level)
+  DevBuf agg;                // int32, per F node: its aggregate (nodes kSaNodes * J + q here)
+  DevBuf tent;               // per F node, its 3 x kSaModes rows of the tentative prolongator
   DevBuf pptr, pidx, pval;   // smoothed prolongator, rows = F nodes, 3x3 blocks (row-major)
   DevBuf tptr, tidx, tpos;   // its transpose: per node of this level, (F node, block index in pval)
   DevBuf aptr, aidx, aval;   // A_F P, rows = F nodes
@@ -73,6 +80,7 @@ struct MgLevel {
   std::vector<int64_t> h_ptr;
   std::vector<int32_t> h_idx;
   std::vector<int32_t> h_agg;
+  std::vector<double> h_tent;
   std::vector<int64_t> h_pptr, h_tptr, h_tpos, h_aptr;
   std::vector<int32_t> h_pidx, h_tidx, h_aidx;
 };
@@ -292,16 +300,29 @@ __global__ __launch_bounds__(256) void k_amg_block_norm(int64_t nn, const int64_
   }
 }
 
-// Smoothed aggregation.  The tentative prolongator maps coarse dof (J, d) to the free fine
-// dofs (i, d) with agg(i) = J; the prolongator is P = (I - omega D^-1 A) P_tent, D the 3x3
-// block diagonal (binv holds its inverse).  Row i of P has a 3x3 block for every aggregate
+// Near-null-space modes per aggregate: the three translations and, with 6, the three
+// rigid rotations.  An aggregate J becomes kSaNodes coarse nodes kSaNodes * J + q, each
+// with 3 dofs (coarse dof (J, 3q + d) is mode 3q + d), so every level keeps the 3x3
+// node-block layout.
+#ifndef FEDD_SA_MODES
+#define FEDD_SA_MODES 6
+#endif
+constexpr int kSaModes = FEDD_SA_MODES;
+constexpr int kSaNodes = FEDD_SA_MODES / 3;
+static_assert(kSaModes == 3 || kSaModes == 6, "3 or 6 near-null-space modes");
+
+// Smoothed aggregation.  The tentative prolongator T maps the coarse dofs of aggregate J
+// to its free fine dofs: row (i, d) of T is row (i, d) of the aggregate's orthonormalised
+// mode matrix (tent holds these 3 x kSaModes rows per fine node, zero for constrained
+// dofs).  The prolongator is P = (I - omega D^-1 A) T, D the 3x3 block diagonal (binv
+// holds its inverse).  Row i of P has a 3x3 block for every coarse node of the aggregates
 // of i's neighbours; rows of constrained fine dofs are zero.
 __global__ __launch_bounds__(256) void k_sa_smooth_p(int64_t nf, const int64_t* __restrict__ fptr,
                                                      const int32_t* __restrict__ fidx, const double* __restrict__ fval,
                                                      const double* __restrict__ binv, const uint8_t* __restrict__ ffree,
-                                                     const int32_t* __restrict__ agg, const int64_t* __restrict__ pptr,
-                                                     const int32_t* __restrict__ pidx, double* __restrict__ pval,
-                                                     double omega) {
+                                                     const int32_t* __restrict__ agg, const double* __restrict__ tent,
+                                                     const int64_t* __restrict__ pptr, const int32_t* __restrict__ pidx,
+                                                     double* __restrict__ pval, double omega) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= nf) return;
   const int64_t p0 = pptr[i];
@@ -312,18 +333,29 @@ __global__ __launch_bounds__(256) void k_sa_smooth_p(int64_t nf, const int64_t* 
   const int64_t f0 = fptr[i];
   const int degf = (int)(fptr[i + 1] - f0);
   const double* fv = fval + 9 * f0;
-  for (int s = 0; s < degf; ++s) {  // S = A P_tent
+  for (int s = 0; s < degf; ++s) {  // S = A T
     const int32_t j = fidx[f0 + s];
-    const int slot = find_slot(prow, np, agg[j]);
-    if (slot < 0) continue;  // cannot happen: row i holds the aggregate of every neighbour
+    const double* T = tent + 3 * kSaModes * (int64_t)j;
+    double a[3][3];
 #pragma unroll
     for (int d1 = 0; d1 < 3; ++d1)
 #pragma unroll
-      for (int d2 = 0; d2 < 3; ++d2)
-        if (ffree[3 * (int64_t)j + d2]) pv[9 * slot + 3 * d1 + d2] += fv[3 * degf * d1 + 3 * s + d2];
+      for (int e = 0; e < 3; ++e) a[d1][e] = fv[3 * degf * d1 + 3 * s + e];
+#pragma unroll
+    for (int q = 0; q < kSaNodes; ++q) {
+      const int slot = find_slot(prow, np, kSaNodes * agg[j] + q);
+      if (slot < 0) continue;  // cannot happen: row i holds the aggregate of every neighbour
+#pragma unroll
+      for (int d1 = 0; d1 < 3; ++d1)
+#pragma unroll
+        for (int d2 = 0; d2 < 3; ++d2)
+          pv[9 * slot + 3 * d1 + d2] += a[d1][0] * T[3 * q + d2] + a[d1][1] * T[kSaModes + 3 * q + d2] +
+                                        a[d1][2] * T[2 * kSaModes + 3 * q + d2];
+    }
   }
   const double* B = binv + 9 * i;
-  const int32_t own = agg[i];
+  const double* Ti = tent + 3 * kSaModes * i;
+  const int32_t own = kSaNodes * agg[i];
   for (int slot = 0; slot < np; ++slot) {
     double t[3][3];
 #pragma unroll
@@ -332,12 +364,13 @@ __global__ __launch_bounds__(256) void k_sa_smooth_p(int64_t nf, const int64_t* 
       for (int d2 = 0; d2 < 3; ++d2)
         t[d1][d2] = B[3 * d1] * pv[9 * slot + d2] + B[3 * d1 + 1] * pv[9 * slot + 3 + d2] +
                     B[3 * d1 + 2] * pv[9 * slot + 6 + d2];
+    const int q = prow[slot] - own;  // this slot is coarse node q of i's own aggregate
 #pragma unroll
     for (int d1 = 0; d1 < 3; ++d1)
 #pragma unroll
       for (int d2 = 0; d2 < 3; ++d2) {
         double v = -omega * t[d1][d2];
-        if (prow[slot] == own && d1 == d2) v += 1.0;
+        if (q >= 0 && q < kSaNodes) v += Ti[kSaModes * d1 + 3 * q + d2];
         pv[9 * slot + 3 * d1 + d2] = ffree[3 * i + d1] ? v : 0.0;
       }
   }
@@ -457,19 +490,30 @@ __global__ __launch_bounds__(256) void k_sa_restrict(int64_t nc, const int64_t* 
 }
 
 // x_f += P x_c (rows of constrained fine dofs are zero).
+template <int G>  // lanes per fine row
 __global__ __launch_bounds__(256) void k_sa_prolong(int64_t nf, const int64_t* __restrict__ pptr,
                                                     const int32_t* __restrict__ pidx, const double* __restrict__ pval,
                                                     const double* __restrict__ xc, double* __restrict__ xf) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= nf) return;
+  const int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G;
+  const int lane = threadIdx.x % G;
   double s[3] = {0.0, 0.0, 0.0};
-  for (int64_t k = pptr[i]; k < pptr[i + 1]; ++k) {
-    const double* P = pval + 9 * k;
-    const int64_t J = pidx[k];
-    const double x0 = xc[3 * J], x1 = xc[3 * J + 1], x2 = xc[3 * J + 2];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) s[d] += P[3 * d] * x0 + P[3 * d + 1] * x1 + P[3 * d + 2] * x2;
+  if (i < nf) {  // the row's blocks are contiguous: the G lanes stream its 9 np values
+    const int64_t k0 = pptr[i];
+    const int L = 9 * (int)(pptr[i + 1] - k0);
+    const double* P = pval + 9 * k0;
+    for (int q = lane; q < L; q += G) {
+      const int k = q / 9, r = q - 9 * k, d1 = r / 3, d2 = r - 3 * d1;
+      const double v = P[q] * xc[3 * (int64_t)pidx[k0 + k] + d2];
+      s[0] += d1 == 0 ? v : 0.0;
+      s[1] += d1 == 1 ? v : 0.0;
+      s[2] += d1 == 2 ? v : 0.0;
+    }
   }
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) s[d] += __shfl_xor(s[d], o, G);
+  if (i >= nf || lane != 0) return;
 #pragma unroll
   for (int d = 0; d < 3; ++d) xf[3 * i + d] += s[d];
 }
@@ -835,9 +879,13 @@ static int64_t aggregate(int64_t n, const std::vector<int64_t>& ptr, const std::
 #ifndef FEDD_SA_OMEGA
 #define FEDD_SA_OMEGA 0.67
 #endif
-// Levels whose average row holds at most this many blocks aggregate over strong couplings only.
+// Levels whose average row holds at most this many blocks aggregate over strong couplings
+// only.  Off: with the six rigid-body modes, aggregating the P2 level over its strong
+// couplings only (threshold 64) gives aggregates of 14 nodes and a first coarse level with
+// more blocks than the finest (4.2M vs 3.3M at 24^3x6; 267 CG iterations, 0.43 s); over
+// all couplings, 56 nodes and 0.54M blocks (333 iterations, 0.24 s).
 #ifndef FEDD_AMG_FILTER_DEG
-#define FEDD_AMG_FILTER_DEG 64
+#define FEDD_AMG_FILTER_DEG 0
 #endif
 // Strong coupling: ||A_ij||_F >= theta sqrt(||A_ii||_F ||A_jj||_F).
 #ifndef FEDD_AMG_THETA
@@ -861,6 +909,7 @@ static int level_device_setup(Ctx* c, MgLevel& L, size_t l, int mode, const std:
     L.d_vals = L.vals.as<double>();
     if (mode == 1) {
       if ((rc = upload(c, L.agg, L.h_agg.data(), L.h_agg.size() * 4)) ||
+          (rc = upload(c, L.tent, L.h_tent.data(), L.h_tent.size() * 8)) ||
           (rc = upload(c, L.pptr, L.h_pptr.data(), L.h_pptr.size() * 8)) ||
           (rc = upload(c, L.pidx, L.h_pidx.data(), std::max<size_t>(L.h_pidx.size(), 1) * 4)) ||
           (rc = alloc(c, L.pval, std::max<size_t>(L.h_pidx.size(), 1) * 9 * 8)) ||
@@ -881,6 +930,7 @@ static int level_device_setup(Ctx* c, MgLevel& L, size_t l, int mode, const std:
   std::vector<int64_t>().swap(L.h_ptr);
   std::vector<int32_t>().swap(L.h_idx);
   std::vector<int32_t>().swap(L.h_agg);
+  std::vector<double>().swap(L.h_tent);
   for (auto* v : {&L.h_pptr, &L.h_tptr, &L.h_tpos, &L.h_aptr}) std::vector<int64_t>().swap(*v);
   for (auto* v : {&L.h_pidx, &L.h_tidx, &L.h_aidx}) std::vector<int32_t>().swap(*v);
   return FEDD_OK;
@@ -892,7 +942,8 @@ static int sa_galerkin(Ctx* c, MgHost* g, size_t l) {
   const MgLevel& F = g->lv[l - 1];
   MgLevel& L = g->lv[l];
   k_sa_smooth_p<<<nbk(F.nn, 256), 256, 0, c->stream>>>(F.nn, F.d_nptr, F.d_nidx, F.d_vals, F.binv.as<double>(),
-                                                      F.free.as<uint8_t>(), L.agg.as<int32_t>(), L.pptr.as<int64_t>(),
+                                                      F.free.as<uint8_t>(), L.agg.as<int32_t>(), L.tent.as<double>(),
+                                                      L.pptr.as<int64_t>(),
                                                       L.pidx.as<int32_t>(), L.pval.as<double>(), FEDD_SA_OMEGA);
   const unsigned gf = (unsigned)std::min<int64_t>(F.nn, (int64_t)c->num_cu * 16);
   k_sa_ap<<<gf, 64, (size_t)9 * L.maxdeg_ap * 8, c->stream>>>(F.nn, F.d_nptr, F.d_nidx, F.d_vals,
@@ -946,6 +997,82 @@ static void union_rows(int64_t nrows, int64_t ncols, RowFn rowfn, std::vector<in
   }
 }
 
+// Tentative prolongator of one aggregation step.  nsp holds, per fine node, the 3 x
+// kSaModes rows of the near-null-space modes (level 0: translations and rotations about
+// the mesh centroid; below: the R factors of the step above).  Per aggregate J the rows
+// of its members' free dofs are stacked into a (3 n_J) x kSaModes matrix and factored
+// V = Q R by modified Gram-Schmidt with one reorthogonalisation; a column whose remainder
+// falls below 1e-8 of its norm (a singleton's rotations, a fully constrained aggregate)
+// is dropped: its Q column and R row are zero and its coarse dof is marked constrained.
+// tent gets Q's rows, the coarse nodes kSaNodes J + q get R's rows 3q..3q+2 as their
+// modes (T R = V: the coarse space reproduces the fine level's modes exactly).
+static void sa_tentative(int64_t nf, int64_t nagg, const std::vector<int32_t>& agg, const std::vector<double>& nsp,
+                         const std::vector<uint8_t>& ffree, std::vector<double>& tent, std::vector<double>& cnsp,
+                         std::vector<uint8_t>& cfree) {
+  constexpr int W = 3 * kSaModes;  // doubles per node
+  std::vector<int64_t> mptr(nagg + 1, 0);
+  for (int64_t i = 0; i < nf; ++i) mptr[agg[i] + 1]++;
+  for (int64_t J = 0; J < nagg; ++J) mptr[J + 1] += mptr[J];
+  std::vector<int32_t> mem(nf);
+  {
+    std::vector<int64_t> fill(mptr.begin(), mptr.end() - 1);
+    for (int64_t i = 0; i < nf; ++i) mem[fill[agg[i]]++] = (int32_t)i;
+  }
+  tent.assign(nf * W, 0.0);
+  cnsp.assign(nagg * kSaNodes * W, 0.0);
+  cfree.assign(nagg * kSaNodes * 3, 0);
+  parallel_for(nagg, [&](int64_t a, int64_t b, int) {
+    std::vector<double> V;
+    for (int64_t J = a; J < b; ++J) {
+      const int64_t m0 = mptr[J];
+      const int rows = (int)(3 * (mptr[J + 1] - m0));
+      V.assign((size_t)rows * kSaModes, 0.0);  // column-major: V[k * rows + r]
+      for (int r = 0; r < rows; ++r) {
+        const int64_t i = mem[m0 + r / 3];
+        const int d = r % 3;
+        if (!ffree[3 * i + d]) continue;
+        for (int k = 0; k < kSaModes; ++k) V[(size_t)k * rows + r] = nsp[W * i + kSaModes * d + k];
+      }
+      double R[kSaModes][kSaModes] = {};
+      for (int k = 0; k < kSaModes; ++k) {
+        double* v = V.data() + (size_t)k * rows;
+        double n0 = 0.0;
+        for (int r = 0; r < rows; ++r) n0 += v[r] * v[r];
+        n0 = std::sqrt(n0);
+        for (int pass = 0; pass < 2; ++pass)
+          for (int j = 0; j < k; ++j) {
+            const double* qj = V.data() + (size_t)j * rows;
+            double dot = 0.0;
+            for (int r = 0; r < rows; ++r) dot += qj[r] * v[r];
+            R[j][k] += dot;
+            for (int r = 0; r < rows; ++r) v[r] -= dot * qj[r];
+          }
+        double n1 = 0.0;
+        for (int r = 0; r < rows; ++r) n1 += v[r] * v[r];
+        n1 = std::sqrt(n1);
+        if (n0 > 0.0 && n1 > 1e-8 * n0) {
+          for (int r = 0; r < rows; ++r) v[r] /= n1;
+          R[k][k] = n1;
+        } else {
+          for (int r = 0; r < rows; ++r) v[r] = 0.0;  // V_k = sum_j<k q_j R[j][k]: kept in R's column k
+        }
+      }
+      for (int r = 0; r < rows; ++r) {
+        const int64_t i = mem[m0 + r / 3];
+        const int d = r % 3;
+        for (int k = 0; k < kSaModes; ++k) tent[W * i + kSaModes * d + k] = V[(size_t)k * rows + r];
+      }
+      for (int q = 0; q < kSaNodes; ++q) {
+        const int64_t I = kSaNodes * J + q;
+        for (int d = 0; d < 3; ++d) {
+          cfree[3 * I + d] = R[3 * q + d][3 * q + d] != 0.0;
+          for (int k = 0; k < kSaModes; ++k) cnsp[W * I + kSaModes * d + k] = R[3 * q + d][k];
+        }
+      }
+    }
+  });
+}
+
 // Aggregation hierarchy from the node graph of level 0 (any mesh).
 static int build_aggregation(Ctx* c, MgHost* g, std::vector<std::vector<uint8_t>>& frees) {
   const int64_t n0 = c->mesh[0].npts;
@@ -975,6 +1102,25 @@ static int build_aggregation(Ctx* c, MgHost* g, std::vector<std::vector<uint8_t>
         strong[k] = idx[k] == i || nrm[k] >= (float)FEDD_AMG_THETA * std::sqrt(diag[i] * diag[idx[k]]);
     return FEDD_OK;
   };
+  // near-null-space modes of level 0: translations and rotations about the centroid
+  std::vector<double> nsp((size_t)n0 * 3 * kSaModes, 0.0);
+  {
+    const std::vector<double>& X = c->mesh[0].pts;
+    double cen[3] = {0.0, 0.0, 0.0};
+    for (int64_t i = 0; i < n0; ++i)
+      for (int d = 0; d < 3; ++d) cen[d] += X[3 * i + d] / (double)n0;
+    for (int64_t i = 0; i < n0; ++i) {
+      double* B = nsp.data() + 3 * kSaModes * i;  // B[kSaModes * d + k]
+      for (int d = 0; d < 3; ++d) B[kSaModes * d + d] = 1.0;
+      if (kSaModes == 6) {
+        const double x = X[3 * i] - cen[0], y = X[3 * i + 1] - cen[1], z = X[3 * i + 2] - cen[2];
+        // rotations about the x, y and z axes: omega x (x, y, z)
+        B[kSaModes * 1 + 3] = -z, B[kSaModes * 2 + 3] = y;
+        B[kSaModes * 0 + 4] = z, B[kSaModes * 2 + 4] = -x;
+        B[kSaModes * 0 + 5] = -y, B[kSaModes * 1 + 5] = x;
+      }
+    }
+  }
   while (g->lv.back().nn > FEDD_AMG_COARSEST && g->lv.size() < 16) {
     const MgLevel& F = g->lv.back();
     int rc;
@@ -987,14 +1133,16 @@ static int build_aggregation(Ctx* c, MgHost* g, std::vector<std::vector<uint8_t>
     const bool filter = (int64_t)idx.size() <= FEDD_AMG_FILTER_DEG * F.nn;
     if (filter && (rc = strength(F, strong))) return rc;
     std::vector<int32_t> agg;
-    const int64_t nc = aggregate(F.nn, ptr, idx, filter ? &strong : nullptr, agg);
+    const int64_t nagg = aggregate(F.nn, ptr, idx, filter ? &strong : nullptr, agg);
+    const int64_t nc = kSaNodes * nagg;
     if (nc >= F.nn) break;  // no coarsening left
     MgLevel C;
     C.nn = nc;
-    // prolongator graph: row i holds the aggregates of i's neighbours
+    // prolongator graph: row i holds the coarse nodes of the aggregates of i's neighbours
     int64_t md = 0;
     union_rows(F.nn, nc, [&](int64_t i, auto&& emit) {
-      for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k) emit(agg[idx[k]]);
+      for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k)
+        for (int q = 0; q < kSaNodes; ++q) emit(kSaNodes * agg[idx[k]] + q);
     }, C.h_pptr, C.h_pidx, &md);
     // its transpose
     C.h_tptr.assign(nc + 1, 0);
@@ -1028,11 +1176,11 @@ static int build_aggregation(Ctx* c, MgHost* g, std::vector<std::vector<uint8_t>
     C.nblk = C.h_ptr[nc];
     if (9 * std::max(C.maxdeg, C.maxdeg_ap) * 8 > 65536)
       return fail(c, FEDD_ERR_LIMIT, "aggregation: a coarse row exceeds the LDS budget");
-    // coarse dof free iff some member's dof is free
-    std::vector<uint8_t> cf(3 * nc, 0);
-    const std::vector<uint8_t>& ff = frees.back();
-    for (int64_t i = 0; i < F.nn; ++i)
-      for (int d = 0; d < 3; ++d) cf[3 * (int64_t)agg[i] + d] |= ff[3 * i + d];
+    // tentative prolongator, the coarse level's modes and its constrained dofs
+    std::vector<uint8_t> cf;
+    std::vector<double> cnsp;
+    sa_tentative(F.nn, nagg, agg, nsp, frees.back(), C.h_tent, cnsp, cf);
+    nsp = std::move(cnsp);
     frees.push_back(std::move(cf));
     C.h_agg = std::move(agg);
     ptr = C.h_ptr;
@@ -1200,12 +1348,16 @@ static void vcycle(Ctx* c, size_t l, const double* b, double* x, bool first_done
     return;
   }
   const double w = FEDD_MG_OMEGA;
-  const unsigned gs = l == 0 ? nbk(L.nn * FEDD_MG_G0, 256) : nbk(L.nn * 16, 256);
+  // coarse rows of the aggregation hierarchy hold 100+ blocks: a whole wave per node
+  const bool wide = l > 0 && L.nblk > 48 * L.nn;
+  const unsigned gs = nbk(L.nn * (l == 0 ? FEDD_MG_G0 : wide ? 64 : 16), 256);
   auto sweep = [&](int mode, const double* xin, double* out) {
     if (l == 0 && L.vals32.p)
       launch_sweep<FEDD_MG_G0>(c->stream, gs, mode, L, L.vals32.as<float>(), b, xin, out, w);
     else if (l == 0)
       launch_sweep<FEDD_MG_G0>(c->stream, gs, mode, L, (const double*)L.d_vals, b, xin, out, w);
+    else if (wide)
+      launch_sweep<64>(c->stream, gs, mode, L, (const double*)L.d_vals, b, xin, out, w);
     else
       launch_sweep<16>(c->stream, gs, mode, L, (const double*)L.d_vals, b, xin, out, w);
   };
@@ -1215,7 +1367,7 @@ static void vcycle(Ctx* c, size_t l, const double* b, double* x, bool first_done
   double* xb = L.x2.as<double>();
   if (!first_done) sweep(1, nullptr, xb);  // from x = 0
   double* cur = xb;
-  const int nu = l == 0 ? FEDD_MG_NU : FEDD_MG_NU_COARSE;
+  const int nu = l == 0 ? FEDD_MG_NU : g->mode == 1 ? FEDD_AMG_NU_COARSE : FEDD_MG_NU_COARSE;
   for (int s = 1; s < nu; ++s) {
     double* nxt = cur == xa ? xb : xa;
     sweep(0, cur, nxt);
@@ -1236,8 +1388,8 @@ static void vcycle(Ctx* c, size_t l, const double* b, double* x, bool first_done
         fuse ? C.binv.as<double>() : nullptr, fuse ? C.x2.as<double>() : nullptr, w);
   vcycle(c, l + 1, C.b.as<double>(), C.x.as<double>(), fuse);
   if (g->mode == 1)
-    k_sa_prolong<<<nbk(L.nn, 256), 256, 0, c->stream>>>(L.nn, C.pptr.as<int64_t>(), C.pidx.as<int32_t>(),
-                                                        C.pval.as<double>(), C.x.as<double>(), cur);
+    k_sa_prolong<16><<<nbk(L.nn * 16, 256), 256, 0, c->stream>>>(L.nn, C.pptr.as<int64_t>(), C.pidx.as<int32_t>(),
+                                                                 C.pval.as<double>(), C.x.as<double>(), cur);
   else
     k_mg_prolong<<<nbk(L.nn, 256), 256, 0, c->stream>>>(L.P, C.P, C.x.as<double>(), L.free.as<uint8_t>(),
                                                         C.free.as<uint8_t>(), cur);

@@ -320,9 +320,10 @@ int fedd_linear_solve(fedd_ctx* ctx, double rtol, int maxit, int* iters, double*
 /* 0: 3x3 block Jacobi (any mesh); 1: geometric multigrid V-cycle on the structured
  * lattice (P2 -> P1 -> coarser P1, Galerkin coarse operators, damped block-Jacobi
  * smoothing, dense coarsest solve); 2: smoothed-aggregation multigrid on any mesh (greedy
- * node aggregates, prolongator (I - omega D^-1 A) P_tent with P_tent piecewise constant per
- * dof component, Galerkin coarse operators, the same smoother and coarsest solve).  1 and 2 stand in for the reference's
- * FROSch/MueLu preconditioners (Preconditioner_def.hpp:206-291). */
+ * node aggregates, prolongator (I - omega D^-1 A) P_tent with P_tent spanning the six
+ * rigid-body modes per aggregate, Galerkin coarse operators, the same smoother and
+ * coarsest solve).  1 and 2 stand in for the reference's FROSch/MueLu preconditioners
+ * (Preconditioner_def.hpp:206-291). */
 int fedd_set_preconditioner(fedd_ctx* ctx, int kind);
 /* No counterpart in the reference.  For callers that cannot allocate device memory
  * (code under stream capture, tight serving loops, harnesses that forbid hipMalloc):
