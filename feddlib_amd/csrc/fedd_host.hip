@@ -185,6 +185,26 @@ struct PlanSpec {
 
 constexpr int kSegment = 64;  // pairs per wave
 
+// Meta of the pair (row node i, element e, local index li of i in e): loc, node-in-chunk
+// (left 0), the slots of the canonical columns in row i, then the second block's slots.
+static void pair_meta_of(const PlanSpec& s, int li, int64_t i, int32_t e, uint8_t m[16]) {
+  for (int k = 0; k < 16; ++k) m[k] = 0;
+  m[0] = (uint8_t)li;
+  // the relabelling acts on the column element: P2 columns use the tet-10
+  // permutation (its vertex part is the tet-4 one), P1 columns the tet-4 one
+  const int ncol = s.cols_main->npe;
+  const int8_t* perm = ncol == 10 ? kPermP2[li] : kPermP1[li];
+  for (int cpos = 0; cpos < ncol; ++cpos) {
+    const int32_t j = s.cols_main->elems[(int64_t)e * ncol + perm[cpos]];
+    m[2 + cpos] = (uint8_t)find_slot(*s.main, i, j);
+  }
+  if (s.second)
+    for (int cpos = 0; cpos < 4; ++cpos) {
+      const int32_t j = s.cols_second->elems[(int64_t)e * 4 + perm[cpos]];
+      m[12 + cpos] = (uint8_t)find_slot(*s.second, i, j);
+    }
+}
+
 int build_plan(Ctx* c, const Incidence& inc, int64_t nnodes, const Mesh& rows_mesh,
                const PlanSpec& s, GatherPlan& plan) {
   std::vector<int32_t> chunk_node(1, 0), chunk_pair(1, 0), chunk_split;
@@ -215,26 +235,11 @@ int build_plan(Ctx* c, const Incidence& inc, int64_t nnodes, const Mesh& rows_me
   };
   int64_t n0 = 0;
   auto emit = [&](int64_t i, int64_t q) {  // pair (node i, incidence entry q) of the chunk at n0
-    const int li = inc.loc[q];
     const int32_t e = inc.elem[q];
     pair_elem.push_back(e);
-    uint8_t m[16] = {0};
-    m[0] = (uint8_t)li;
+    uint8_t m[16];
+    pair_meta_of(s, inc.loc[q], i, e, m);
     m[1] = (uint8_t)(i - n0);
-    // the relabelling acts on the column element: P2 columns use the tet-10
-    // permutation (its vertex part is the tet-4 one), P1 columns the tet-4 one
-    const int ncol = s.cols_main->npe;
-    const int8_t* perm = ncol == 10 ? kPermP2[li] : kPermP1[li];
-    for (int cpos = 0; cpos < ncol; ++cpos) {
-      int32_t j = s.cols_main->elems[(int64_t)e * ncol + perm[cpos]];
-      m[2 + cpos] = (uint8_t)find_slot(*s.main, i, j);
-    }
-    if (s.second) {
-      for (int cpos = 0; cpos < 4; ++cpos) {
-        int32_t j = s.cols_second->elems[(int64_t)e * 4 + perm[cpos]];
-        m[12 + cpos] = (uint8_t)find_slot(*s.second, i, j);
-      }
-    }
     pair_meta.insert(pair_meta.end(), m, m + 16);
   };
   while (n0 < nnodes) {
@@ -321,6 +326,97 @@ int build_plan(Ctx* c, const Incidence& inc, int64_t nnodes, const Mesh& rows_me
 struct GraphHost {
   Adjacency a, bt, b;
 };
+
+// Greedy first-fit colouring in ascending element order: an element takes the
+// smallest colour not used by any element sharing a node with it.  Elements sorted by
+// colour (ascending id inside a colour) in perm, colour k at [ptr[k], ptr[k+1]).
+int colour_elements(const Mesh& m, std::vector<int64_t>& ptr, std::vector<int32_t>& perm) {
+  Incidence inc;
+  build_incidence(m, inc);
+  std::vector<int32_t> col(m.nelem, -1);
+  std::vector<int64_t> used;  // used[colour] == e marks colour as taken for element e
+  int nc = 0;
+  for (int64_t e = 0; e < m.nelem; ++e) {
+    for (int k = 0; k < m.npe; ++k) {
+      int32_t n = m.elems[e * m.npe + k];
+      for (int64_t q = inc.ptr[n]; q < inc.ptr[n + 1]; ++q) {
+        int32_t f = inc.elem[q];
+        if (f < e && col[f] >= 0) {
+          if ((int)used.size() <= col[f]) used.resize(col[f] + 1, -1);
+          used[col[f]] = e;
+        }
+      }
+    }
+    int k = 0;
+    while (k < (int)used.size() && used[k] == e) ++k;
+    col[e] = k;
+    nc = std::max(nc, k + 1);
+  }
+  ptr.assign(nc + 1, 0);
+  for (int64_t e = 0; e < m.nelem; ++e) ptr[col[e] + 1]++;
+  for (int k = 0; k < nc; ++k) ptr[k + 1] += ptr[k];
+  perm.resize(m.nelem);
+  std::vector<int64_t> fill(ptr.begin(), ptr.end() - 1);
+  for (int64_t e = 0; e < m.nelem; ++e) perm[fill[col[e]]++] = (int32_t)e;
+  return nc;
+}
+
+// Coloured scatter schedule of the Navier-Stokes graph: per colour, the velocity pairs
+// (element, local node) with the vertex class first, and the pressure pairs; the meta
+// of each pair is the row gathers' (vs, ps: their plan specifications).
+int build_scatter(Ctx* c, const Mesh& m, const Mesh& pm, const PlanSpec& vs, const PlanSpec& ps,
+                  const GraphHost& g) {
+  ScatterPlan& sp = c->scat;
+  sp = ScatterPlan();
+  std::vector<int64_t> cptr;
+  std::vector<int32_t> cperm;
+  const int nc = colour_elements(m, cptr, cperm);
+  uint8_t mt[16];
+  for (int k = 0; k < nc; ++k) {
+    sp.vq.push_back((int64_t)sp.h_velem.size());
+    for (int cls = 0; cls < 2; ++cls) {
+      if (cls == 1) sp.vsplit.push_back((int64_t)sp.h_velem.size());
+      for (int64_t q = cptr[k]; q < cptr[k + 1]; ++q) {
+        const int32_t e = cperm[q];
+        for (int li = cls ? 4 : 0; li < (cls ? 10 : 4); ++li) {
+          const int32_t i = m.elems[(int64_t)e * 10 + li];
+          pair_meta_of(vs, li, i, e, mt);
+          sp.h_velem.push_back(e);
+          sp.h_vnode.push_back(i);
+          sp.h_vmeta.insert(sp.h_vmeta.end(), mt, mt + 16);
+        }
+      }
+    }
+    sp.pq.push_back((int64_t)sp.h_pelem.size());
+    for (int64_t q = cptr[k]; q < cptr[k + 1]; ++q) {
+      const int32_t e = cperm[q];
+      for (int li = 0; li < 4; ++li) {
+        const int32_t pn = pm.elems[(int64_t)e * 4 + li];
+        pair_meta_of(ps, li, pn, e, mt);
+        sp.h_pelem.push_back(e);
+        sp.h_pnode.push_back(pn);
+        sp.h_pmeta.insert(sp.h_pmeta.end(), mt, mt + 16);
+      }
+    }
+  }
+  sp.vq.push_back((int64_t)sp.h_velem.size());
+  sp.pq.push_back((int64_t)sp.h_pelem.size());
+  sp.a_ptr = g.a.ptr;
+  sp.bt_ptr = g.bt.ptr;
+  sp.b_ptr = g.b.ptr;
+  int rc;
+  if ((rc = upload(c, sp.d_velem, sp.h_velem.data(), sp.h_velem.size() * 4)) ||
+      (rc = upload(c, sp.d_vnode, sp.h_vnode.data(), sp.h_vnode.size() * 4)) ||
+      (rc = upload(c, sp.d_vmeta, sp.h_vmeta.data(), sp.h_vmeta.size())) ||
+      (rc = upload(c, sp.d_pelem, sp.h_pelem.data(), sp.h_pelem.size() * 4)) ||
+      (rc = upload(c, sp.d_pnode, sp.h_pnode.data(), sp.h_pnode.size() * 4)) ||
+      (rc = upload(c, sp.d_pmeta, sp.h_pmeta.data(), sp.h_pmeta.size())))
+    return rc;
+  sp.ncolours = nc;
+  c->colour_ptr = std::move(cptr);  // the same colouring, for fedd_colour_export
+  c->colour_perm = std::move(cperm);
+  return FEDD_OK;
+}
 
 }  // namespace
 }  // namespace fedd
@@ -574,6 +670,7 @@ int fedd_graph_build(fedd_ctx* c, int kind) {
     b.d_colind.release();
     b.d_vals.release();
   }
+  c->scat = ScatterPlan();
   Incidence inc;
   build_incidence(m, inc);
   GraphHost g;
@@ -669,6 +766,10 @@ int fedd_graph_build(fedd_ctx* c, int kind) {
       c->graph_kind = GK_NONE;
       return rc;
     }
+    if (c->colour_scatter && (rc = build_scatter(c, m, c->mesh[1], vs, ps, g))) {
+      c->graph_kind = GK_NONE;
+      return rc;
+    }
   }
   FEDD_HIP(c, hipStreamSynchronize(c->stream));
   c->newton_step = 0;
@@ -678,6 +779,38 @@ int fedd_graph_build(fedd_ctx* c, int kind) {
 int fedd_set_reproducible(fedd_ctx* c, int on) {
   if (!c || (on != 0 && on != 1)) return FEDD_ERR_ARG;
   c->reproducible = on != 0;
+  return FEDD_OK;
+}
+
+int fedd_set_assembly_method(fedd_ctx* c, int method) {
+  if (!c || (method != FEDD_METHOD_ROW_GATHER && method != FEDD_METHOD_COLOUR_SCATTER)) return FEDD_ERR_ARG;
+  c->colour_scatter = method == FEDD_METHOD_COLOUR_SCATTER;
+  return FEDD_OK;
+}
+
+int fedd_scatter_indices(fedd_ctx* c, int block, int64_t* idx) {
+  if (!c || !idx || block < 0 || block > 2) return FEDD_ERR_ARG;
+  const ScatterPlan& sp = c->scat;
+  if (c->graph_kind != FEDD_GRAPH_NS || sp.ncolours == 0)
+    return fail(c, FEDD_ERR_STATE, "no coloured scatter schedule (set the method before building the NS graph)");
+  if (block < 2) {
+    for (size_t q = 0; q < sp.h_velem.size(); ++q) {
+      const int64_t e = sp.h_velem[q], i = sp.h_vnode[q];
+      const uint8_t* mt = sp.h_vmeta.data() + 16 * q;
+      const int li = mt[0];
+      if (block == 0)
+        for (int cpos = 0; cpos < 10; ++cpos) idx[100 * e + 10 * li + kPermP2[li][cpos]] = sp.a_ptr[i] + mt[2 + cpos];
+      else
+        for (int cpos = 0; cpos < 4; ++cpos) idx[40 * e + 4 * li + kPermP2[li][cpos]] = sp.bt_ptr[i] + mt[12 + cpos];
+    }
+  } else {
+    for (size_t q = 0; q < sp.h_pelem.size(); ++q) {
+      const int64_t e = sp.h_pelem[q], pn = sp.h_pnode[q];
+      const uint8_t* mt = sp.h_pmeta.data() + 16 * q;
+      const int li = mt[0];
+      for (int cpos = 0; cpos < 10; ++cpos) idx[40 * e + 10 * li + kPermP2[li][cpos]] = sp.b_ptr[pn] + mt[2 + cpos];
+    }
+  }
   return FEDD_OK;
 }
 
@@ -840,35 +973,7 @@ int fedd_colour_build(fedd_ctx* c, int* ncolours) {
   if (!c) return FEDD_ERR_ARG;
   Mesh& m = c->mesh[0];
   if (!m.valid()) return fail(c, FEDD_ERR_STATE, "no primary mesh");
-  // Greedy first-fit colouring in ascending element order: an element takes the
-  // smallest colour not used by any element sharing a node with it.
-  Incidence inc;
-  build_incidence(m, inc);
-  std::vector<int32_t> col(m.nelem, -1);
-  std::vector<int64_t> used;  // used[colour] == e marks colour as taken for element e
-  int nc = 0;
-  for (int64_t e = 0; e < m.nelem; ++e) {
-    for (int k = 0; k < m.npe; ++k) {
-      int32_t n = m.elems[e * m.npe + k];
-      for (int64_t q = inc.ptr[n]; q < inc.ptr[n + 1]; ++q) {
-        int32_t f = inc.elem[q];
-        if (f < e && col[f] >= 0) {
-          if ((int)used.size() <= col[f]) used.resize(col[f] + 1, -1);
-          used[col[f]] = e;
-        }
-      }
-    }
-    int k = 0;
-    while (k < (int)used.size() && used[k] == e) ++k;
-    col[e] = k;
-    nc = std::max(nc, k + 1);
-  }
-  c->colour_ptr.assign(nc + 1, 0);
-  for (int64_t e = 0; e < m.nelem; ++e) c->colour_ptr[col[e] + 1]++;
-  for (int k = 0; k < nc; ++k) c->colour_ptr[k + 1] += c->colour_ptr[k];
-  c->colour_perm.resize(m.nelem);
-  std::vector<int64_t> fill(c->colour_ptr.begin(), c->colour_ptr.end() - 1);
-  for (int64_t e = 0; e < m.nelem; ++e) c->colour_perm[fill[col[e]]++] = (int32_t)e;
+  const int nc = colour_elements(m, c->colour_ptr, c->colour_perm);
   if (ncolours) *ncolours = nc;
   return FEDD_OK;
 }

@@ -92,6 +92,22 @@ struct GatherPlan {
   DevBuf d_pair_meta;           // uint4 [npairs]: loc, node-in-chunk, 14 slots
 };
 
+// Coloured element scatter of the Navier-Stokes matrices (fedd_set_assembly_method):
+// the (element, local node) pairs of every element colour, velocity pairs of a colour
+// vertex class (local node < 4) first.  The 16-byte meta is the gather plans' (loc, -,
+// 10 column slots in the row, 4 pressure slots); the row is named by its node.
+struct ScatterPlan {
+  int ncolours = 0;
+  std::vector<int64_t> vq, vsplit;  // velocity pairs of colour k: [vq[k], vq[k+1]); edge class from vsplit[k]
+  std::vector<int64_t> pq;          // pressure pairs of colour k: [pq[k], pq[k+1])
+  DevBuf d_velem, d_vnode, d_vmeta;
+  DevBuf d_pelem, d_pnode, d_pmeta;
+  // host copies for fedd_scatter_indices: pairs and the node-level row pointers
+  std::vector<int32_t> h_velem, h_vnode, h_pelem, h_pnode;
+  std::vector<uint8_t> h_vmeta, h_pmeta;
+  std::vector<int64_t> a_ptr, bt_ptr, b_ptr;
+};
+
 enum GraphKind { GK_NONE = -1 };
 
 // Navier-Stokes velocity row gather (k_ns_vel_rows): LDS budget of a chunk's rows in
@@ -128,6 +144,8 @@ struct Ctx {
   DevBuf d_nidx;                  // int32 node-level column indices of A (node blocks)
   GatherPlan vplan, pplan;        // velocity (primary) and pressure row gathers
   bool reproducible = false;      // fedd_set_reproducible: segmented plans at the next graph build
+  bool colour_scatter = false;    // fedd_set_assembly_method: NS matrices by the coloured scatter
+  ScatterPlan scat;               // its schedule (built with the NS graph)
   DevBuf d_emod;                  // per-element Young's modulus field (E-field elasticity overload)
   DevBuf d_qp;                    // elasticity quadrature-point records (52 doubles each)
   DevBuf d_rhsq;                  // assemblyRHS: quadrature points / source values at them

@@ -608,6 +608,79 @@ __global__ __launch_bounds__(256) void k_ns_pres_rows(NsArgs a) {
   for (int k = threadIdx.x; k < len; k += blockDim.x) __builtin_nontemporal_store(lds[k], out + k);
 }
 
+// Coloured element scatter (FEDD_METHOD_COLOUR_SCATTER): one lane per (element, local
+// node) pair of one colour.  The lane forms the pair's canonical row exactly as the row
+// gather does and adds it straight into the global values with no-return FP64 atomics.
+// Elements of one colour share no node, so no two lanes of a launch add into the same
+// value, and a value receives its contributions colour by colour in launch order: the
+// sums repeat bit for bit.  Velocity pairs [q0, qs) are vertex class, [qs, q1) edge
+// class; the vertex class is padded to whole waves so that each wave runs one row kernel.
+template <typename T>
+__global__ __launch_bounds__(256) void k_ns_vel_scatter(NsArgs a, const int32_t* __restrict__ elem,
+                                                        const int32_t* __restrict__ node,
+                                                        const uint4* __restrict__ meta, int64_t q0, int64_t qs,
+                                                        int64_t q1) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t nv = qs - q0, nvpad = (nv + 63) & ~(int64_t)63;
+  const bool vert = t < nvpad;  // uniform across the wave
+  if (vert && t >= nv) return;
+  const int64_t q = vert ? q0 + t : qs + (t - nvpad);
+  if (q >= q1) return;
+  const int32_t e = elem[q];
+  const int64_t g = node[q];
+  const uint4 mv = meta[q];
+  const uint8_t* m = reinterpret_cast<const uint8_t*>(&mv);
+  const int64_t k0 = a.nptr[g];
+  double* rowA = a.valsA + 9 * k0;
+  const int deg = (int)(a.nptr[g + 1] - k0);
+  double* rowBT = nullptr;
+  int pdeg = 0;
+  if (a.with_bt) {
+    const int64_t b0 = a.pptr[g];
+    rowBT = a.valsBT + 3 * b0;
+    pdeg = (int)(a.pptr[g + 1] - b0);
+  }
+  if (vert)
+    VelRow<T, 0, 0>::run(a, e, m, rowA, deg, rowBT, pdeg, nullptr);
+  else
+    VelRow<T, 1, 0>::run(a, e, m, rowA, deg, rowBT, pdeg, nullptr);
+}
+
+// Pressure rows of B by the coloured scatter: one lane per (element, pressure vertex)
+// pair of one colour, the row of k_ns_pres_rows added into the global values.
+template <typename T>
+__global__ __launch_bounds__(256) void k_ns_pres_scatter(NsArgs a, const int32_t* __restrict__ elem,
+                                                         const int32_t* __restrict__ node,
+                                                         const uint4* __restrict__ meta, int64_t q0, int64_t q1) {
+  const int64_t q = q0 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q >= q1) return;
+  const int32_t e = elem[q];
+  const uint4 mv = meta[q];
+  const uint8_t* m = reinterpret_cast<const uint8_t*>(&mv);
+  const int li = m[0];
+  int32_t nodes[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) nodes[c] = a.conn[(int64_t)c_perm2[li][c] * a.nelem + e];
+  Geo g;
+  geometry(a.xyz, a.npts, nodes, g);
+  double* row = a.valsB + 3 * a.bptr[node[q]];
+#pragma unroll
+  for (int c = 0; c < 10; ++c)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      T v = 0;
+#pragma unroll
+      for (int ax = 0; ax < 3; ++ax)
+        if (ft::RP[ax][c] != 0.0) v += (T)g.Bi[ax][d] * (T)ft::RP[ax][c];
+      atomicAdd(row + 3 * m[2 + c] + d, -(double)v * g.adet);
+    }
+}
+
+__global__ void k_zero_f64(int64_t n, double* __restrict__ p) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
 // Residual (assembleRHS, :200-224): velocity rows r_(i,d) = c00 sum_j S_ij u_jd
 // - c01 sum_m B_m,(i,d) p_m ; pressure rows r_m = -c10 sum_(j,d) B_m,(j,d) u_jd.
 template <typename T, int CLS>
@@ -1106,9 +1179,69 @@ static NsArgs ns_args(Ctx* c, const fedd_ns_params* p, const GatherPlan& pl) {
   return a;
 }
 
+// The NS residual (both rows, FP64 in both precision modes).
+static int launch_ns_residual(Ctx* c, const fedd_ns_params* p) {
+  NsArgs a = ns_args(c, p, c->vplan);
+  ktimer_begin(c);
+  k_ns_res_vel<double><<<(unsigned)c->vplan.nchunks, 256, 3 * 256 * 8, c->stream>>>(a);
+  FEDD_HIP(c, hipGetLastError());
+  NsArgs b = ns_args(c, p, c->pplan);
+  k_ns_res_pres<double><<<(unsigned)c->pplan.nchunks, 256, 256 * 8, c->stream>>>(b);
+  FEDD_HIP(c, hipGetLastError());
+  ktimer_end(c, "ns_residual");
+  return FEDD_OK;
+}
+
+// The matrices by the coloured scatter: zero the values, then one launch per colour
+// for the velocity rows (A, B^T) and, unless reAssemble, one per colour for B.
+static int launch_ns_scatter(Ctx* c, const fedd_ns_params* q, int reassemble, bool f32) {
+  const ScatterPlan& sp = c->scat;
+  NsArgs a = ns_args(c, q, c->vplan);
+  a.with_bt = reassemble ? 0 : 1;
+  ktimer_begin(c);
+  const int nb = reassemble ? 1 : 3;
+  for (int k = 0; k < nb; ++k) {
+    const int64_t n = c->blk[k].nnz;
+    if (n > 0) k_zero_f64<<<nblk(n, 256), 256, 0, c->stream>>>(n, c->blk[k].d_vals.as<double>());
+  }
+  const int32_t* ve = sp.d_velem.as<int32_t>();
+  const int32_t* vn = sp.d_vnode.as<int32_t>();
+  const uint4* vm = sp.d_vmeta.as<uint4>();
+  for (int k = 0; k < sp.ncolours; ++k) {
+    const int64_t q0 = sp.vq[k], qs = sp.vsplit[k], q1 = sp.vq[k + 1];
+    const int64_t nt = ((qs - q0 + 63) & ~(int64_t)63) + (q1 - qs);
+    if (nt == 0) continue;
+    if (f32)
+      k_ns_vel_scatter<float><<<nblk(nt, 256), 256, 0, c->stream>>>(a, ve, vn, vm, q0, qs, q1);
+    else
+      k_ns_vel_scatter<double><<<nblk(nt, 256), 256, 0, c->stream>>>(a, ve, vn, vm, q0, qs, q1);
+  }
+  if (!reassemble)
+    for (int k = 0; k < sp.ncolours; ++k) {
+      const int64_t q0 = sp.pq[k], q1 = sp.pq[k + 1];
+      if (q1 == q0) continue;
+      if (f32)
+        k_ns_pres_scatter<float><<<nblk(q1 - q0, 256), 256, 0, c->stream>>>(
+            a, sp.d_pelem.as<int32_t>(), sp.d_pnode.as<int32_t>(), sp.d_pmeta.as<uint4>(), q0, q1);
+      else
+        k_ns_pres_scatter<double><<<nblk(q1 - q0, 256), 256, 0, c->stream>>>(
+            a, sp.d_pelem.as<int32_t>(), sp.d_pnode.as<int32_t>(), sp.d_pmeta.as<uint4>(), q0, q1);
+    }
+  FEDD_HIP(c, hipGetLastError());
+  ktimer_end(c, "ns_scatter");
+  return FEDD_OK;
+}
+
 int launch_ns_assembly(Ctx* c, const fedd_ns_params* p, int mode, int reassemble, int precision) {
   const bool f32 = precision == FEDD_FP32_LOCAL;
   const bool fused = mode == FEDD_NS_JACOBIAN_RHS || mode == FEDD_NS_FIXEDPOINT_RHS;
+  if (mode != FEDD_NS_RHS && c->colour_scatter && c->scat.ncolours > 0) {
+    fedd_ns_params q = *p;
+    if (mode == FEDD_NS_FIXEDPOINT || mode == FEDD_NS_FIXEDPOINT_RHS) q.newton = 0;  // as below
+    int rc = launch_ns_scatter(c, &q, reassemble, f32);
+    if (rc == FEDD_OK && fused) rc = launch_ns_residual(c, p);
+    return rc;
+  }
   if (mode != FEDD_NS_RHS) {
     fedd_ns_params q = *p;
     // the fixed-point matrix never holds W (:115-146)
@@ -1167,14 +1300,7 @@ int launch_ns_assembly(Ctx* c, const fedd_ns_params* p, int mode, int reassemble
   } else {
     // The Newton residual stays FP64 in both precision modes (FP32-local applies to
     // the element matrices only).
-    NsArgs a = ns_args(c, p, c->vplan);
-    ktimer_begin(c);
-    k_ns_res_vel<double><<<(unsigned)c->vplan.nchunks, 256, 3 * 256 * 8, c->stream>>>(a);
-    FEDD_HIP(c, hipGetLastError());
-    NsArgs b = ns_args(c, p, c->pplan);
-    k_ns_res_pres<double><<<(unsigned)c->pplan.nchunks, 256, 256 * 8, c->stream>>>(b);
-    FEDD_HIP(c, hipGetLastError());
-    ktimer_end(c, "ns_residual");
+    return launch_ns_residual(c, p);
   }
   return FEDD_OK;
 }

@@ -20,6 +20,7 @@ MAX_SURFACE_FLAGS = 16
 FP64, FP32_LOCAL = 0, 1
 BC_RHS, BC_MINUS_VECTOR, BC_VECTOR_MINUS_BC = 0, 1, 2
 NS_JACOBIAN, NS_FIXEDPOINT, NS_RHS, NS_JACOBIAN_RHS, NS_FIXEDPOINT_RHS = 0, 1, 2, 3, 4
+METHOD_ROW_GATHER, METHOD_COLOUR_SCATTER = 0, 1
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
@@ -159,6 +160,8 @@ _SIGS = {
                                           ctypes.POINTER(ctypes.c_float)]),
     "fedd_kernel_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fedd_set_reproducible": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "fedd_set_assembly_method": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "fedd_scatter_indices": (ctypes.c_int, [_vp, ctypes.c_int, _lp]),
     "fedd_kernel_time": (ctypes.c_int, [_vp, ctypes.c_char_p, _dp, _lp]),
 }
 
@@ -301,6 +304,16 @@ class Context:
     def set_reproducible(self, on=True):
         """Bitwise-repeatable row-gather sums for graphs built afterwards (see the header)."""
         self._chk(self._L.fedd_set_reproducible(self.h, 1 if on else 0))
+
+    def set_assembly_method(self, method):
+        """METHOD_ROW_GATHER or METHOD_COLOUR_SCATTER for NS graphs built afterwards."""
+        self._chk(self._L.fedd_set_assembly_method(self.h, method))
+
+    def scatter_indices(self, block, nelem):
+        """Node-level CRS positions of every element block (see fedd_scatter_indices)."""
+        out = np.zeros(nelem * (100 if block == 0 else 40), np.int64)
+        self._chk(self._L.fedd_scatter_indices(self.h, block, _p(out, _lp)))
+        return out
 
     def graph_build(self, kind):
         self._chk(self._L.fedd_graph_build(self.h, kind))

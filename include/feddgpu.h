@@ -145,6 +145,37 @@ int fedd_graph_build(fedd_ctx* ctx, int kind);
  * take about 15% longer (40^3 x 6).  The vector-graph (elasticity) plan is not changed.
  * Applies to graphs built after the call. */
 int fedd_set_reproducible(fedd_ctx* ctx, int on);
+/* How the Navier-Stokes matrices are assembled.  The reference inserts each element's
+ * matrix into the Tpetra CrsMatrix (FE::assemblyNavierStokes element loop,
+ * FE_def.hpp:1153-1176; addFeBlockMatrix :982-1062, insertGlobalValues adding into
+ * existing entries); both methods fill the same graph with values equal to the parity
+ * rules.
+ *   FEDD_METHOD_ROW_GATHER (default): owner-computes row gathers, every value written
+ *     once from LDS.
+ *   FEDD_METHOD_COLOUR_SCATTER: the elements are coloured (fedd_colour_build's greedy
+ *     rule; elements of one colour share no node) and, colour after colour, each
+ *     (element, local node) pair adds its row into the global values with no-return FP64
+ *     atomics that never meet inside a colour.  Every value receives its contributions in
+ *     colour order, so repeated assemblies agree bit for bit.  The values are zeroed first;
+ *     each contribution is a read-modify-write in the L2, so this method moves several
+ *     times the row gather's bytes.  The fused residual modes assemble the matrices this
+ *     way and the residual as FEDD_NS_RHS does.
+ * Applies to NS graphs built after the call (the scatter schedule is built with them). */
+enum {
+  FEDD_METHOD_ROW_GATHER = 0,
+  FEDD_METHOD_COLOUR_SCATTER = 1
+};
+int fedd_set_assembly_method(fedd_ctx* ctx, int method);
+/* Scatter indices of the coloured scatter schedule (NS graph built with
+ * FEDD_METHOD_COLOUR_SCATTER; else FEDD_ERR_STATE): for every element e, the node-level
+ * CRS position of each of its element-matrix blocks, local node order of the element.
+ *   block 0 (A):   idx[100 e + 10 a + b] = k with node-block column nidx[k] = node b of e
+ *                  in node row a (entry (3 i + d1, 3 j + d2) is value 9 k0 + 3 deg d1 +
+ *                  3 (k - k0) + d2 of block A, k0 the row's first block, deg its blocks);
+ *   block 1 (B^T): idx[40 e + 4 a + m], velocity node a, pressure vertex m;
+ *   block 2 (B):   idx[40 e + 10 m + b], pressure vertex m, velocity node b.
+ * idx holds 100 nelem (block 0) or 40 nelem (blocks 1, 2) entries. */
+int fedd_scatter_indices(fedd_ctx* ctx, int block, int64_t* idx);
 int fedd_block_info(fedd_ctx* ctx, int block, int64_t* nrows, int64_t* ncols, int64_t* nnz);
 /* Host copies: rowptr has nrows+1 entries, colind nnz (local = global ids on one rank). */
 int fedd_block_graph(fedd_ctx* ctx, int block, int64_t* rowptr, int32_t* colind);
@@ -175,11 +206,11 @@ int fedd_hdf5_read_vector(const char* path, const char* name, double* out, int64
 
 /* ------------------------------------------------------------- colouring */
 /* Element colouring of the primary mesh: elements of one colour share no node.
- * No counterpart in the reference (it assembles serially per rank).  No kernel of this
- * library uses it: the row gathers write every CRS value once without colouring.  It is
- * exported for callers that scatter element-parallel themselves (e.g. a threaded CPU
- * assembly).  colour_ptr has ncolours+1 entries, elem_perm nelem entries (elements
- * sorted by colour, ascending id). */
+ * No counterpart in the reference (it assembles serially per rank).  The coloured
+ * scatter (FEDD_METHOD_COLOUR_SCATTER) runs over it; it is also exported for callers
+ * that scatter element-parallel themselves (e.g. a threaded CPU assembly).  colour_ptr
+ * has ncolours+1 entries, elem_perm nelem entries (elements sorted by colour, ascending
+ * id). */
 int fedd_colour_build(fedd_ctx* ctx, int* ncolours);
 int fedd_colour_export(fedd_ctx* ctx, int64_t* colour_ptr, int32_t* elem_perm);
 
