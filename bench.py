@@ -96,6 +96,9 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the STREAM copy kernel (1 GiB read + 1 GiB write, 20 times): the "
                          "attainable bandwidth and the calibration of rocprofv3 byte counters")
+    ap.add_argument("--tangent-probe", action="store_true",
+                    help="only run the learned-tangent assembly at 24^3x6 five times (surrogate inference "
+                         "k_mlp_tangent on 414,720 points, no multigrid graphs): for rocprofv3 counter passes")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: there is no multi-GPU path (the elements would be "
@@ -111,6 +114,10 @@ def main():
             ms = ctx.stream_copy(COPY_BYTES, 20)
         print(json.dumps({"probe": "stream_copy", "bytes_read": COPY_BYTES, "bytes_written": COPY_BYTES,
                           "launches": 21, "ms_per_copy": ms, "GBs": 2 * COPY_BYTES / (ms * 1e-3) / 1e9}))
+        return
+
+    if args.tangent_probe:
+        print(json.dumps(tangent_probe(N)))
         return
 
     M = args.M
@@ -302,6 +309,32 @@ def fp32_error_figures(v32, v64, rowptr, chunk=1 << 25):
             rw = max(rw, float(np.max(d / rm)))
         r0 = r1
     return {"row_floor": rf, "per_matrix": pm, "rowwise": rw}
+
+
+def tangent_probe(N, M=24, reps=5):
+    """The learned-tangent assembly (exact residual, surrogate tangent) on the Newton leg's
+    24^3x6 P2 cube at a smooth displacement, with a seeded random network: every launch of
+    k_mlp_tangent infers CC at all 5 points of every element.  Graph-free, so rocprofv3's
+    counter passes can serialise every launch."""
+    pts, el, _ = _nh_problem(N, M)
+    with N.Context(0) as ctx:
+        ctx.mlp_init(1, EMOD, POISSON, 0)
+        ctx.mlp_set_params(np.random.default_rng(0).normal(0.0, 0.1, N.MLP_PARAMS).astype(np.float32))
+        ctx.mesh_upload(N.MESH_PRIMARY, pts, el)
+        ctx.graph_build(N.GRAPH_VECTOR)
+        x, y, z = pts[:, 0], pts[:, 1], pts[:, 2]
+        ctx.vector_upload(N.VEC_U, 0.1 * np.stack([z * x, -0.5 * z * y, 0.3 * x * y], axis=1).ravel())
+        ctx.assembly_nonlinear_elasticity_tangent(1, EMOD, POISSON, N.FP64, tangent=1)
+        ctx.kernel_timing(True)
+        for _ in range(reps):
+            ctx.assembly_nonlinear_elasticity_tangent(1, EMOD, POISSON, N.FP64, tangent=1)
+        ctx.synchronize()
+        t, n = ctx.kernel_time("mlp_tangent")
+        ctx.kernel_timing(False)
+    nqp = 5 * el.shape[0]
+    ms = t / max(n, 1)
+    return {"probe": "mlp_tangent", "points_per_launch": nqp, "launches": n, "avg_ms": ms,
+            "tflops": 12032.0 * nqp / (ms * 1e-3) / 1e12}
 
 
 def scatter_leg(ctx, N, iterate, E, ms_gather, bytes_step, gatherA, steps, warmup):
