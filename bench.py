@@ -79,7 +79,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    # The first launches after the GPU sat idle run 10-20% slower (clocks settling): 50
+    # untimed steps (about 25 ms at 40^3) reach the steady state the timed steps measure.
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--M", type=int, default=40)
     ap.add_argument("--cpu-sample", type=int, default=50000,
                     help="elements of the CPU-baseline sample (0 disables it)")
