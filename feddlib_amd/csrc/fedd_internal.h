@@ -112,10 +112,13 @@ enum GraphKind { GK_NONE = -1 };
 
 // Navier-Stokes velocity row gather (k_ns_vel_rows): LDS budget of a chunk's rows in
 // doubles, most nodes per chunk (the size of its per-node table), threads per workgroup.
-// 36 KiB of rows + a 1 KiB table: four workgroups of three waves fit a CU's 160 KiB of
-// LDS and its 12 waves of 168 VGPRs (three waves per SIMD).
+// 32 KiB of rows + a 1 KiB table: four workgroups of three waves fit a CU's 160 KiB of
+// LDS and its 12 waves of 168 VGPRs (three waves per SIMD).  40^3 x 6 Jacobian at steady
+// clocks, ms per step with 28 / 30 / 32 / 34 / 36 / 40 KiB: 0.49 / 0.455 / 0.458 / 0.480 /
+// 0.471 / 0.53 (the chunk count against the CU count matters more than the size); the
+// 64^3 fused FP32-local iteration takes 2.87-2.90 ms with 32 KiB, 2.83-2.86 with 36.
 #ifndef FEDD_NS_LDS_BUDGET
-#define FEDD_NS_LDS_BUDGET 4608
+#define FEDD_NS_LDS_BUDGET 4096
 #endif
 #ifndef FEDD_NS_CHUNK_NODES
 #define FEDD_NS_CHUNK_NODES 64
