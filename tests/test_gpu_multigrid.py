@@ -146,7 +146,7 @@ def _relabelled(M, seed=0):
 @pytest.mark.parametrize("M,relabel", [(3, False), (6, False), (6, True)])
 def test_amg_linear_solve(gpu_ctx, M, relabel):
     """Aggregation AMG (preconditioner 2): the direct solution, on the lattice and on a
-    relabelled, jittered mesh, in far fewer iterations than block Jacobi."""
+    relabelled, jittered mesh, in under a fifth of block Jacobi's iterations."""
     pts, el, dofs = _relabelled(M) if relabel else _problem(M)
     n = 3 * pts.shape[0]
     gpu_ctx.mesh_upload(N.MESH_PRIMARY, pts, el)
@@ -173,7 +173,10 @@ def test_amg_linear_solve(gpu_ctx, M, relabel):
     finally:
         gpu_ctx.set_preconditioner(0)
     if M >= 6:
-        assert its[2] * 3 < its[0], its
+        # the six rigid-body modes in the tentative prolongator: 51 (lattice) and 59
+        # (relabelled) iterations against block Jacobi's 368; with the translations only
+        # the aggregation hierarchy needs 95 and 102, which this bound rejects
+        assert its[2] * 5 < its[0], its
 
 
 def test_amg_newton_on_unstructured_mesh(gpu_ctx):
