@@ -74,9 +74,11 @@ struct MgLevel {
   DevBuf agg;                // int32, per F node: its aggregate (nodes kSaNodes * J + q here)
   DevBuf tent;               // per F node, its 3 x kSaModes rows of the tentative prolongator
   DevBuf pptr, pidx, pval;   // smoothed prolongator, rows = F nodes, 3x3 blocks (row-major)
+  DevBuf pval32, ptval32;    // FP32 copies for the V-cycle: P in pval's order, and in the transpose lists' order
   DevBuf tptr, tidx, tpos;   // its transpose: per node of this level, (F node, block index in pval)
   DevBuf aptr, aidx, aval;   // A_F P, rows = F nodes
   int64_t maxdeg_ap = 0;
+  int64_t nblk_p = 0;  // blocks of P (aggregation levels)
   std::vector<int64_t> h_ptr;
   std::vector<int32_t> h_idx;
   std::vector<int32_t> h_agg;
@@ -463,10 +465,29 @@ __global__ __launch_bounds__(64) void k_sa_ptap(int64_t nc, const int64_t* __res
   }
 }
 
-// b_c = P^T r: one wave per coarse node over its transpose list, shuffle reduction.
+// FP32 copies of P for the V-cycle's transfers, one pass over the transpose lists: the
+// prolongation reads P in its fine-row order, the restriction reads the same rounded
+// blocks in coarse-row order (contiguous per coarse node), so restriction stays exactly
+// the transpose of prolongation and the V-cycle stays symmetric.
+__global__ __launch_bounds__(256) void k_sa_p32(int64_t nq, const int64_t* __restrict__ tpos,
+                                                const double* __restrict__ pval, float* __restrict__ pval32,
+                                                float* __restrict__ ptval32) {
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  const int64_t k = tpos[q];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) {
+    const float v = (float)pval[9 * k + e];
+    pval32[9 * k + e] = v;
+    ptval32[9 * q + e] = v;
+  }
+}
+
+// b_c = P^T r: one wave per coarse node over its transpose list (its blocks contiguous
+// in ptval32), shuffle reduction.
 __global__ __launch_bounds__(256) void k_sa_restrict(int64_t nc, const int64_t* __restrict__ tptr,
                                                      const int32_t* __restrict__ tidx,
-                                                     const int64_t* __restrict__ tpos, const double* __restrict__ pval,
+                                                     const float* __restrict__ ptval32,
                                                      const double* __restrict__ rf, const uint8_t* __restrict__ cfree,
                                                      double* __restrict__ bc) {
   const int64_t I = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
@@ -475,10 +496,10 @@ __global__ __launch_bounds__(256) void k_sa_restrict(int64_t nc, const int64_t* 
   if (I < nc)
     for (int64_t q = tptr[I] + lane; q < tptr[I + 1]; q += 64) {
       const int64_t i = tidx[q];
-      const double* P = pval + 9 * tpos[q];
+      const float* P = ptval32 + 9 * q;
       const double r0 = rf[3 * i], r1 = rf[3 * i + 1], r2 = rf[3 * i + 2];
 #pragma unroll
-      for (int d = 0; d < 3; ++d) acc[d] += P[d] * r0 + P[3 + d] * r1 + P[6 + d] * r2;
+      for (int d = 0; d < 3; ++d) acc[d] += (double)P[d] * r0 + (double)P[3 + d] * r1 + (double)P[6 + d] * r2;
     }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1)
@@ -492,7 +513,7 @@ __global__ __launch_bounds__(256) void k_sa_restrict(int64_t nc, const int64_t* 
 // x_f += P x_c (rows of constrained fine dofs are zero).
 template <int G>  // lanes per fine row
 __global__ __launch_bounds__(256) void k_sa_prolong(int64_t nf, const int64_t* __restrict__ pptr,
-                                                    const int32_t* __restrict__ pidx, const double* __restrict__ pval,
+                                                    const int32_t* __restrict__ pidx, const float* __restrict__ pval,
                                                     const double* __restrict__ xc, double* __restrict__ xf) {
   const int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G;
   const int lane = threadIdx.x % G;
@@ -500,10 +521,10 @@ __global__ __launch_bounds__(256) void k_sa_prolong(int64_t nf, const int64_t* _
   if (i < nf) {  // the row's blocks are contiguous: the G lanes stream its 9 np values
     const int64_t k0 = pptr[i];
     const int L = 9 * (int)(pptr[i + 1] - k0);
-    const double* P = pval + 9 * k0;
+    const float* P = pval + 9 * k0;
     for (int q = lane; q < L; q += G) {
       const int k = q / 9, r = q - 9 * k, d1 = r / 3, d2 = r - 3 * d1;
-      const double v = P[q] * xc[3 * (int64_t)pidx[k0 + k] + d2];
+      const double v = (double)P[q] * xc[3 * (int64_t)pidx[k0 + k] + d2];
       s[0] += d1 == 0 ? v : 0.0;
       s[1] += d1 == 1 ? v : 0.0;
       s[2] += d1 == 2 ? v : 0.0;
@@ -908,11 +929,14 @@ static int level_device_setup(Ctx* c, MgLevel& L, size_t l, int mode, const std:
     L.d_nidx = L.nidx.as<int32_t>();
     L.d_vals = L.vals.as<double>();
     if (mode == 1) {
+      L.nblk_p = (int64_t)L.h_pidx.size();
       if ((rc = upload(c, L.agg, L.h_agg.data(), L.h_agg.size() * 4)) ||
           (rc = upload(c, L.tent, L.h_tent.data(), L.h_tent.size() * 8)) ||
           (rc = upload(c, L.pptr, L.h_pptr.data(), L.h_pptr.size() * 8)) ||
           (rc = upload(c, L.pidx, L.h_pidx.data(), std::max<size_t>(L.h_pidx.size(), 1) * 4)) ||
           (rc = alloc(c, L.pval, std::max<size_t>(L.h_pidx.size(), 1) * 9 * 8)) ||
+          (rc = alloc(c, L.pval32, std::max<size_t>(L.h_pidx.size(), 1) * 9 * 4)) ||
+          (rc = alloc(c, L.ptval32, std::max<size_t>(L.h_pidx.size(), 1) * 9 * 4)) ||
           (rc = upload(c, L.tptr, L.h_tptr.data(), L.h_tptr.size() * 8)) ||
           (rc = upload(c, L.tidx, L.h_tidx.data(), std::max<size_t>(L.h_tidx.size(), 1) * 4)) ||
           (rc = upload(c, L.tpos, L.h_tpos.data(), std::max<size_t>(L.h_tpos.size(), 1) * 8)) ||
@@ -945,6 +969,9 @@ static int sa_galerkin(Ctx* c, MgHost* g, size_t l) {
                                                       F.free.as<uint8_t>(), L.agg.as<int32_t>(), L.tent.as<double>(),
                                                       L.pptr.as<int64_t>(),
                                                       L.pidx.as<int32_t>(), L.pval.as<double>(), FEDD_SA_OMEGA);
+  if (L.nblk_p > 0)
+    k_sa_p32<<<nbk(L.nblk_p, 256), 256, 0, c->stream>>>(L.nblk_p, L.tpos.as<int64_t>(), L.pval.as<double>(),
+                                                        L.pval32.as<float>(), L.ptval32.as<float>());
   const unsigned gf = (unsigned)std::min<int64_t>(F.nn, (int64_t)c->num_cu * 16);
   k_sa_ap<<<gf, 64, (size_t)9 * L.maxdeg_ap * 8, c->stream>>>(F.nn, F.d_nptr, F.d_nidx, F.d_vals,
                                                                L.pptr.as<int64_t>(), L.pidx.as<int32_t>(),
@@ -1378,9 +1405,8 @@ static void vcycle(Ctx* c, size_t l, const double* b, double* x, bool first_done
   MgLevel& C = g->lv[l + 1];
   if (g->mode == 1)
     k_sa_restrict<<<nbk(C.nn * 64, 256), 256, 0, c->stream>>>(C.nn, C.tptr.as<int64_t>(), C.tidx.as<int32_t>(),
-                                                              C.tpos.as<int64_t>(), C.pval.as<double>(),
-                                                              L.r.as<double>(), C.free.as<uint8_t>(),
-                                                              C.b.as<double>());
+                                                              C.ptval32.as<float>(), L.r.as<double>(),
+                                                              C.free.as<uint8_t>(), C.b.as<double>());
   const bool fuse = g->mode != 1 && l + 2 < g->lv.size();  // the coarsest level is a dense solve
   if (g->mode != 1)
     k_mg_restrict<<<nbk(C.nn * 16, 256), 256, 0, c->stream>>>(
@@ -1389,7 +1415,7 @@ static void vcycle(Ctx* c, size_t l, const double* b, double* x, bool first_done
   vcycle(c, l + 1, C.b.as<double>(), C.x.as<double>(), fuse);
   if (g->mode == 1)
     k_sa_prolong<16><<<nbk(L.nn * 16, 256), 256, 0, c->stream>>>(L.nn, C.pptr.as<int64_t>(), C.pidx.as<int32_t>(),
-                                                                 C.pval.as<double>(), C.x.as<double>(), cur);
+                                                                 C.pval32.as<float>(), C.x.as<double>(), cur);
   else
     k_mg_prolong<<<nbk(L.nn, 256), 256, 0, c->stream>>>(L.P, C.P, C.x.as<double>(), L.free.as<uint8_t>(),
                                                         C.free.as<uint8_t>(), cur);
