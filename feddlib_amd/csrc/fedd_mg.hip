@@ -69,7 +69,7 @@ struct MgLevel {
   const int32_t* d_nidx = nullptr;
   double* d_vals = nullptr;
   DevBuf binv, free, x, x2, b, r;
-  DevBuf vals32;  // finest level: FP32 copy of the operator used by the smoother
+  DevBuf vals32;  // FP32 copy of the operator used by the smoother (finest level; wide aggregation levels)
   // aggregation hierarchy: transfer from the next finer level F to this level
   DevBuf agg;                // int32, per F node: its aggregate (nodes kSaNodes * J + q here)
   DevBuf tent;               // per F node, its 3 x kSaModes rows of the tentative prolongator
@@ -951,6 +951,8 @@ static int level_device_setup(Ctx* c, MgLevel& L, size_t l, int mode, const std:
     if ((rc = alloc(c, *b, L.nn * 3 * 8))) return rc;
   if ((rc = alloc(c, L.binv, L.nn * 9 * 8))) return rc;
   if (l == 0 && FEDD_MG_FP32_FINE && (rc = alloc(c, L.vals32, L.nblk * 9 * 4))) return rc;
+  // the wide first coarse level of the aggregation hierarchy is smoothed from an FP32 copy too
+  if (l > 0 && mode == 1 && L.nblk > 48 * L.nn && (rc = alloc(c, L.vals32, L.nblk * 9 * 4))) return rc;
   std::vector<int64_t>().swap(L.h_ptr);
   std::vector<int32_t>().swap(L.h_idx);
   std::vector<int32_t>().swap(L.h_agg);
@@ -1383,6 +1385,8 @@ static void vcycle(Ctx* c, size_t l, const double* b, double* x, bool first_done
       launch_sweep<FEDD_MG_G0>(c->stream, gs, mode, L, L.vals32.as<float>(), b, xin, out, w);
     else if (l == 0)
       launch_sweep<FEDD_MG_G0>(c->stream, gs, mode, L, (const double*)L.d_vals, b, xin, out, w);
+    else if (wide && L.vals32.p)
+      launch_sweep<64>(c->stream, gs, mode, L, L.vals32.as<float>(), b, xin, out, w);
     else if (wide)
       launch_sweep<64>(c->stream, gs, mode, L, (const double*)L.d_vals, b, xin, out, w);
     else
