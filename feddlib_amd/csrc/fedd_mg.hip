@@ -378,7 +378,8 @@ __global__ __launch_bounds__(256) void k_sa_smooth_p(int64_t nf, const int64_t* 
   }
 }
 
-// AP = A_F P: one workgroup (64 threads) per fine row, accumulated in LDS by aggregate.
+// AP = A_F P: one workgroup (64 threads) per fine row, accumulated in LDS by aggregate
+// (the row's column list is staged in LDS after the accumulators for the slot searches).
 __global__ __launch_bounds__(64) void k_sa_ap(int64_t nf, const int64_t* __restrict__ fptr,
                                               const int32_t* __restrict__ fidx, const double* __restrict__ fval,
                                               const int64_t* __restrict__ pptr, const int32_t* __restrict__ pidx,
@@ -388,8 +389,9 @@ __global__ __launch_bounds__(64) void k_sa_ap(int64_t nf, const int64_t* __restr
   for (int64_t i = blockIdx.x; i < nf; i += gridDim.x) {
     const int64_t a0 = aptr[i];
     const int na = (int)(aptr[i + 1] - a0);
-    const int32_t* arow = aidx + a0;
+    int32_t* arow = reinterpret_cast<int32_t*>(acc + 9 * na);
     for (int k = threadIdx.x; k < 9 * na; k += blockDim.x) acc[k] = 0.0;
+    for (int k = threadIdx.x; k < na; k += blockDim.x) arow[k] = aidx[a0 + k];
     __syncthreads();
     const int64_t f0 = fptr[i];
     const int degf = (int)(fptr[i + 1] - f0);
@@ -422,7 +424,8 @@ __global__ __launch_bounds__(64) void k_sa_ap(int64_t nf, const int64_t* __restr
 
 // Coarse operator P^T (A P): one workgroup per coarse row I over the fine rows i with a
 // block of P in column I (the transpose lists), written in the node-block layout;
-// constrained coarse dofs (all-zero columns of P) get identity rows.
+// constrained coarse dofs (all-zero columns of P) get identity rows.  The row's column
+// list is staged in LDS after the accumulators.
 __global__ __launch_bounds__(64) void k_sa_ptap(int64_t nc, const int64_t* __restrict__ tptr,
                                                 const int32_t* __restrict__ tidx, const int64_t* __restrict__ tpos,
                                                 const double* __restrict__ pval, const int64_t* __restrict__ aptr,
@@ -433,8 +436,9 @@ __global__ __launch_bounds__(64) void k_sa_ptap(int64_t nc, const int64_t* __res
   for (int64_t I = blockIdx.x; I < nc; I += gridDim.x) {
     const int64_t c0 = cptr[I];
     const int degc = (int)(cptr[I + 1] - c0);
-    const int32_t* crow = cidx + c0;
+    int32_t* crow = reinterpret_cast<int32_t*>(acc + 9 * degc);
     for (int k = threadIdx.x; k < 9 * degc; k += blockDim.x) acc[k] = 0.0;
+    for (int k = threadIdx.x; k < degc; k += blockDim.x) crow[k] = cidx[c0 + k];
     __syncthreads();
     for (int64_t q = tptr[I] + threadIdx.x; q < tptr[I + 1]; q += blockDim.x) {
       const int64_t i = tidx[q];
@@ -975,12 +979,12 @@ static int sa_galerkin(Ctx* c, MgHost* g, size_t l) {
     k_sa_p32<<<nbk(L.nblk_p, 256), 256, 0, c->stream>>>(L.nblk_p, L.tpos.as<int64_t>(), L.pval.as<double>(),
                                                         L.pval32.as<float>(), L.ptval32.as<float>());
   const unsigned gf = (unsigned)std::min<int64_t>(F.nn, (int64_t)c->num_cu * 16);
-  k_sa_ap<<<gf, 64, (size_t)9 * L.maxdeg_ap * 8, c->stream>>>(F.nn, F.d_nptr, F.d_nidx, F.d_vals,
+  k_sa_ap<<<gf, 64, (size_t)L.maxdeg_ap * (9 * 8 + 4), c->stream>>>(F.nn, F.d_nptr, F.d_nidx, F.d_vals,
                                                                L.pptr.as<int64_t>(), L.pidx.as<int32_t>(),
                                                                L.pval.as<double>(), L.aptr.as<int64_t>(),
                                                                L.aidx.as<int32_t>(), L.aval.as<double>());
   const unsigned gc = (unsigned)std::min<int64_t>(L.nn, (int64_t)c->num_cu * 16);
-  k_sa_ptap<<<gc, 64, (size_t)9 * L.maxdeg * 8, c->stream>>>(
+  k_sa_ptap<<<gc, 64, (size_t)L.maxdeg * (9 * 8 + 4), c->stream>>>(
       L.nn, L.tptr.as<int64_t>(), L.tidx.as<int32_t>(), L.tpos.as<int64_t>(), L.pval.as<double>(),
       L.aptr.as<int64_t>(), L.aidx.as<int32_t>(), L.aval.as<double>(), L.d_nptr, L.d_nidx, L.d_vals,
       L.free.as<uint8_t>());
@@ -1203,7 +1207,7 @@ static int build_aggregation(Ctx* c, MgHost* g, std::vector<std::vector<uint8_t>
       }
     }, C.h_ptr, C.h_idx, &C.maxdeg);
     C.nblk = C.h_ptr[nc];
-    if (9 * std::max(C.maxdeg, C.maxdeg_ap) * 8 > 65536)
+    if ((9 * 8 + 4) * std::max(C.maxdeg, C.maxdeg_ap) > 65536)
       return fail(c, FEDD_ERR_LIMIT, "aggregation: a coarse row exceeds the LDS budget");
     // tentative prolongator, the coarse level's modes and its constrained dofs
     std::vector<uint8_t> cf;
