@@ -158,3 +158,20 @@ def test_scatter_indices_need_the_schedule(gpu_ctx):
     assert e.value.code == -3
     with pytest.raises(N.FeddError):
         gpu_ctx.set_assembly_method(2)
+
+
+@pytest.mark.parametrize("curved", [False, True])
+def test_scatter_on_an_unstructured_mesh(scatter_ctx, curved):
+    """Jittered, renumbered, shuffled and reoriented elements (edge nodes off the midpoints
+    when curved): every NS mode by the coloured scatter meets the oracle, and the scatter
+    indices are the oracle graph's positions."""
+    from test_gpu_ns import _unstructured_modes, _unstructured_ns
+    p2, e2, p1, e1, u, p = _unstructured_ns(3, curved=curved)
+    _unstructured_modes(scatter_ctx, p2, e2, p1, e1, u, p)
+    scatter_ctx.graph_build(N.GRAPH_NS)  # the helper ends on a scalar (Laplace) graph
+    ref = O.ns_assembly(p2, e2, e1, p1.shape[0], u, p, NU, RHO, True, 0, False)
+    keys, _ = _node_keys(ref["A"][0], ref["A"][1], p2.shape[0], 3, 3, p2.shape[0])
+    rows = np.repeat(e2[:, :, None], 10, axis=2)
+    cols = np.repeat(e2[:, None, :], 10, axis=1)
+    exp = _expected(keys, rows.ravel(), cols.ravel(), p2.shape[0])
+    assert np.array_equal(scatter_ctx.scatter_indices(0, e2.shape[0]), exp)
