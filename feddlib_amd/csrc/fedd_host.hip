@@ -517,6 +517,15 @@ extern "C" int fedd_mesh_correct_orientation(const double* pts, int64_t npts, in
 #define FEDD_LDS_BUDGET 5120
 #endif
 static const int kLdsBudget = FEDD_LDS_BUDGET;
+// The vector (hyperelastic) plan: k_elas_rows is held to two waves per SIMD by its
+// registers (two 256-thread workgroups per CU), so its chunks can use the LDS that the
+// occupancy leaves: larger chunks, fewer of them.  32^3 x 6 assembly (FP64 / FP32-local)
+// with 40 / 48 / 56 / 64 KiB: 1.23-1.24 / 1.18 / 1.13 / 1.21 ms and 1.95 / 1.77 / 1.66 /
+// 1.70 ms.
+#ifndef FEDD_ELAS_LDS_BUDGET
+#define FEDD_ELAS_LDS_BUDGET 7168
+#endif
+static const int kElasLdsBudget = FEDD_ELAS_LDS_BUDGET;
 
 extern "C" {
 
@@ -731,7 +740,7 @@ int fedd_graph_build(fedd_ctx* c, int kind) {
   PlanSpec vs{};
   vs.main = &g.a;
   vs.cols_main = &m;
-  vs.lds_budget_doubles = kLdsBudget;
+  vs.lds_budget_doubles = kind == FEDD_GRAPH_VECTOR ? kElasLdsBudget : kLdsBudget;
   if (kind == FEDD_GRAPH_SCALAR) {
     vs.rows_per_node = 1;
     vs.comps_per_col = 1;
