@@ -761,32 +761,54 @@ __device__ __forceinline__ void res_row(const NsArgs& a, int32_t e, int li, doub
   for (int d = 0; d < 3; ++d) r[d] = g.adet * ((double)((T)a.coeff[0] * ru[d]) + (double)((T)a.coeff[1] * rp[d]));
 }
 
+// Residual rows with one wave per chunk of the velocity plan, four chunks per
+// workgroup.  The plan's chunks are sized for the matrix rows (about 130 pairs each
+// at 40^3), so a workgroup of 256 threads per chunk leaves half its lanes idle; here a
+// wave walks its chunk's vertex pairs, then its edge pairs, 64 at a time (each pass
+// class-uniform), and accumulates the chunk's 3 x (nodes) residual values in its own
+// LDS slice.
+constexpr int kResWaves = 4;
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
 template <typename T>
-__global__ __launch_bounds__(256) void k_ns_res_vel(NsArgs a) {
-  extern __shared__ double lds[];
-  const int ch = (int)xcd_chunk(blockIdx.x, gridDim.x);
+__global__ __launch_bounds__(64 * kResWaves) void k_ns_res_vel(NsArgs a) {
+  __shared__ double lds[kResWaves][3 * kNsChunkNodes];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t ch = kResWaves * xcd_chunk(blockIdx.x, gridDim.x) + w;
+  if (ch >= a.nchunks) return;  // the whole wave; no block barrier follows
+  double* acc = lds[w];
   const int n0 = a.chunk_node[ch], n1 = a.chunk_node[ch + 1];
   const int q0 = a.chunk_pair[ch], q1 = a.chunk_pair[ch + 1], qs = a.chunk_split[ch];
   const int len = 3 * (n1 - n0);
-  for (int k = threadIdx.x; k < len; k += blockDim.x) lds[k] = 0.0;
-  __syncthreads();
-  for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+  for (int k = lane; k < len; k += 64) acc[k] = 0.0;
+  wave_lds_sync();
+  for (int q = q0 + lane; q < qs; q += 64) {
     const int32_t e = a.pair_elem[q];
     if (e < 0) continue;  // padding pair
     const uint4 mv = a.pair_meta[q];
     const uint8_t* m = reinterpret_cast<const uint8_t*>(&mv);
     double r[3];
-    if (q < qs)
-      res_row<T, 0>(a, e, m[0], r);
-    else
-      res_row<T, 1>(a, e, m[0], r);
-    double* dst = lds + 3 * m[1];
-    atomicAdd(dst + 0, r[0]);
-    atomicAdd(dst + 1, r[1]);
-    atomicAdd(dst + 2, r[2]);
+    res_row<T, 0>(a, e, m[0], r);
+    atomicAdd(acc + 3 * m[1] + 0, r[0]);
+    atomicAdd(acc + 3 * m[1] + 1, r[1]);
+    atomicAdd(acc + 3 * m[1] + 2, r[2]);
   }
-  __syncthreads();
-  for (int k = threadIdx.x; k < len; k += blockDim.x) a.res_u[3 * (int64_t)n0 + k] = lds[k];
+  for (int q = qs + lane; q < q1; q += 64) {
+    const int32_t e = a.pair_elem[q];
+    if (e < 0) continue;
+    const uint4 mv = a.pair_meta[q];
+    const uint8_t* m = reinterpret_cast<const uint8_t*>(&mv);
+    double r[3];
+    res_row<T, 1>(a, e, m[0], r);
+    atomicAdd(acc + 3 * m[1] + 0, r[0]);
+    atomicAdd(acc + 3 * m[1] + 1, r[1]);
+    atomicAdd(acc + 3 * m[1] + 2, r[2]);
+  }
+  wave_lds_sync();
+  for (int k = lane; k < len; k += 64) a.res_u[3 * (int64_t)n0 + k] = acc[k];
 }
 
 template <typename T>
@@ -1204,7 +1226,8 @@ static NsArgs ns_args(Ctx* c, const fedd_ns_params* p, const GatherPlan& pl) {
 static int launch_ns_residual(Ctx* c, const fedd_ns_params* p) {
   NsArgs a = ns_args(c, p, c->vplan);
   ktimer_begin(c);
-  k_ns_res_vel<double><<<(unsigned)c->vplan.nchunks, 256, 3 * 256 * 8, c->stream>>>(a);
+  k_ns_res_vel<double><<<(unsigned)((c->vplan.nchunks + kResWaves - 1) / kResWaves), 64 * kResWaves, 0,
+                          c->stream>>>(a);
   FEDD_HIP(c, hipGetLastError());
   NsArgs b = ns_args(c, p, c->pplan);
   k_ns_res_pres<double><<<(unsigned)c->pplan.nchunks, 256, 256 * 8, c->stream>>>(b);
